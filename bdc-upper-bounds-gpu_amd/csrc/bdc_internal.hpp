@@ -1,0 +1,152 @@
+// Internal declarations of the MI355X Blahut–Arimoto engine (gfx950 only).
+//
+// Problem:   BDC_{n,k}, inputs x in {0,1}^n split as x = (x1, x2) with x1 the
+//            high n1 = ceil(n/2) bits and x2 the low n2 = n - n1 bits
+//            (backend/BAA.cu:340-349).  P is p laid out as an N1 x N2 row-major
+//            FP64 matrix (N1 = 2^n1 rows x1, N2 = 2^n2 columns x2).
+// Counts:    A_j = cache_{n1,j} (N1 x 2^j), B_m = cache_{n2,m} (N2 x 2^m), and
+//            c(x,y) = sum_j A_j[x1, y>>(k-j)] * B_{k-j}[x2, y mod 2^(k-j)]
+//            (backend/BAA.cu:355-392).
+// Splits j are partitioned into JL = {j : 2j <= k} (contract x1 first) and
+// JR = {j : 2j > k} (contract x2 first); stacked count operands
+//   Abar = [A_j, j in JL]          (N1 x KL)
+//   Bbar = [B_{k-j}, j in JR]      (N2 x KR)
+// make the two large contractions of one BA step
+//   G1:  Tbar = Abar^T P  (KL x N2) and Ubar = Bbar^T P^T (KR x N1)
+//   G2:  S    = [Abar | Wbar] [Vbar ; Bbar^T]   (N1 x N2, K = KL + KR)
+// with the small stage GEMMs q_j = T_j B_{k-j} (j in JL), q_j = A_j^T U_{k-j}
+// (j in JR), V_j = L_j B_{k-j}^T (j in JL), W_j = A_j L_j (j in JR), where
+// L = ln q - mu is the centred log output law.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/bdc.h"
+
+namespace bdc {
+
+constexpr int kMaxN = 32;       // backend/config.hpp:24 (MAX_N = 33 -> n <= 32)
+constexpr int kMaxHalf = 16;    // half-length caches up to n' = 16
+constexpr int kPartStride = 4;  // doubles per block partial in the fused epilogue
+
+void set_error(const char *fmt, ...);
+int fail(int code, const char *fmt, ...);
+
+#define BDC_HIP(call)                                                                   \
+  do {                                                                                  \
+    hipError_t e_ = (call);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      return ::bdc::fail(BDC_ERR_HIP, "%s:%d %s: %s", __FILE__, __LINE__, #call,        \
+                         hipGetErrorString(e_));                                        \
+  } while (0)
+
+#define BDC_TRY(expr)               \
+  do {                              \
+    int r_ = (expr);                \
+    if (r_ != BDC_OK) return r_;    \
+  } while (0)
+
+// Device buffer owned by a context (hipMalloc / hipFree).
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  int alloc(size_t b);
+  void release();
+  template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+// One small stage GEMM C[M x N] = sum_K A[m,k] B[k,n]; element (r,c) of an
+// operand lives at ptr[r*rs + c*cs].  Type codes: 0 = f64, 1 = u16 counts.
+struct StageGemm {
+  const void *a;
+  int64_t ars, acs;
+  int atype;
+  const void *b;
+  int64_t brs, bcs;
+  int btype;
+  double *c;
+  int64_t crs, ccs;
+  int M, N, K;
+};
+
+struct Timer {
+  const char *name;
+  std::vector<hipEvent_t> start, stop;
+  int used = 0;
+};
+
+struct Split {  // split-FP16 configuration, fixed at context creation
+  int g1_limbs = 4, g1_bits = 9, g1_chunk = 128;
+  int g2_limbs = 4, g2_bits = 11;
+  bool ok = false;
+};
+
+}  // namespace bdc
+
+struct bdc_ctx {
+  int n, k, n1, n2;
+  int N1, N2;             // 2^n1, 2^n2
+  uint64_t nx;            // 2^n
+  uint64_t norb;          // number of orbits
+  double nck, inv_c, log_c;
+  hipStream_t stream = nullptr;
+  int device = 0;
+
+  // split sets
+  std::vector<int> JL, JR;      // j values (ascending)
+  std::vector<int> offL, offR;  // column offsets of A_j (j in JL) in Abar, of B_{k-j} (j in JR) in Bbar
+  int KL = 0, KR = 0, KLp = 0, KRp = 0, KT = 0;
+
+  // counts (uint16) per half length: cnt1[j] = cache_{n1,j} (j < n1), cnt2[m] = cache_{n2,m}
+  bdc::DevBuf cnt1[bdc::kMaxHalf + 1], cnt2[bdc::kMaxHalf + 1];
+  bdc::DevBuf abar16;   // Abar as u16, N1 x KLp (zero padded)
+  bdc::DevBuf bbar16;   // Bbar as u16, N2 x KRp
+  bdc::DevBuf abarh;    // Abar as fp16 (exact counts or rounded; residual list below), N1 x KLp
+  bdc::DevBuf bbarth;   // Bbar^T as fp16, KRp x N2
+
+  // symmetry / orbits
+  bdc::DevBuf repmask;   // 2^n bits (uint64 words), bit x set iff rep(x) == x
+  bdc::DevBuf wordbase;  // exclusive prefix popcount per 64-bit word (uint32 for n <= 30, uint64 beyond)
+  bdc::DevBuf orbits;    // ascending representatives (uint32)
+
+  // per-input state (full space, N1 x N2 row-major)
+  bdc::DevBuf P;   // p-tilde (unnormalised), FP64; p = pscale * P
+  bdc::DevBuf Hc;  // H(x)/C - ln C, FP64
+  bool have_p = false;
+  bool have_h = false;
+
+  // contraction buffers
+  bdc::DevBuf T;      // Tbar (KLp x N2) FP64
+  bdc::DevBuf U;      // Ubar (KRp x N1) FP64 (general path)
+  bdc::DevBuf qpart;  // per-j output-law slices (|J| x 2^k) FP64
+  bdc::DevBuf qv;     // q (2^k) FP64, then L = ln q - mu
+  bdc::DevBuf g2a;    // Wbar part: N1 x KRp FP64
+  bdc::DevBuf g2b;    // Vbar part: KLp x N2 FP64
+  bdc::DevBuf splitk; // split-K partials for stage GEMMs
+  size_t splitk_elems = 0;
+  bdc::DevBuf part;   // per-block partials of the fused epilogue
+  size_t part_blocks = 0;
+  bdc::DevBuf scal;   // device scalars (see kScal* in kernels)
+  bdc::DevBuf hist;   // per-iteration stats (device)
+  int hist_cap = 0;
+
+  // split-FP16 buffers
+  bdc::Split split;
+  bdc::DevBuf plimb;   // P limbs, fp16 [L][N1][N2]
+  bdc::DevBuf ptexp;   // per-tile exponents of P (int32), [N1/128][N2/128]
+  bdc::DevBuf vlimb;   // Vbar limbs fp16 [L][KLp][N2]
+  bdc::DevBuf wlimb;   // Wbar limbs fp16 [L][N1][KRp]
+  bdc::DevBuf resid;   // sparse residual corrections of non-representable counts
+  int n_resid_a = 0, n_resid_b = 0;
+
+  // dual refinement state
+  bdc::DevBuf theta, adam_m, adam_v, grad, dvec;
+
+  // timers
+  bool timing = false;
+  bdc::Timer t_g1{"g1"}, t_g2{"g2"};
+};

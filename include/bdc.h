@@ -1,0 +1,142 @@
+/*
+ * bdc.h — C ABI of the MI355X Blahut–Arimoto engine for the binary deletion
+ * channel BDC_{n,k} (libbdc.so, built from bdc-upper-bounds-gpu_amd/csrc/).
+ *
+ * The reference (Pintz45/BDC-Upper-Bounds-GPU) has no plugin or FFI layer; its
+ * seam is the host function set declared in backend/BAA.cuh:16-29 and
+ * backend/BAA.cuh:295-362, called only by compute_C_n_k (backend/main_BAA.cu).
+ * Each entry point below names the reference function(s) it replaces.
+ *
+ * Conventions (all entry points):
+ *   - plain C types and pointers only; host buffers are borrowed and copied;
+ *   - return BDC_OK (0) or a negative status; the message of the last failure
+ *     on the calling thread is available from bdc_last_error();
+ *   - nothing calls exit() (the reference exits on every CUDA error,
+ *     backend/BAA.cuh:5-13, and on a missing file, backend/utils.cc:46-54);
+ *   - one context per thread; calls on one context are serialised by the
+ *     caller; each context owns one HIP stream and all its device memory.
+ * Bit order is MSB-first: bit (n-1-i) of an input x is its i-th symbol.
+ * Probabilities are per string: p(x) for every x of an orbit is the orbit's
+ * value, and sum_orbits |O| * value = 1 (backend/main_BAA.cu:64, BAA.cuh:66-79).
+ */
+#ifndef BDC_H
+#define BDC_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  BDC_OK = 0,
+  BDC_ERR_ARG = -1,      /* invalid argument (n, k, length, precision ...) */
+  BDC_ERR_HIP = -2,      /* HIP runtime error */
+  BDC_ERR_MEMORY = -3,   /* device allocation failed / insufficient HBM */
+  BDC_ERR_IO = -4,       /* file missing, short or malformed */
+  BDC_ERR_STATE = -5,    /* call out of order (e.g. iterate before p is set) */
+  BDC_ERR_NUMERIC = -6,  /* a precondition of exact split arithmetic failed */
+};
+
+enum {
+  BDC_PREC_FP64 = 0,  /* plain FP64 MFMA contractions */
+  BDC_PREC_SPLIT16 = 1, /* split-FP16 MFMA (exact integer limbs, FP32 accumulate) */
+};
+
+typedef struct bdc_ctx bdc_ctx;
+
+/* Per-iteration record.  All values in nats except mx_bits. */
+typedef struct {
+  double mx_bits; /* max_x log2(p_{t+1}(x)/p_t(x)) — the reference stopping quantity (BAA.cu:916-946) */
+  double max_d;   /* max_x D(x||q_t) */
+  double log_z;   /* ln sum_x p_t(x) exp(D(x||q_t)) */
+  double info;    /* I(p_t) = sum_x p_t(x) D(x||q_t) */
+} bdc_iter_stats;
+
+/* Bounds of the current p, evaluated entirely in FP64.  Nats. */
+typedef struct {
+  double lower;      /* I(p) — the reference's reported rate (main_BAA.cu:123-125) */
+  double upper_cert; /* max_x D(x||q(p)) — certified upper bound on C_{n,k} */
+  double sum_p;      /* sum_x p(x) (should be 1) */
+} bdc_certificate;
+
+const char *bdc_last_error(void);
+int bdc_version(void);
+
+/* ---- caches: generate_bit_transition_cache.cpp:23-77, cache_io.h:7-44 ---- */
+/* c(x,y) for all x in {0,1}^n, y in {0,1}^k into out (2^(n+k) uint32, index
+ * (x<<k)|y), computed by the subsequence-count DP on the GPU.  0 <= k <= n <= 20. */
+int bdc_generate_cache(int n, int k, uint32_t *out);
+/* Writes <dir>/cache_{n'}_{k'} for 0 <= k' < n' <= n_max, byte-identical to the
+ * reference generator's files (raw native-endian uint32, no header). */
+int bdc_write_caches(int n_max, const char *dir);
+
+/* ---- context: load_cache (BAA.cu:183-188) + orbit list (main_BAA.cu:46-50) ---- */
+/* Creates an engine for BDC_{n,k}, 2 <= n <= 32 even... see bdc_supported().
+ * cache_dir: directory holding cache_{n'}_{k'} files to load (validated for
+ * size); NULL or "" generates the needed half-length caches on the GPU. */
+int bdc_create(int n, int k, const char *cache_dir, bdc_ctx **out);
+int bdc_destroy(bdc_ctx *ctx);
+/* 1 if (n,k) is supported by this build, else 0 (message in bdc_last_error) */
+int bdc_supported(int n, int k);
+/* number of complement/reversal orbits = length of a dist file */
+int bdc_num_orbits(bdc_ctx *ctx, uint64_t *out);
+/* ascending representatives (dist-file line order) */
+int bdc_get_orbits(bdc_ctx *ctx, uint32_t *out, uint64_t len);
+
+/* ---- distribution: main_BAA.cu:53-65 (init) and :3-18 / :83-96 (file) ---- */
+int bdc_set_p_uniform(bdc_ctx *ctx);                              /* 1/2^n per string (main_BAA.cu:64) */
+int bdc_set_p_orbits(bdc_ctx *ctx, const double *q, uint64_t len); /* dist-file order */
+int bdc_get_p_orbits(bdc_ctx *ctx, double *q, uint64_t len);
+/* Dirichlet(alpha) over orbit masses with seed (no reference counterpart):
+ * orbit mass m_i ~ Gamma(alpha) (Marsaglia–Tsang on a counter-based PRNG),
+ * per-string value m_i / (|O_i| * sum m).  Deterministic in (n, alpha, seed). */
+int bdc_set_p_dirichlet(bdc_ctx *ctx, double alpha, uint64_t seed);
+
+/* ---- iteration: baa_iteration (BAA.cu:590-639) + compute_max_deviation (:916-946) ---- */
+/* Runs `iters` BA steps at the given precision; stats (may be NULL) receives one
+ * record per step.  If tol > 0 it stops early after the first step with
+ * mx_bits < tol (main_BAA.cu:73) and *done receives the steps run. */
+int bdc_iterate(bdc_ctx *ctx, int precision, int iters, double tol, bdc_iter_stats *stats, int *done);
+
+/* ---- bounds: compute_all_log_D + compute_rate (BAA.cu:642-724, 836-913) ---- */
+int bdc_certify(bdc_ctx *ctx, bdc_certificate *out);
+
+/* ---- building blocks (compute_den_kernel BAA.cu:429-485, compute_KL_kernel :533-571) ---- */
+/* q(y) = sum_x w(x) c(x,y) / C(n,k) for arbitrary full-space weights w (2^n) -> q (2^k) */
+int bdc_output_law(bdc_ctx *ctx, int precision, const double *w, double *q);
+/* D(x||q) for all x (2^n) from ln q over all y (2^k) */
+int bdc_kl(bdc_ctx *ctx, int precision, const double *log_q, double *d);
+/* H(x)/C(n,k) - ln C(n,k) = sum_y W ln W for all x (2^n) — the hoisted entropy term
+ * (the per-pair W ln W of log_W_term, BAA.cu:401-411) */
+int bdc_entropy_term(bdc_ctx *ctx, double *h);
+
+/* ---- smoothed dual bound and its adjoint (no reference counterpart) ---- */
+/* U_beta(theta) = beta^-1 log sum_x exp(beta D(x||softmax theta)) and, when grad
+ * is not NULL, dU/dtheta = softmax(theta) - W^T pi, pi = softmax_x(beta D).
+ * theta has 2^k entries. */
+int bdc_dual(bdc_ctx *ctx, int precision, const double *theta, double beta, double *u, double *grad);
+/* Adam on theta = log q (device resident): `steps` steps at learning rate lr with
+ * the default Adam constants (0.9, 0.999, 1e-8).  theta is read and written
+ * (2^k).  u_hist (may be NULL) receives U_beta before each step. */
+int bdc_refine_dual(bdc_ctx *ctx, int precision, double *theta, double beta, int steps, double lr, double *u_hist);
+/* Certified upper bound max_x D(x||softmax theta) in FP64, nats. */
+int bdc_certify_q(bdc_ctx *ctx, const double *theta, double *upper);
+/* theta = ln q(p) for the current p (FP64), 2^k entries */
+int bdc_log_output(bdc_ctx *ctx, double *theta);
+
+/* ---- timing / introspection ---- */
+/* synchronises the context stream */
+int bdc_sync(bdc_ctx *ctx);
+/* average device duration (ms) of the named kernel over its last `n` launches
+ * measured with HIP events on the context stream; names: "g1", "g2" */
+int bdc_kernel_time(bdc_ctx *ctx, const char *name, double *avg_ms, int *count);
+int bdc_reset_kernel_timers(bdc_ctx *ctx, int enable);
+/* algorithmic work of one BA step: flops of the two big contractions, and HBM
+ * bytes of the streaming parts, at the given precision */
+int bdc_work_model(bdc_ctx *ctx, int precision, double *g1_flops, double *g2_flops, double *stream_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
