@@ -1,0 +1,852 @@
+// C ABI of the engine (include/bdc.h): context lifetime, caches, distribution
+// I/O, the BA step pipeline and the bounds.  Every entry point returns a
+// status; nothing exits.
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "bdc_internal.hpp"
+#include "device_common.hpp"
+#include "kernels.hpp"
+
+namespace bdc {
+
+static thread_local std::string g_err;
+
+void set_error(const char *fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+}
+int fail(int code, const char *fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+int DevBuf::alloc(size_t b) {
+  release();
+  if (b == 0) return BDC_OK;
+  hipError_t e = hipMalloc(&p, b);
+  if (e != hipSuccess) {
+    p = nullptr;
+    (void)hipGetLastError();
+    return fail(BDC_ERR_MEMORY, "hipMalloc(%zu bytes) failed: %s", b, hipGetErrorString(e));
+  }
+  bytes = b;
+  return BDC_OK;
+}
+void DevBuf::release() {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+  bytes = 0;
+}
+
+static int check_launch(int rc, const char *what) {
+  if (rc != BDC_OK) {
+    hipError_t e = hipGetLastError();
+    return fail(rc, "%s: launch failed (%s)", what, hipGetErrorString(e));
+  }
+  return BDC_OK;
+}
+#define BDC_LAUNCH(expr) BDC_TRY(check_launch((expr), #expr))
+// inside bdc_create: release the half-built context on failure
+#define BDC_TRY_C(expr)                 \
+  do {                                  \
+    int r_ = (expr);                    \
+    if (r_ != BDC_OK) return cleanup(r_); \
+  } while (0)
+
+static uint32_t nck_u32(uint32_t n, uint32_t k) {  // backend/BAA.cu:15-24
+  if (k > n) return 0;
+  if (k > n - k) k = n - k;
+  uint64_t r = 1;
+  for (uint32_t i = 1; i <= k; ++i) r = r * (n - i + 1) / i;
+  return (uint32_t)r;
+}
+
+static uint64_t burnside_orbits(int n) {
+  const uint64_t full = 1ull << n, h = 1ull << ((n + 1) / 2);
+  const uint64_t pal = (n % 2 == 0) ? (1ull << (n / 2)) : 0ull;  // anti-palindromes exist only for even n
+  return (full + h + pal) / 4;
+}
+
+static int timer_begin(bdc_ctx *c, Timer &t) {
+  if (!c->timing) return BDC_OK;
+  if ((int)t.start.size() <= t.used) {
+    hipEvent_t a, b;
+    BDC_HIP(hipEventCreate(&a));
+    BDC_HIP(hipEventCreate(&b));
+    t.start.push_back(a);
+    t.stop.push_back(b);
+  }
+  BDC_HIP(hipEventRecord(t.start[t.used], c->stream));
+  return BDC_OK;
+}
+static int timer_end(bdc_ctx *c, Timer &t) {
+  if (!c->timing) return BDC_OK;
+  BDC_HIP(hipEventRecord(t.stop[t.used], c->stream));
+  ++t.used;
+  return BDC_OK;
+}
+
+// ------------------------------------------------------------- caches
+static int make_cache_u16(bdc_ctx *c, int len, int m, DevBuf &dst, const char *dir) {
+  BDC_TRY(dst.alloc(sizeof(uint16_t) << (len + m)));
+  if (dir && dir[0]) {
+    // cache_io.h:7-14 name, :29-44 raw uint32 content; size is validated here
+    std::string path = std::string(dir) + "/cache_" + std::to_string(len) + "_" + std::to_string(m);
+    FILE *f = fopen(path.c_str(), "rb");
+    if (!f) return fail(BDC_ERR_IO, "cannot open %s", path.c_str());
+    const size_t cnt = (size_t)1 << (len + m);
+    std::vector<uint32_t> h(cnt);
+    const size_t got = fread(h.data(), sizeof(uint32_t), cnt, f);
+    const int extra = fgetc(f);
+    fclose(f);
+    if (got != cnt || extra != EOF)
+      return fail(BDC_ERR_IO, "%s: expected %zu uint32 counts, file size differs", path.c_str(), cnt);
+    std::vector<uint16_t> h16(cnt);
+    for (size_t i = 0; i < cnt; ++i) {
+      if (h[i] > 65535u) return fail(BDC_ERR_IO, "%s: count %u does not fit the u16 device layout", path.c_str(), h[i]);
+      h16[i] = (uint16_t)h[i];
+    }
+    BDC_HIP(hipMemcpyAsync(dst.p, h16.data(), cnt * sizeof(uint16_t), hipMemcpyHostToDevice, c->stream));
+    BDC_HIP(hipStreamSynchronize(c->stream));
+    return BDC_OK;
+  }
+  BDC_LAUNCH(launch_count_cache_u16(len, m, dst.as<uint16_t>(), c->stream));
+  return BDC_OK;
+}
+
+}  // namespace bdc
+
+using namespace bdc;
+
+extern "C" {
+
+const char *bdc_last_error(void) { return g_err.c_str(); }
+int bdc_version(void) { return 1; }
+
+int bdc_generate_cache(int n, int k, uint32_t *out) {
+  if (n < 0 || n > 20 || k < 0 || k > n || n + k > 32 || !out)
+    return fail(BDC_ERR_ARG, "bdc_generate_cache: need 0 <= k <= n <= 20 and n + k <= 32");
+  DevBuf d;
+  const size_t bytes = sizeof(uint32_t) << (n + k);
+  BDC_TRY(d.alloc(bytes));
+  int rc = launch_count_cache_u32(n, k, d.as<uint32_t>(), 0);
+  if (rc != BDC_OK) {
+    d.release();
+    return fail(rc, "count cache kernel failed");
+  }
+  hipError_t e = hipMemcpy(out, d.p, bytes, hipMemcpyDeviceToHost);
+  d.release();
+  if (e != hipSuccess) return fail(BDC_ERR_HIP, "hipMemcpy: %s", hipGetErrorString(e));
+  return BDC_OK;
+}
+
+int bdc_write_caches(int n_max, const char *dir) {
+  // generate_bit_transition_cache.cpp:56-73: 0 <= k' < n' <= n_max
+  if (n_max < 0 || n_max > 20 || !dir) return fail(BDC_ERR_ARG, "bdc_write_caches: need 0 <= n_max <= 20");
+  std::vector<uint32_t> h;
+  for (int n = 0; n <= n_max; ++n)
+    for (int k = 0; k < n; ++k) {
+      h.resize((size_t)1 << (n + k));
+      BDC_TRY(bdc_generate_cache(n, k, h.data()));
+      std::string path = std::string(dir) + "/cache_" + std::to_string(n) + "_" + std::to_string(k);
+      std::string tmp = path + ".tmp";
+      FILE *f = fopen(tmp.c_str(), "wb");
+      if (!f) return fail(BDC_ERR_IO, "cannot open %s for writing", tmp.c_str());
+      const size_t w = fwrite(h.data(), sizeof(uint32_t), h.size(), f);
+      const int cl = fclose(f);
+      if (w != h.size() || cl != 0) return fail(BDC_ERR_IO, "short write to %s", tmp.c_str());
+      if (rename(tmp.c_str(), path.c_str()) != 0) return fail(BDC_ERR_IO, "rename %s failed", tmp.c_str());
+    }
+  return BDC_OK;
+}
+
+int bdc_supported(int n, int k) {
+  if (n < 2 || n > 28) return fail(0, "n must be in [2, 28] in this build"), 0;
+  if (k < 1 || k >= n) return fail(0, "k must be in [1, n-1] (C_{n,n} = n bits needs no iteration)"), 0;
+  if (k > 20) return fail(0, "k <= 20 in this build"), 0;
+  return 1;
+}
+
+int bdc_create(int n, int k, const char *cache_dir, bdc_ctx **out) {
+  if (!out) return fail(BDC_ERR_ARG, "out is NULL");
+  *out = nullptr;
+  if (!bdc_supported(n, k)) return BDC_ERR_ARG;
+  bdc_ctx *c = new bdc_ctx();
+  auto cleanup = [&](int rc) {
+    bdc_destroy(c);
+    return rc;
+  };
+  c->n = n;
+  c->k = k;
+  c->n1 = (n + 1) / 2;  // backend/BAA.cu:341-342
+  c->n2 = n - c->n1;
+  c->N1 = 1 << c->n1;
+  c->N2 = 1 << c->n2;
+  c->nx = 1ull << n;
+  c->nck = (double)nck_u32(n, k);
+  c->inv_c = 1.0 / c->nck;  // backend/BAA.cu:96-111
+  c->log_c = std::log(c->nck);
+  if (hipGetDevice(&c->device) != hipSuccess) return cleanup(fail(BDC_ERR_HIP, "no HIP device"));
+  if (hipStreamCreate(&c->stream) != hipSuccess) return cleanup(fail(BDC_ERR_HIP, "hipStreamCreate failed"));
+
+  // split sets
+  const int jlo = std::max(0, k - c->n2), jhi = std::min(k, c->n1);
+  for (int j = jlo; j <= jhi; ++j) {
+    if (2 * j <= k) {
+      c->offL.push_back(c->KL);
+      c->JL.push_back(j);
+      c->KL += 1 << j;
+    } else {
+      c->offR.push_back(c->KR);
+      c->JR.push_back(j);
+      c->KR += 1 << (k - j);
+    }
+  }
+  for (int j : c->JL)
+    if (j == c->n1) return cleanup(fail(BDC_ERR_ARG, "identity factor in the stacked operand (k too large)"));
+  for (int j : c->JR)
+    if (k - j == c->n2) return cleanup(fail(BDC_ERR_ARG, "identity factor in the stacked operand (k too large)"));
+  c->KLp = (c->KL + 63) / 64 * 64;
+  c->KRp = c->KR ? (c->KR + 63) / 64 * 64 : 0;
+  c->KT = c->KL + c->KR;
+
+  // half-length caches (only the ones a split uses; identities are implicit)
+  for (int j = jlo; j <= jhi; ++j) {
+    if (j < c->n1 && !c->cnt1[j].p) BDC_TRY_C(make_cache_u16(c, c->n1, j, c->cnt1[j], cache_dir));
+    const int m = k - j;
+    if (m < c->n2) {
+      if (c->n2 == c->n1) {
+        if (!c->cnt1[m].p) BDC_TRY_C(make_cache_u16(c, c->n1, m, c->cnt1[m], cache_dir));
+      } else if (!c->cnt2[m].p) {
+        BDC_TRY_C(make_cache_u16(c, c->n2, m, c->cnt2[m], cache_dir));
+      }
+    }
+  }
+  // stacked operands
+  BDC_TRY_C(c->abar16.alloc(sizeof(uint16_t) * (size_t)c->N1 * c->KLp));
+  if (hipMemsetAsync(c->abar16.p, 0, c->abar16.bytes, c->stream) != hipSuccess) return cleanup(fail(BDC_ERR_HIP, "memset"));
+  for (size_t i = 0; i < c->JL.size(); ++i)
+    BDC_TRY_C(check_launch(launch_stack_cols(c->cnt1[c->JL[i]].as<uint16_t>(), c->JL[i], c->N1, c->abar16.as<uint16_t>(),
+                                             c->KLp, c->offL[i], c->stream), "stack A"));
+  if (c->KR) {
+    BDC_TRY_C(c->bbar16.alloc(sizeof(uint16_t) * (size_t)c->N2 * c->KRp));
+    if (hipMemsetAsync(c->bbar16.p, 0, c->bbar16.bytes, c->stream) != hipSuccess) return cleanup(fail(BDC_ERR_HIP, "memset"));
+    for (size_t i = 0; i < c->JR.size(); ++i) {
+      const int m = k - c->JR[i];
+      BDC_TRY_C(check_launch(launch_stack_cols(c->c2(m), m, c->N2, c->bbar16.as<uint16_t>(), c->KRp, c->offR[i],
+                                               c->stream), "stack B"));
+    }
+  }
+
+  // orbits: representative mask, word ranks, ascending list (main_BAA.cu:46-50)
+  {
+    const uint64_t nwords = std::max<uint64_t>(1, c->nx / 64);
+    DevBuf pop, bsum, total;
+    BDC_TRY_C(c->repmask.alloc(nwords * 8));
+    BDC_TRY_C(c->wordbase.alloc(nwords * 4));
+    BDC_TRY_C(pop.alloc(nwords * 4));
+    BDC_TRY_C(bsum.alloc(((nwords + 1023) / 1024 + 1) * 4));
+    BDC_TRY_C(total.alloc(8));
+    BDC_TRY_C(check_launch(launch_repmask(n, nwords, c->repmask.as<uint64_t>(), pop.as<uint32_t>(), c->stream), "repmask"));
+    BDC_TRY_C(check_launch(launch_exclusive_scan(pop.as<uint32_t>(), nwords, c->wordbase.as<uint32_t>(),
+                                                 bsum.as<uint32_t>(), total.as<uint32_t>(), c->stream), "scan"));
+    uint32_t tot = 0;
+    if (hipMemcpyAsync(&tot, total.p, 4, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess)
+      return cleanup(fail(BDC_ERR_HIP, "orbit count copy"));
+    c->norb = tot;
+    if (c->norb != burnside_orbits(n))
+      return cleanup(fail(BDC_ERR_NUMERIC, "orbit count %llu != Burnside count %llu", (unsigned long long)c->norb,
+                          (unsigned long long)burnside_orbits(n)));
+    BDC_TRY_C(c->orbits.alloc(c->norb * 4));
+    BDC_TRY_C(check_launch(launch_orbit_list(nwords, c->repmask.as<uint64_t>(), c->wordbase.as<uint32_t>(),
+                                             c->orbits.as<uint32_t>(), c->stream), "orbit list"));
+    pop.release();
+    bsum.release();
+    total.release();
+  }
+
+  // work buffers
+  BDC_TRY_C(c->P.alloc(c->nx * 8));
+  BDC_TRY_C(c->Hc.alloc(c->nx * 8));
+  BDC_TRY_C(c->T.alloc((size_t)c->KLp * c->N2 * 8));
+  if (c->KR) BDC_TRY_C(c->U.alloc((size_t)c->KRp * c->N1 * 8));
+  const size_t ny = (size_t)1 << k;
+  BDC_TRY_C(c->qpart.alloc((c->JL.size() + c->JR.size()) * ny * 8));
+  BDC_TRY_C(c->qv.alloc(4 * ny * 8));  // q, L, q2 (scratch), grad
+  if (c->KR) {
+    BDC_TRY_C(c->g2a.alloc((size_t)c->N1 * c->KRp * 8));
+    if (hipMemsetAsync(c->g2a.p, 0, c->g2a.bytes, c->stream) != hipSuccess) return cleanup(fail(BDC_ERR_HIP, "memset"));
+  }
+  BDC_TRY_C(c->g2b.alloc((size_t)c->KLp * c->N2 * 8));
+  if (hipMemsetAsync(c->g2b.p, 0, c->g2b.bytes, c->stream) != hipSuccess) return cleanup(fail(BDC_ERR_HIP, "memset"));
+  c->splitk_elems = (size_t)64 * ny;  // split-K partials for the stage GEMMs: up to 64 splits of 2^k outputs
+  BDC_TRY_C(c->splitk.alloc(c->splitk_elems * 8));
+  c->part_blocks = std::max<size_t>(4096, ((size_t)c->N1 / 64) * ((size_t)c->N2 / 64));
+  BDC_TRY_C(c->part.alloc(c->part_blocks * kPartStride * 8));
+  BDC_TRY_C(c->scal.alloc(kScalCount * 8));
+  {
+    double h[kScalCount] = {0};
+    h[kScalPScale] = 1.0;
+    h[kScalOne] = 1.0;
+    if (hipMemcpyAsync(c->scal.p, h, sizeof h, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess)
+      return cleanup(fail(BDC_ERR_HIP, "scalars"));
+  }
+
+  // hoisted entropy term H(x)/C - ln C for every x (computed at representatives, expanded)
+  {
+    DevBuf hrep;
+    BDC_TRY_C(hrep.alloc(c->norb * 8));
+    BDC_TRY_C(check_launch(launch_entropy_reps(n, k, c->n1, c->n2, c->repmask.as<uint64_t>(), c->wordbase.as<uint32_t>(),
+                                               hrep.as<double>(), c->stream), "entropy"));
+    BDC_TRY_C(check_launch(launch_expand_orbits(n, hrep.as<double>(), c->repmask.as<uint64_t>(), c->wordbase.as<uint32_t>(),
+                                                c->inv_c, -c->log_c, c->Hc.as<double>(), c->stream), "expand H"));
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return cleanup(fail(BDC_ERR_HIP, "entropy sync"));
+    hrep.release();
+    c->have_h = true;
+  }
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return cleanup(fail(BDC_ERR_HIP, "setup sync"));
+  *out = c;
+  return BDC_OK;
+}
+
+int bdc_destroy(bdc_ctx *c) {
+  if (!c) return BDC_OK;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (auto *t : {&c->t_g1, &c->t_g2}) {
+    for (auto e : t->start) (void)hipEventDestroy(e);
+    for (auto e : t->stop) (void)hipEventDestroy(e);
+  }
+  for (auto &b : c->cnt1) b.release();
+  for (auto &b : c->cnt2) b.release();
+  for (bdc::DevBuf *b : {&c->abar16, &c->bbar16, &c->abarh, &c->bbarth, &c->repmask, &c->wordbase, &c->orbits, &c->P,
+                         &c->Hc, &c->T, &c->U, &c->qpart, &c->qv, &c->g2a, &c->g2b, &c->splitk, &c->part, &c->scal,
+                         &c->hist, &c->plimb, &c->ptexp, &c->vlimb, &c->wlimb, &c->resid, &c->theta, &c->adam_m,
+                         &c->adam_v, &c->grad, &c->dvec})
+    b->release();
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return BDC_OK;
+}
+
+}  // extern "C"
+
+// =====================================================================
+// BA step pipeline
+// =====================================================================
+namespace bdc {
+
+static int pick_splits(int M, int N, int K) {
+  const int tiles = ((M + 15) / 16) * ((N + 63) / 64) / (M > 16 && N > 16 ? 4 : 1);
+  int s = std::max(1, 512 / std::max(1, tiles));
+  s = std::min(s, std::max(1, K / 256));
+  return std::min(s, 64);
+}
+
+// C (M x N, row-major, contiguous) = A B with split-K through c->splitk
+static int stage_gemm(bdc_ctx *c, int M, int N, int K, const Opnd &A, const Opnd &B, double *C) {
+  const Opnd none{nullptr, 0, 0, 0};
+  int splits = pick_splits(M, N, K);
+  if ((size_t)splits * M * N > c->splitk_elems) splits = std::max<int>(1, (int)(c->splitk_elems / ((size_t)M * N)));
+  if (splits == 1) {
+    BDC_LAUNCH(gemm64(M, N, A, B, K, none, none, 0, C, N, 1, 1, 0, EPI_STORE, nullptr, c->stream, nullptr));
+    return BDC_OK;
+  }
+  double *part = c->splitk.as<double>();
+  BDC_LAUNCH(gemm64(M, N, A, B, K, none, none, 0, part, N, 1, splits, (int64_t)M * N, EPI_STORE, nullptr, c->stream,
+                    nullptr));
+  BDC_LAUNCH(launch_reduce_splits(part, splits, (int64_t)M * N, (uint64_t)M * N, C, c->stream));
+  return BDC_OK;
+}
+
+// q(y) and L(y) = ln q - mu from full-space weights X (p = scal[scale_slot] * X).
+// mirror: use q_j(y) = q_{k-j}(rev y) for j in JR (X reversal-symmetric, n even).
+static int output_law(bdc_ctx *c, int prec, const double *X, int scale_slot, bool mirror, double *qout, double *Lout) {
+  const int k = c->k, N1 = c->N1, N2 = c->N2;
+  const size_t ny = (size_t)1 << k;
+  const Opnd none{nullptr, 0, 0, 0};
+  if (c->n1 != c->n2) mirror = false;
+  // G1: Tbar = Abar^T X   (KL x N2, K = N1)
+  BDC_TRY(timer_begin(c, c->t_g1));
+  if (prec == BDC_PREC_FP64) {
+    const Opnd A{c->abar16.p, 1, c->KLp, 1}, B{X, N2, 1, 0};
+    BDC_LAUNCH(gemm64(c->KL, N2, A, B, N1, none, none, 0, c->T.as<double>(), N2, 1, 1, 0, EPI_STORE, nullptr, c->stream,
+                      nullptr));
+  } else {
+    BDC_TRY(split_output_law_g1(c, X));
+  }
+  BDC_TRY(timer_end(c, c->t_g1));
+  if (!mirror && c->KR) {
+    // Ubar = Bbar^T X^T  (KR x N1, K = N2)
+    const Opnd A{c->bbar16.p, 1, c->KRp, 1}, B{X, 1, N2, 0};
+    BDC_LAUNCH(gemm64(c->KR, N1, A, B, N2, none, none, 0, c->U.as<double>(), N1, 1, 1, 0, EPI_STORE, nullptr, c->stream,
+                      nullptr));
+  }
+  double *qL = c->qpart.as<double>();
+  double *qR = qL + c->JL.size() * ny;
+  // stage 2, j in JL: q_j = T_j B_{k-j}
+  for (size_t i = 0; i < c->JL.size(); ++i) {
+    const int j = c->JL[i], m = k - j;
+    const double *Tj = c->T.as<double>() + (size_t)c->offL[i] * N2;
+    double *dst = qL + i * ny;
+    if (m == c->n2) {
+      BDC_LAUNCH(launch_copy2d(Tj, N2, 1, dst, 1 << m, 1, 1 << j, 1 << m, c->stream));
+    } else {
+      const Opnd A{Tj, N2, 1, 0}, B{c->c2(m), 1 << m, 1, 1};
+      BDC_TRY(stage_gemm(c, 1 << j, 1 << m, N2, A, B, dst));
+    }
+  }
+  QCombine qc{};
+  qc.k = k;
+  qc.nL = (int)c->JL.size();
+  qc.qL = qL;
+  qc.scale_slot = scale_slot;
+  if (mirror) {
+    qc.mirror = 1;
+    for (int j : c->JR) {
+      const int m = k - j;
+      for (size_t i = 0; i < c->JL.size(); ++i)
+        if (c->JL[i] == m) qc.mirrorSel |= 1ull << i;
+    }
+  } else {
+    // stage 2, j in JR: q_j = A_j^T U_{k-j}
+    for (size_t i = 0; i < c->JR.size(); ++i) {
+      const int j = c->JR[i], m = k - j;
+      const double *Uj = c->U.as<double>() + (size_t)c->offR[i] * N1;
+      double *dst = qR + i * ny;
+      if (j == c->n1) {
+        // q_j[y1][z] = U[off + z][y1]
+        BDC_LAUNCH(launch_copy2d(Uj, 1, N1, dst, 1 << m, 1, 1 << j, 1 << m, c->stream));
+      } else {
+        const Opnd A{c->c1(j), 1, 1 << j, 1}, B{Uj, 1, N1, 0};
+        BDC_TRY(stage_gemm(c, 1 << j, 1 << m, N1, A, B, dst));
+      }
+    }
+    qc.nR = (int)c->JR.size();
+    qc.qR = qR;
+  }
+  BDC_LAUNCH(launch_q_combine(qc, c->inv_c, c->scal.as<double>(), qout, Lout, c->stream));
+  return BDC_OK;
+}
+
+// stage 1: Vbar rows (j in JL) and Wbar columns (j in JR) from L (FP64)
+static int cross_operands(bdc_ctx *c, const double *L) {
+  const int k = c->k, N1 = c->N1, N2 = c->N2;
+  for (size_t i = 0; i < c->JL.size(); ++i) {
+    const int j = c->JL[i], m = k - j;
+    double *dst = c->g2b.as<double>() + (size_t)c->offL[i] * N2;
+    if (m == c->n2) {
+      BDC_LAUNCH(launch_copy2d(L, 1 << m, 1, dst, N2, 1, 1 << j, N2, c->stream));
+    } else {
+      const Opnd A{L, 1 << m, 1, 0}, B{c->c2(m), 1, 1 << m, 1};
+      const Opnd none{nullptr, 0, 0, 0};
+      BDC_LAUNCH(gemm64(1 << j, N2, A, B, 1 << m, none, none, 0, dst, N2, 1, 1, 0, EPI_STORE, nullptr, c->stream, nullptr));
+    }
+  }
+  for (size_t i = 0; i < c->JR.size(); ++i) {
+    const int j = c->JR[i], m = k - j;
+    double *dst = c->g2a.as<double>() + c->offR[i];
+    if (j == c->n1) {
+      BDC_LAUNCH(launch_copy2d(L, 1 << m, 1, dst, c->KRp, 1, N1, 1 << m, c->stream));
+    } else {
+      const Opnd A{c->c1(j), 1 << j, 1, 1}, B{L, 1 << m, 1, 0};
+      const Opnd none{nullptr, 0, 0, 0};
+      BDC_LAUNCH(gemm64(N1, 1 << m, A, B, 1 << j, none, none, 0, dst, c->KRp, 1, 1, 0, EPI_STORE, nullptr, c->stream,
+                        nullptr));
+    }
+  }
+  return BDC_OK;
+}
+
+// G2 with the given epilogue; returns the number of partial blocks
+static int cross_term(bdc_ctx *c, int prec, int epi, double *Dout, int *nblocks) {
+  EpiArgs ea{};
+  ea.P = c->P.as<double>();
+  ea.Hc = c->Hc.as<double>();
+  ea.D = Dout;
+  ea.part = c->part.as<double>();
+  ea.scal = c->scal.as<double>();
+  ea.inv_c = c->inv_c;
+  ea.ldx = c->N2;
+  BDC_TRY(timer_begin(c, c->t_g2));
+  if (prec == BDC_PREC_FP64) {
+    const Opnd A1{c->abar16.p, c->KLp, 1, 1}, B1{c->g2b.p, c->N2, 1, 0};
+    const Opnd A2{c->g2a.p, c->KRp, 1, 0}, B2{c->bbar16.p, 1, c->KRp, 1};
+    BDC_LAUNCH(gemm64(c->N1, c->N2, A1, B1, c->KL, A2, B2, c->KR, nullptr, 0, 0, 1, 0, epi, &ea, c->stream, nblocks));
+  } else {
+    BDC_TRY(split_cross_term(c, epi, ea, nblocks));
+  }
+  BDC_TRY(timer_end(c, c->t_g2));
+  return BDC_OK;
+}
+
+static int ensure_hist(bdc_ctx *c, int n) {
+  if (c->hist_cap >= n) return BDC_OK;
+  BDC_TRY(c->hist.alloc((size_t)n * 4 * 8));
+  c->hist_cap = n;
+  return BDC_OK;
+}
+
+static int ensure_dvec(bdc_ctx *c) {
+  if (c->dvec.p) return BDC_OK;
+  return c->dvec.alloc(c->nx * 8);
+}
+
+// one BA step on the device; stats land in hist[idx]
+static int ba_step(bdc_ctx *c, int prec, int idx) {
+  double *q = c->qv.as<double>(), *L = q + ((size_t)1 << c->k);
+  BDC_TRY(output_law(c, prec, c->P.as<double>(), kScalPScale, true, q, L));
+  if (prec == BDC_PREC_FP64) {
+    BDC_TRY(cross_operands(c, L));
+  } else {
+    BDC_TRY(split_cross_operands(c, L));
+  }
+  int nb = 0;
+  BDC_TRY(cross_term(c, prec, EPI_BA, nullptr, &nb));
+  BDC_LAUNCH(launch_ba_finalize(c->part.as<double>(), nb, c->scal.as<double>(), c->hist.as<double>(), idx, c->stream));
+  return BDC_OK;
+}
+
+static int reset_scalars(bdc_ctx *c, double pscale) {
+  double h[kScalCount] = {0};
+  h[kScalPScale] = pscale;
+  h[kScalOne] = 1.0;
+  BDC_HIP(hipMemcpyAsync(c->scal.p, h, sizeof h, hipMemcpyHostToDevice, c->stream));
+  BDC_HIP(hipStreamSynchronize(c->stream));
+  return BDC_OK;
+}
+
+}  // namespace bdc
+
+extern "C" {
+
+int bdc_num_orbits(bdc_ctx *c, uint64_t *out) {
+  if (!c || !out) return fail(BDC_ERR_ARG, "null argument");
+  *out = c->norb;
+  return BDC_OK;
+}
+
+int bdc_get_orbits(bdc_ctx *c, uint32_t *out, uint64_t len) {
+  if (!c || !out || len != c->norb) return fail(BDC_ERR_ARG, "orbit buffer length must be %llu", (unsigned long long)(c ? c->norb : 0));
+  BDC_HIP(hipMemcpyAsync(out, c->orbits.p, len * 4, hipMemcpyDeviceToHost, c->stream));
+  BDC_HIP(hipStreamSynchronize(c->stream));
+  return BDC_OK;
+}
+
+int bdc_set_p_uniform(bdc_ctx *c) {
+  if (!c) return fail(BDC_ERR_ARG, "null context");
+  BDC_LAUNCH(launch_fill(c->P.as<double>(), c->nx, 1.0 / (double)c->nx, c->stream));  // main_BAA.cu:64
+  BDC_TRY(reset_scalars(c, 1.0));
+  c->have_p = true;
+  return BDC_OK;
+}
+
+int bdc_set_p_orbits(bdc_ctx *c, const double *q, uint64_t len) {
+  if (!c || !q) return fail(BDC_ERR_ARG, "null argument");
+  // the reference reads whatever the file holds (main_BAA.cu:3-18); here the
+  // length, sign and finiteness are checked
+  if (len != c->norb)
+    return fail(BDC_ERR_ARG, "distribution has %llu values, BDC_{%d,%d} has %llu orbits", (unsigned long long)len, c->n,
+                c->k, (unsigned long long)c->norb);
+  for (uint64_t i = 0; i < len; ++i)
+    if (!(q[i] >= 0.0) || !std::isfinite(q[i]))
+      return fail(BDC_ERR_ARG, "distribution value %llu is negative or not finite", (unsigned long long)i);
+  DevBuf d;
+  BDC_TRY(d.alloc(len * 8));
+  BDC_HIP(hipMemcpyAsync(d.p, q, len * 8, hipMemcpyHostToDevice, c->stream));
+  BDC_LAUNCH(launch_expand_orbits(c->n, d.as<double>(), c->repmask.as<uint64_t>(), c->wordbase.as<uint32_t>(), 1.0, 0.0,
+                                  c->P.as<double>(), c->stream));
+  BDC_HIP(hipStreamSynchronize(c->stream));
+  d.release();
+  BDC_TRY(reset_scalars(c, 1.0));
+  c->have_p = true;
+  return BDC_OK;
+}
+
+int bdc_get_p_orbits(bdc_ctx *c, double *q, uint64_t len) {
+  if (!c || !q || len != c->norb) return fail(BDC_ERR_ARG, "buffer length must equal the orbit count");
+  if (!c->have_p) return fail(BDC_ERR_STATE, "p is not set");
+  DevBuf d;
+  BDC_TRY(d.alloc(len * 8));
+  BDC_LAUNCH(launch_gather_orbits(len, c->orbits.as<uint32_t>(), c->P.as<double>(), c->scal.as<double>(), d.as<double>(),
+                                  c->stream));
+  BDC_HIP(hipMemcpyAsync(q, d.p, len * 8, hipMemcpyDeviceToHost, c->stream));
+  BDC_HIP(hipStreamSynchronize(c->stream));
+  d.release();
+  return BDC_OK;
+}
+
+int bdc_set_p_dirichlet(bdc_ctx *c, double alpha, uint64_t seed) {
+  if (!c || !(alpha > 0.0)) return fail(BDC_ERR_ARG, "alpha must be > 0");
+  DevBuf Q, mass, part, tot;
+  BDC_TRY(Q.alloc(c->norb * 8));
+  BDC_TRY(mass.alloc(c->norb * 8));
+  BDC_TRY(part.alloc(1024 * 8));
+  BDC_TRY(tot.alloc(8));
+  BDC_LAUNCH(launch_dirichlet_orbits(c->n, c->norb, c->orbits.as<uint32_t>(), alpha, seed, Q.as<double>(), mass.as<double>(),
+                                     c->stream));
+  BDC_LAUNCH(launch_sum(mass.as<double>(), c->norb, part.as<double>(), 1024, tot.as<double>(), c->stream));
+  double total = 0.0;
+  BDC_HIP(hipMemcpyAsync(&total, tot.p, 8, hipMemcpyDeviceToHost, c->stream));
+  BDC_HIP(hipStreamSynchronize(c->stream));
+  if (!(total > 0.0)) return fail(BDC_ERR_NUMERIC, "Dirichlet masses sum to %g", total);
+  BDC_LAUNCH(launch_expand_orbits(c->n, Q.as<double>(), c->repmask.as<uint64_t>(), c->wordbase.as<uint32_t>(), 1.0 / total,
+                                  0.0, c->P.as<double>(), c->stream));
+  BDC_HIP(hipStreamSynchronize(c->stream));
+  BDC_TRY(reset_scalars(c, 1.0));
+  c->have_p = true;
+  return BDC_OK;
+}
+
+int bdc_iterate(bdc_ctx *c, int prec, int iters, double tol, bdc_iter_stats *stats, int *done) {
+  if (!c) return fail(BDC_ERR_ARG, "null context");
+  if (prec != BDC_PREC_FP64 && prec != BDC_PREC_SPLIT16) return fail(BDC_ERR_ARG, "unknown precision %d", prec);
+  if (iters < 0) return fail(BDC_ERR_ARG, "iters < 0");
+  if (!c->have_p) return fail(BDC_ERR_STATE, "p is not set");
+  if (prec == BDC_PREC_SPLIT16) BDC_TRY(split_prepare(c));
+  BDC_TRY(ensure_hist(c, std::max(iters, 1)));
+  int ran = 0;
+  for (int it = 0; it < iters; ++it) {
+    BDC_TRY(ba_step(c, prec, it));
+    ++ran;
+    if (tol > 0.0) {
+      double mx = 0.0;
+      BDC_HIP(hipMemcpyAsync(&mx, c->hist.as<double>() + 4 * it, 8, hipMemcpyDeviceToHost, c->stream));
+      BDC_HIP(hipStreamSynchronize(c->stream));
+      if (mx < tol) break;  // main_BAA.cu:73 (loop runs while mx >= a)
+    }
+  }
+  if (stats && ran) BDC_HIP(hipMemcpyAsync(stats, c->hist.p, (size_t)ran * sizeof(bdc_iter_stats), hipMemcpyDeviceToHost, c->stream));
+  BDC_HIP(hipStreamSynchronize(c->stream));
+  if (done) *done = ran;
+  return BDC_OK;
+}
+
+int bdc_certify(bdc_ctx *c, bdc_certificate *out) {
+  if (!c || !out) return fail(BDC_ERR_ARG, "null argument");
+  if (!c->have_p) return fail(BDC_ERR_STATE, "p is not set");
+  BDC_TRY(ensure_dvec(c));
+  double *q = c->qv.as<double>(), *L = q + ((size_t)1 << c->k);
+  // FP64 throughout, no symmetry assumption
+  BDC_TRY(output_law(c, BDC_PREC_FP64, c->P.as<double>(), kScalPScale, false, q, L));
+  BDC_TRY(cross_operands(c, L));
+  int nb = 0;
+  BDC_TRY(cross_term(c, BDC_PREC_FP64, EPI_D, c->dvec.as<double>(), &nb));
+  double r[3], ps = 0.0;
+  BDC_LAUNCH(launch_dot(c->P.as<double>(), c->dvec.as<double>(), c->nx, c->part.as<double>(), c->splitk.as<double>(), c->stream));
+  BDC_HIP(hipMemcpyAsync(r, c->splitk.p, 24, hipMemcpyDeviceToHost, c->stream));
+  BDC_HIP(hipMemcpyAsync(&ps, c->scal.as<double>() + kScalPScale, 8, hipMemcpyDeviceToHost, c->stream));
+  BDC_HIP(hipStreamSynchronize(c->stream));
+  out->lower = ps * r[0];
+  out->upper_cert = r[1];
+  out->sum_p = ps * r[2];
+  return BDC_OK;
+}
+
+int bdc_output_law(bdc_ctx *c, int prec, const double *w, double *qo) {
+  if (!c || !w || !qo) return fail(BDC_ERR_ARG, "null argument");
+  if (prec == BDC_PREC_SPLIT16) BDC_TRY(split_prepare(c));
+  BDC_TRY(ensure_dvec(c));
+  BDC_HIP(hipMemcpyAsync(c->dvec.p, w, c->nx * 8, hipMemcpyHostToDevice, c->stream));
+  double *q = c->qv.as<double>() + 2 * ((size_t)1 << c->k), *L = c->qv.as<double>() + 3 * ((size_t)1 << c->k);
+  BDC_TRY(output_law(c, prec, c->dvec.as<double>(), kScalOne, false, q, L));
+  BDC_HIP(hipMemcpyAsync(qo, q, ((size_t)1 << c->k) * 8, hipMemcpyDeviceToHost, c->stream));
+  BDC_HIP(hipStreamSynchronize(c->stream));
+  return BDC_OK;
+}
+
+int bdc_kl(bdc_ctx *c, int prec, const double *log_q, double *d) {
+  if (!c || !log_q || !d) return fail(BDC_ERR_ARG, "null argument");
+  if (prec == BDC_PREC_SPLIT16) BDC_TRY(split_prepare(c));
+  BDC_TRY(ensure_dvec(c));
+  const size_t ny = (size_t)1 << c->k;
+  double *L = c->qv.as<double>() + ny;
+  // centre ln q by its mean (exact: sum_y W(y|x) = 1)
+  double mu = 0.0;
+  for (size_t y = 0; y < ny; ++y) mu += log_q[y];
+  mu /= (double)ny;
+  std::vector<double> Lh(ny);
+  for (size_t y = 0; y < ny; ++y) Lh[y] = log_q[y] - mu;
+  BDC_HIP(hipMemcpyAsync(L, Lh.data(), ny * 8, hipMemcpyHostToDevice, c->stream));
+  BDC_HIP(hipMemcpyAsync(c->scal.as<double>() + kScalMu, &mu, 8, hipMemcpyHostToDevice, c->stream));
+  if (prec == BDC_PREC_FP64)
+    BDC_TRY(cross_operands(c, L));
+  else
+    BDC_TRY(split_cross_operands(c, L));
+  int nb = 0;
+  BDC_TRY(cross_term(c, prec, EPI_D, c->dvec.as<double>(), &nb));
+  BDC_HIP(hipMemcpyAsync(d, c->dvec.p, c->nx * 8, hipMemcpyDeviceToHost, c->stream));
+  BDC_HIP(hipStreamSynchronize(c->stream));
+  return BDC_OK;
+}
+
+int bdc_entropy_term(bdc_ctx *c, double *h) {
+  if (!c || !h) return fail(BDC_ERR_ARG, "null argument");
+  BDC_HIP(hipMemcpyAsync(h, c->Hc.p, c->nx * 8, hipMemcpyDeviceToHost, c->stream));
+  BDC_HIP(hipStreamSynchronize(c->stream));
+  return BDC_OK;
+}
+
+}  // extern "C"
+
+namespace bdc {
+// U_beta(theta) and its gradient, theta resident in c->theta.  Writes U to
+// devU (device) and the gradient to c->grad.
+static int dual_device(bdc_ctx *c, int prec, double beta, double *devU, bool want_grad) {
+  const size_t ny = (size_t)1 << c->k;
+  double *q = c->qv.as<double>(), *L = q + ny, *wq = q + 2 * ny, *wl = q + 3 * ny;
+  BDC_LAUNCH(launch_softmax_log(c->theta.as<double>(), c->k, q, L, c->scal.as<double>(), c->stream));
+  if (prec == BDC_PREC_FP64)
+    BDC_TRY(cross_operands(c, L));
+  else
+    BDC_TRY(split_cross_operands(c, L));
+  int nb = 0;
+  BDC_TRY(cross_term(c, prec, EPI_D, c->dvec.as<double>(), &nb));
+  BDC_LAUNCH(launch_dual_weights(c->dvec.as<double>(), c->nx, beta, c->part.as<double>(), nb, c->scal.as<double>(), devU,
+                                 c->stream));
+  if (!want_grad) return BDC_OK;
+  // W^T pi through the output-law contraction (adjoint of the cross term)
+  BDC_TRY(output_law(c, prec, c->dvec.as<double>(), kScalDualScale, false, wq, wl));
+  BDC_LAUNCH(launch_vec_sub(q, wq, 1.0, c->grad.as<double>(), ny, c->stream));
+  return BDC_OK;
+}
+static int ensure_dual(bdc_ctx *c) {
+  const size_t ny = (size_t)1 << c->k;
+  BDC_TRY(ensure_dvec(c));
+  if (!c->theta.p) BDC_TRY(c->theta.alloc(ny * 8));
+  if (!c->grad.p) BDC_TRY(c->grad.alloc(ny * 8));
+  if (!c->adam_m.p) BDC_TRY(c->adam_m.alloc(ny * 8));
+  if (!c->adam_v.p) BDC_TRY(c->adam_v.alloc(ny * 8));
+  return BDC_OK;
+}
+}  // namespace bdc
+
+extern "C" {
+
+int bdc_dual(bdc_ctx *c, int prec, const double *theta, double beta, double *u, double *grad) {
+  if (!c || !theta || !u || !(beta > 0.0)) return fail(BDC_ERR_ARG, "bad argument (beta must be > 0)");
+  if (prec == BDC_PREC_SPLIT16) BDC_TRY(split_prepare(c));
+  BDC_TRY(ensure_dual(c));
+  const size_t ny = (size_t)1 << c->k;
+  BDC_HIP(hipMemcpyAsync(c->theta.p, theta, ny * 8, hipMemcpyHostToDevice, c->stream));
+  double *devU = c->splitk.as<double>();
+  BDC_TRY(dual_device(c, prec, beta, devU, grad != nullptr));
+  BDC_HIP(hipMemcpyAsync(u, devU, 8, hipMemcpyDeviceToHost, c->stream));
+  if (grad) BDC_HIP(hipMemcpyAsync(grad, c->grad.p, ny * 8, hipMemcpyDeviceToHost, c->stream));
+  BDC_HIP(hipStreamSynchronize(c->stream));
+  return BDC_OK;
+}
+
+int bdc_refine_dual(bdc_ctx *c, int prec, double *theta, double beta, int steps, double lr, double *u_hist) {
+  if (!c || !theta || !(beta > 0.0) || steps < 0 || !(lr > 0.0)) return fail(BDC_ERR_ARG, "bad argument");
+  if (prec == BDC_PREC_SPLIT16) BDC_TRY(split_prepare(c));
+  BDC_TRY(ensure_dual(c));
+  const size_t ny = (size_t)1 << c->k;
+  BDC_HIP(hipMemcpyAsync(c->theta.p, theta, ny * 8, hipMemcpyHostToDevice, c->stream));
+  BDC_HIP(hipMemsetAsync(c->adam_m.p, 0, ny * 8, c->stream));
+  BDC_HIP(hipMemsetAsync(c->adam_v.p, 0, ny * 8, c->stream));
+  DevBuf uh;
+  BDC_TRY(uh.alloc((size_t)std::max(steps, 1) * 8));
+  for (int s = 0; s < steps; ++s) {
+    BDC_TRY(dual_device(c, prec, beta, uh.as<double>() + s, true));
+    BDC_LAUNCH(launch_adam(c->theta.as<double>(), c->grad.as<double>(), c->adam_m.as<double>(), c->adam_v.as<double>(), ny,
+                           lr, 0.9, 0.999, 1e-8, s + 1, c->stream));
+  }
+  BDC_HIP(hipMemcpyAsync(theta, c->theta.p, ny * 8, hipMemcpyDeviceToHost, c->stream));
+  if (u_hist && steps) BDC_HIP(hipMemcpyAsync(u_hist, uh.p, (size_t)steps * 8, hipMemcpyDeviceToHost, c->stream));
+  BDC_HIP(hipStreamSynchronize(c->stream));
+  return BDC_OK;
+}
+
+int bdc_certify_q(bdc_ctx *c, const double *theta, double *upper) {
+  if (!c || !theta || !upper) return fail(BDC_ERR_ARG, "null argument");
+  BDC_TRY(ensure_dual(c));
+  const size_t ny = (size_t)1 << c->k;
+  BDC_HIP(hipMemcpyAsync(c->theta.p, theta, ny * 8, hipMemcpyHostToDevice, c->stream));
+  double *q = c->qv.as<double>(), *L = q + ny;
+  BDC_LAUNCH(launch_softmax_log(c->theta.as<double>(), c->k, q, L, c->scal.as<double>(), c->stream));
+  BDC_TRY(cross_operands(c, L));
+  int nb = 0;
+  BDC_TRY(cross_term(c, BDC_PREC_FP64, EPI_D, c->dvec.as<double>(), &nb));
+  double r[3];
+  BDC_LAUNCH(launch_dot(c->dvec.as<double>(), c->dvec.as<double>(), c->nx, c->part.as<double>(), c->splitk.as<double>(), c->stream));
+  BDC_HIP(hipMemcpyAsync(r, c->splitk.p, 24, hipMemcpyDeviceToHost, c->stream));
+  BDC_HIP(hipStreamSynchronize(c->stream));
+  *upper = r[1];
+  return BDC_OK;
+}
+
+int bdc_log_output(bdc_ctx *c, double *theta) {
+  if (!c || !theta) return fail(BDC_ERR_ARG, "null argument");
+  if (!c->have_p) return fail(BDC_ERR_STATE, "p is not set");
+  const size_t ny = (size_t)1 << c->k;
+  double *q = c->qv.as<double>(), *L = q + ny;
+  BDC_TRY(output_law(c, BDC_PREC_FP64, c->P.as<double>(), kScalPScale, false, q, L));
+  std::vector<double> h(ny);
+  BDC_HIP(hipMemcpyAsync(h.data(), q, ny * 8, hipMemcpyDeviceToHost, c->stream));
+  BDC_HIP(hipStreamSynchronize(c->stream));
+  for (size_t y = 0; y < ny; ++y) theta[y] = std::log(h[y]);
+  return BDC_OK;
+}
+
+int bdc_sync(bdc_ctx *c) {
+  if (!c) return fail(BDC_ERR_ARG, "null context");
+  BDC_HIP(hipStreamSynchronize(c->stream));
+  return BDC_OK;
+}
+
+int bdc_reset_kernel_timers(bdc_ctx *c, int enable) {
+  if (!c) return fail(BDC_ERR_ARG, "null context");
+  BDC_HIP(hipStreamSynchronize(c->stream));
+  c->timing = enable != 0;
+  c->t_g1.used = 0;
+  c->t_g2.used = 0;
+  return BDC_OK;
+}
+
+int bdc_kernel_time(bdc_ctx *c, const char *name, double *avg_ms, int *count) {
+  if (!c || !name || !avg_ms) return fail(BDC_ERR_ARG, "null argument");
+  Timer *t = nullptr;
+  if (!strcmp(name, "g1")) t = &c->t_g1;
+  if (!strcmp(name, "g2")) t = &c->t_g2;
+  if (!t) return fail(BDC_ERR_ARG, "unknown timer %s", name);
+  BDC_HIP(hipStreamSynchronize(c->stream));
+  double tot = 0.0;
+  for (int i = 0; i < t->used; ++i) {
+    float ms = 0.f;
+    BDC_HIP(hipEventElapsedTime(&ms, t->start[i], t->stop[i]));
+    tot += ms;
+  }
+  *avg_ms = t->used ? tot / t->used : 0.0;
+  if (count) *count = t->used;
+  return BDC_OK;
+}
+
+int bdc_work_model(bdc_ctx *c, int prec, double *g1, double *g2, double *bytes) {
+  if (!c) return fail(BDC_ERR_ARG, "null context");
+  const double nx = (double)c->nx;
+  // G1 with the mirror rule contracts only Abar (KL columns); G2 has K = KL + KR.
+  if (g1) *g1 = 2.0 * c->KL * nx;
+  if (g2) *g2 = 2.0 * (c->KL + c->KR) * nx;
+  // streaming: G1 reads P once (FP64: 8 B, split: limbs); G2 epilogue reads P and
+  // H and writes P (24 B/x)
+  if (bytes) *bytes = nx * (24.0 + (prec == BDC_PREC_FP64 ? 8.0 : 2.0 * c->split.g1_limbs));
+  return BDC_OK;
+}
+
+}  // extern "C"

@@ -146,7 +146,19 @@ struct bdc_ctx {
   // dual refinement state
   bdc::DevBuf theta, adam_m, adam_v, grad, dvec;
 
+  const uint16_t *c1(int j) const { return cnt1[j].as<uint16_t>(); }
+  const uint16_t *c2(int m) const { return (n2 == n1 ? cnt1[m] : cnt2[m]).as<uint16_t>(); }
+
   // timers
   bool timing = false;
   bdc::Timer t_g1{"g1"}, t_g2{"g2"};
 };
+
+namespace bdc {
+struct EpiArgs;
+// split-FP16 path (split.hip)
+int split_prepare(bdc_ctx *c);
+int split_output_law_g1(bdc_ctx *c, const double *X);
+int split_cross_operands(bdc_ctx *c, const double *L);
+int split_cross_term(bdc_ctx *c, int epi, const EpiArgs &ea, int *nblocks);
+}  // namespace bdc

@@ -6,6 +6,8 @@
 
 namespace bdc {
 
+constexpr int kPartStrideDev = 4;
+
 // ---- symmetry group {id, reverse, complement, reverse∘complement} ----
 // Same group and representative rule as backend/BAA.cu:195-258; written with
 // the hardware bit-reverse instruction instead of the mask ladder.
@@ -112,6 +114,8 @@ enum : int {
   kScalVScale = 5,   // split: exponent (as double) of the V/W limb scale
   kScalDMax = 6,     // scratch: max D of the last D pass
   kScalSumW = 7,     // scratch
+  kScalDualScale = 8, // 1 / sum_x exp(beta (D - Dmax)) of the last dual pass
+  kScalOne = 9,      // constant 1.0
   kScalCount = 16,
 };
 

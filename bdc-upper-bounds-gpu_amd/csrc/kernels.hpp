@@ -1,0 +1,94 @@
+// Host-side launch wrappers of the device kernels (one stream per call).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "../../include/bdc.h"
+
+namespace bdc {
+
+// ---- setup (kernels_setup.hip) ----
+int launch_count_cache_u32(int n, int k, uint32_t *out, hipStream_t s);
+int launch_count_cache_u16(int n, int k, uint16_t *out, hipStream_t s);
+int launch_stack_cols(const uint16_t *src, int m, uint64_t R, uint16_t *dst, int ldd, int off, hipStream_t s);
+int launch_counts_to_half(const uint16_t *src, uint64_t R, uint64_t C, int transpose, _Float16 *dst, hipStream_t s);
+int launch_repmask(int n, uint64_t nwords, uint64_t *mask, uint32_t *pop, hipStream_t s);
+int launch_exclusive_scan(const uint32_t *in, uint64_t len, uint32_t *out, uint32_t *tmp_bsum, uint32_t *total,
+                          hipStream_t s);
+int launch_orbit_list(uint64_t nwords, const uint64_t *mask, const uint32_t *wordbase, uint32_t *orbits, hipStream_t s);
+int launch_fill(double *P, uint64_t len, double val, hipStream_t s);
+int launch_expand_orbits(int n, const double *Q, const uint64_t *mask, const uint32_t *wordbase, double scale,
+                         double add, double *P, hipStream_t s);
+int launch_gather_orbits(uint64_t norb, const uint32_t *orbits, const double *P, const double *scal, double *Q,
+                         hipStream_t s);
+int launch_dirichlet_orbits(int n, uint64_t norb, const uint32_t *orbits, double alpha, uint64_t seed, double *Q,
+                            double *mass, hipStream_t s);
+int launch_sum(const double *v, uint64_t len, double *part, int np, double *out, hipStream_t s);
+int launch_entropy_reps(int n, int k, int n1, int n2, const uint64_t *mask, const uint32_t *wordbase, double *Hrep,
+                        hipStream_t s);
+
+// ---- FP64 contractions (kernels_gemm64.hip) ----
+// Operand element (r, c) at ptr[r*rs + c*cs]; type 0 = f64, 1 = u16 counts.
+struct Opnd {
+  const void *p;
+  int64_t rs, cs;
+  int type;
+};
+enum { EPI_STORE = 0, EPI_BA = 1, EPI_D = 2 };
+struct EpiArgs {
+  double *P;          // p-tilde, updated in place (EPI_BA)
+  const double *Hc;   // H(x)/C - ln C
+  double *D;          // D output (EPI_D)
+  double *part;       // per-block partials
+  const double *scal; // device scalars
+  double inv_c;
+  int64_t ldx;        // row stride of the x-indexed arrays (N2)
+};
+// C[M x N] (=) sum_phase A_ph[M x K_ph] B_ph[K_ph x N].  Phase 2 is skipped when K2 == 0.
+// splits > 1: C + z*split_stride receives the partial over the z-th K range of phase 1
+// (phase 2 must be empty).
+int gemm64(int M, int N, const Opnd &A1, const Opnd &B1, int K1, const Opnd &A2, const Opnd &B2, int K2, double *C,
+           int64_t crs, int64_t ccs, int splits, int64_t split_stride, int epi, const EpiArgs *ea, hipStream_t s,
+           int *nblocks_out);
+// C[i] = sum_z part[z * stride + i] for i < len
+int launch_reduce_splits(const double *part, int splits, int64_t stride, uint64_t len, double *C, hipStream_t s);
+
+// ---- streaming (kernels_stream.hip) ----
+// finalize the fused BA epilogue: reduce block partials, update scalars, write history
+int launch_ba_finalize(const double *part, int nblocks, double *scal, double *hist, int hist_idx, hipStream_t s);
+// q(y) = pscale * inv_c * (sum_{j in JL} qpart[jl][y] + sum_{j in JR} ...) ; L = ln q - mu
+struct QCombine {
+  int k;
+  int nL, nR;                  // number of JL / JR slices
+  const double *qL;            // nL slices of 2^k (indexed by y)
+  const double *qR;            // nR slices of 2^k (general path), or null for the mirror rule
+  int mirror;                  // 1: JR part = sum over qL slices selected by mirrorSel at rev(y)
+  unsigned long long mirrorSel; // bit i set: slice i of qL contributes to the mirrored part
+  int scale_slot;              // device scalar holding the scale of the weights (p = scale * P)
+};
+int launch_q_combine(const QCombine &qc, double inv_c, double *scal, double *qout, double *Lout, hipStream_t s);
+// theta -> q = softmax(theta), L = ln q - mu (dual bound)
+int launch_softmax_log(const double *theta, int k, double *q, double *L, double *scal, hipStream_t s);
+// D-pass reductions for the dual: max D, then pi = softmax(beta D) and U
+int launch_dual_weights(double *D, uint64_t nx, double beta, double *part, int nblocks, double *scal, double *u_out,
+                        hipStream_t s);
+int launch_adam(double *theta, const double *grad, double *m, double *v, uint64_t len, double lr, double b1, double b2,
+                double eps, int step, hipStream_t s);
+int launch_copy2d(const double *src, int64_t srs, int64_t scs, double *dst, int64_t drs, int64_t dcs, int64_t R,
+                  int64_t Cc, hipStream_t s);
+// part needs 3*1024 doubles; out[0] = sum a*b, out[1] = max b, out[2] = sum a
+int launch_dot(const double *a, const double *b, uint64_t len, double *part, double *out, hipStream_t s);
+int launch_vec_sub(const double *a, const double *b, double scale_b, double *out, uint64_t len, hipStream_t s);
+
+// ---- split-FP16 contractions (kernels_split.hip) ----
+int launch_split_p(const double *P, int N1, int N2, int limbs, int bits, _Float16 *limb, int32_t *texp, hipStream_t s);
+int launch_split_vw(const double *V, int64_t vrows, int64_t vcols, const double *W, int64_t wrows, int64_t wcols,
+                    int limbs, int bits, _Float16 *vl, _Float16 *wl, double *scal, hipStream_t s);
+int split_g1(int KLp, int N1, int N2, const _Float16 *abarh, const _Float16 *plimb, const int32_t *texp, int limbs,
+             int bits, int chunk, double *T, hipStream_t s);
+int split_g2(int N1, int N2, int KL, int KLp, int KR, int KRp, const _Float16 *abarh, const _Float16 *bbarth,
+             const _Float16 *vl, const _Float16 *wl, int limbs, int bits, int epi, const EpiArgs &ea, hipStream_t s,
+             int *nblocks_out);
+
+}  // namespace bdc

@@ -1,0 +1,245 @@
+// FP64 contractions on CDNA4 FP64 MFMA (v_mfma_f64_16x16x4f64).
+//
+// One kernel template serves every FP64 contraction of a BA step: the output
+// law G1 = Abar^T P, the cross term G2 = [Abar | Wbar][Vbar ; Bbar^T] (two K
+// phases with different operand types), and the small stage GEMMs that turn
+// G1's slabs into q and ln q into G2's operands.  Operands are read through
+// (row, col) strides so transposed views cost nothing but coalescing; count
+// matrices stay u16 in HBM and are widened to FP64 on the way into LDS.
+//
+// The G2 instantiation carries the BA update in its epilogue
+// (backend/BAA.cu:487-530 log W, :590-639 update): for each x it forms
+// D(x||q) = H(x)/C - ln C - mu - S(x)/C, writes p~(x) exp(D - dref) in place,
+// and emits per-block partials (sum of the new p~, max D, sum p~ D).
+#include "device_common.hpp"
+#include "kernels.hpp"
+
+namespace bdc {
+
+typedef double double4_ __attribute__((ext_vector_type(4)));
+
+template <int T>
+__device__ inline double ld_elem(const void *p, int64_t i) {
+  if constexpr (T == 0)
+    return static_cast<const double *>(p)[i];
+  else
+    return (double)static_cast<const uint16_t *>(p)[i];
+}
+
+// Row of the FP64 16x16 MFMA result held by `lane` in register r.
+__device__ inline int mfma64_row(int lane, int r) { return (lane >> 4) + 4 * r; }
+
+template <int BM, int BN, int WM, int WN, int TA1, int TB1, int TA2, int TB2, int EPI>
+__global__ void __launch_bounds__(256) k_gemm64(int M, int N, Opnd A1, Opnd B1, int K1, Opnd A2, Opnd B2, int K2,
+                                                double *__restrict__ C, int64_t crs, int64_t ccs, int kchunk,
+                                                int64_t split_stride, EpiArgs ea) {
+  constexpr int BK = 16;
+  constexpr int TM = BM / WM, TN = BN / WN, SM = TM / 16, SN = TN / 16;
+  static_assert(WM * WN == 4, "four waves");
+  static_assert(SM >= 1 && SN >= 1, "wave tile at least 16x16");
+  __shared__ double As[BK][BM + 1];
+  __shared__ double Bs[BK][BN + 1];
+  __shared__ double red[48];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave / WN, wn = wave % WN;
+  const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+  const int z = blockIdx.z;
+
+  double4_ acc[SM][SN];
+#pragma unroll
+  for (int i = 0; i < SM; ++i)
+#pragma unroll
+    for (int j = 0; j < SN; ++j) acc[i][j] = (double4_){0.0, 0.0, 0.0, 0.0};
+
+#pragma unroll
+  for (int ph = 0; ph < 2; ++ph) {
+    const Opnd A = ph ? A2 : A1;
+    const Opnd B = ph ? B2 : B1;
+    int kb, ke;
+    if (ph == 0) {
+      kb = z * kchunk;
+      ke = min(K1, kb + kchunk);
+    } else {
+      kb = 0;
+      ke = K2;
+    }
+    for (int k0 = kb; k0 < ke; k0 += BK) {
+      // A tile BM x BK
+#pragma unroll
+      for (int e0 = 0; e0 < BM * BK; e0 += 256) {
+        const int e = e0 + tid;
+        int m, kk;
+        if (A.cs == 1) {
+          kk = e % BK;
+          m = e / BK;
+        } else {
+          m = e % BM;
+          kk = e / BM;
+        }
+        const int gm = m0 + m, gk = k0 + kk;
+        double v = 0.0;
+        if (gm < M && gk < ke) v = ph ? ld_elem<TA2>(A.p, gm * A.rs + gk * A.cs) : ld_elem<TA1>(A.p, gm * A.rs + gk * A.cs);
+        As[kk][m] = v;
+      }
+      // B tile BK x BN
+#pragma unroll
+      for (int e0 = 0; e0 < BN * BK; e0 += 256) {
+        const int e = e0 + tid;
+        int nn, kk;
+        if (B.cs == 1) {
+          nn = e % BN;
+          kk = e / BN;
+        } else {
+          kk = e % BK;
+          nn = e / BK;
+        }
+        const int gn = n0 + nn, gk = k0 + kk;
+        double v = 0.0;
+        if (gn < N && gk < ke) v = ph ? ld_elem<TB2>(B.p, gk * B.rs + gn * B.cs) : ld_elem<TB1>(B.p, gk * B.rs + gn * B.cs);
+        Bs[kk][nn] = v;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int ks = 0; ks < BK / 4; ++ks) {
+        const int kr = ks * 4 + (lane >> 4);
+        double a[SM], b[SN];
+#pragma unroll
+        for (int i = 0; i < SM; ++i) a[i] = As[kr][wm * TM + i * 16 + (lane & 15)];
+#pragma unroll
+        for (int j = 0; j < SN; ++j) b[j] = Bs[kr][wn * TN + j * 16 + (lane & 15)];
+#pragma unroll
+        for (int i = 0; i < SM; ++i)
+#pragma unroll
+          for (int j = 0; j < SN; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+      }
+      __syncthreads();
+    }
+  }
+
+  if constexpr (EPI == EPI_STORE) {
+    double *Cz = C + (int64_t)z * split_stride;
+#pragma unroll
+    for (int i = 0; i < SM; ++i)
+#pragma unroll
+      for (int j = 0; j < SN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = m0 + wm * TM + i * 16 + mfma64_row(lane, r);
+          const int col = n0 + wn * TN + j * 16 + (lane & 15);
+          if (row < M && col < N) Cz[row * crs + col * ccs] = acc[i][j][r];
+        }
+  } else {
+    const double mu = ea.scal[kScalMu], dref = ea.scal[kScalDRef];
+    double s0 = 0.0, m1 = -INFINITY, s2 = 0.0;
+#pragma unroll
+    for (int i = 0; i < SM; ++i)
+#pragma unroll
+      for (int j = 0; j < SN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = m0 + wm * TM + i * 16 + mfma64_row(lane, r);
+          const int col = n0 + wn * TN + j * 16 + (lane & 15);
+          if (row < M && col < N) {
+            const int64_t x = (int64_t)row * ea.ldx + col;
+            const double D = ea.Hc[x] - mu - acc[i][j][r] * ea.inv_c;
+            m1 = fmax(m1, D);
+            if constexpr (EPI == EPI_BA) {
+              const double p = ea.P[x];
+              const double w = p * exp(D - dref);
+              ea.P[x] = w;
+              s0 += w;
+              s2 += p * D;
+            } else {
+              ea.D[x] = D;
+            }
+          }
+        }
+    block_reduce3(s0, m1, s2, red);
+    if (tid == 0) {
+      double *pp = ea.part + (int64_t)(blockIdx.y * gridDim.x + blockIdx.x) * kPartStrideDev;
+      pp[0] = s0;
+      pp[1] = m1;
+      pp[2] = s2;
+    }
+  }
+}
+
+template <int BM, int BN, int WM, int WN, int TA1, int TB1, int TA2, int TB2, int EPI>
+static int run(int M, int N, const Opnd &A1, const Opnd &B1, int K1, const Opnd &A2, const Opnd &B2, int K2, double *C,
+               int64_t crs, int64_t ccs, int splits, int64_t split_stride, const EpiArgs &ea, hipStream_t s, int *nb) {
+  const int kchunk = ((K1 + splits - 1) / splits + 15) / 16 * 16;
+  dim3 grid((N + BN - 1) / BN, (M + BM - 1) / BM, splits);
+  if (nb) *nb = (int)(grid.x * grid.y);
+  hipLaunchKernelGGL((k_gemm64<BM, BN, WM, WN, TA1, TB1, TA2, TB2, EPI>), grid, dim3(256), 0, s, M, N, A1, B1, K1, A2,
+                     B2, K2, C, crs, ccs, kchunk, split_stride, ea);
+  return hipGetLastError() == hipSuccess ? BDC_OK : BDC_ERR_HIP;
+}
+
+template <int TA1, int TB1, int TA2, int TB2, int EPI>
+static int dispatch_shape(int M, int N, const Opnd &A1, const Opnd &B1, int K1, const Opnd &A2, const Opnd &B2, int K2,
+                          double *C, int64_t crs, int64_t ccs, int splits, int64_t split_stride, const EpiArgs &ea,
+                          hipStream_t s, int *nb) {
+  if (M <= 16 && EPI == EPI_STORE)
+    return run<16, 64, 1, 4, TA1, TB1, TA2, TB2, EPI>(M, N, A1, B1, K1, A2, B2, K2, C, crs, ccs, splits, split_stride,
+                                                      ea, s, nb);
+  if (N <= 16 && EPI == EPI_STORE)
+    return run<64, 16, 4, 1, TA1, TB1, TA2, TB2, EPI>(M, N, A1, B1, K1, A2, B2, K2, C, crs, ccs, splits, split_stride,
+                                                      ea, s, nb);
+  return run<64, 64, 2, 2, TA1, TB1, TA2, TB2, EPI>(M, N, A1, B1, K1, A2, B2, K2, C, crs, ccs, splits, split_stride,
+                                                    ea, s, nb);
+}
+
+int gemm64(int M, int N, const Opnd &A1, const Opnd &B1, int K1, const Opnd &A2, const Opnd &B2, int K2, double *C,
+           int64_t crs, int64_t ccs, int splits, int64_t split_stride, int epi, const EpiArgs *ea_in, hipStream_t s,
+           int *nb) {
+  EpiArgs ea{};
+  if (ea_in) ea = *ea_in;
+  if (splits < 1) splits = 1;
+  if (splits > 1 && K2 != 0) return BDC_ERR_ARG;
+  const int code = (A1.type << 3) | (B1.type << 2) | (K2 ? ((A2.type << 1) | B2.type) : 0);
+  if (epi == EPI_STORE) {
+    switch (code) {
+      case 0b0100:  // f64 x u16
+        return dispatch_shape<0, 1, 0, 0, EPI_STORE>(M, N, A1, B1, K1, A2, B2, K2, C, crs, ccs, splits, split_stride,
+                                                     ea, s, nb);
+      case 0b1000:  // u16 x f64
+        return dispatch_shape<1, 0, 0, 0, EPI_STORE>(M, N, A1, B1, K1, A2, B2, K2, C, crs, ccs, splits, split_stride,
+                                                     ea, s, nb);
+      case 0b0000:  // f64 x f64
+        return dispatch_shape<0, 0, 0, 0, EPI_STORE>(M, N, A1, B1, K1, A2, B2, K2, C, crs, ccs, splits, split_stride,
+                                                     ea, s, nb);
+      case 0b1001:  // G2 phases: (u16 x f64) then (f64 x u16)
+        return dispatch_shape<1, 0, 0, 1, EPI_STORE>(M, N, A1, B1, K1, A2, B2, K2, C, crs, ccs, splits, split_stride,
+                                                     ea, s, nb);
+      default:
+        return BDC_ERR_ARG;
+    }
+  }
+  if (code != 0b1001 && code != 0b1000) return BDC_ERR_ARG;
+  if (epi == EPI_BA) {
+    if (code == 0b1001)
+      return run<64, 64, 2, 2, 1, 0, 0, 1, EPI_BA>(M, N, A1, B1, K1, A2, B2, K2, C, crs, ccs, 1, 0, ea, s, nb);
+    return run<64, 64, 2, 2, 1, 0, 0, 0, EPI_BA>(M, N, A1, B1, K1, A2, B2, K2, C, crs, ccs, 1, 0, ea, s, nb);
+  }
+  if (epi == EPI_D) {
+    if (code == 0b1001)
+      return run<64, 64, 2, 2, 1, 0, 0, 1, EPI_D>(M, N, A1, B1, K1, A2, B2, K2, C, crs, ccs, 1, 0, ea, s, nb);
+    return run<64, 64, 2, 2, 1, 0, 0, 0, EPI_D>(M, N, A1, B1, K1, A2, B2, K2, C, crs, ccs, 1, 0, ea, s, nb);
+  }
+  return BDC_ERR_ARG;
+}
+
+__global__ void k_reduce_splits(const double *__restrict__ part, int splits, int64_t stride, uint64_t len,
+                                double *__restrict__ C) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= len) return;
+  double s = 0.0;
+  for (int z = 0; z < splits; ++z) s += part[(int64_t)z * stride + i];
+  C[i] = s;
+}
+int launch_reduce_splits(const double *part, int splits, int64_t stride, uint64_t len, double *C, hipStream_t s) {
+  hipLaunchKernelGGL(k_reduce_splits, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, s, part, splits, stride, len, C);
+  return hipGetLastError() == hipSuccess ? BDC_OK : BDC_ERR_HIP;
+}
+
+}  // namespace bdc

@@ -1,0 +1,171 @@
+"""GPU parity: libbdc.so (through the C ABI) against the CPU oracle.
+
+Counts and orbits are bit-exact; FP64 bounds agree with the oracle to 1e-10
+bits (the FP64 tolerance of the metric); the split-FP16 path to 1e-9 bits.
+"""
+import hashlib
+import json
+import math
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import dense_ba
+
+pytestmark = pytest.mark.gpu
+LN2 = math.log(2.0)
+
+
+def say(msg):
+    print(f"[gpu-case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def bdc():
+    import bdc_gpu
+    bdc_gpu.lib()
+    return bdc_gpu
+
+
+@pytest.fixture(scope="module")
+def golden():
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "cache_sha256.json")
+    with open(here) as f:
+        return json.load(f)["sha256"]
+
+
+def test_caches_match_reference_generator(bdc, golden):
+    for key, digest in sorted(golden.items()):
+        n, k = map(int, key.split("_"))
+        say(f"cache n={n} k={k}")
+        assert hashlib.sha256(bdc.generate_cache(n, k).tobytes()).hexdigest() == digest, key
+
+
+def test_caches_match_oracle_beyond_fixtures(bdc, oracle):
+    for n, k in [(15, 7), (16, 3)]:
+        say(f"cache n={n} k={k}")
+        assert np.array_equal(bdc.generate_cache(n, k), oracle.cache(n, k))
+
+
+def test_write_caches_byte_identical(bdc, golden, tmp_path):
+    say("write_caches 9")
+    bdc.write_caches(9, str(tmp_path))
+    for n in range(1, 10):
+        for k in range(n):
+            with open(tmp_path / f"cache_{n}_{k}", "rb") as f:
+                assert hashlib.sha256(f.read()).hexdigest() == golden[f"{n}_{k}"]
+
+
+@pytest.mark.parametrize("n,k", [(16, 8), (13, 6), (9, 4)])
+def test_orbits(bdc, oracle, n, k):
+    say(f"orbits n={n} k={k}")
+    with bdc.Engine(n, k) as e:
+        assert e.num_orbits == len(oracle.orbit_list(n))
+        assert np.array_equal(e.orbits(), oracle.orbit_list(n))
+
+
+@pytest.mark.parametrize("n,k", [(10, 5), (11, 4), (12, 7), (14, 7)])
+def test_entropy_term(bdc, n, k):
+    say(f"entropy n={n} k={k}")
+    W = dense_ba.channel(n, k)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        ref = np.where(W > 0, W * np.log(np.where(W > 0, W, 1.0)), 0.0).sum(axis=1)
+    with bdc.Engine(n, k) as e:
+        np.testing.assert_allclose(e.entropy_term(), ref, rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("n,k", [(8, 4), (10, 5), (11, 6), (12, 7), (12, 3)])
+def test_output_law_general_weights(bdc, oracle, n, k):
+    say(f"output law n={n} k={k}")
+    rng = np.random.default_rng(n * 100 + k)
+    w = rng.random(1 << n)
+    with bdc.Engine(n, k) as e:
+        q = e.output_law(w)
+    np.testing.assert_allclose(q, oracle.full_output_law(n, k, w), rtol=1e-13)
+
+
+@pytest.mark.parametrize("n,k", [(10, 5), (11, 5), (12, 8)])
+def test_kl_general_q(bdc, oracle, n, k):
+    say(f"kl n={n} k={k}")
+    rng = np.random.default_rng(7 + n)
+    q = rng.random(1 << k) + 0.05
+    q /= q.sum()
+    with bdc.Engine(n, k) as e:
+        d = e.kl(np.log(q))
+    np.testing.assert_allclose(d, oracle.full_kl(n, k, np.log(q)), rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("n,k,iters", [(16, 8, 25), (13, 6, 15), (12, 6, 15), (18, 9, 4)])
+def test_ba_fp64_matches_oracle(bdc, oracle, n, k, iters):
+    say(f"ba fp64 n={n} k={k} iters={iters}")
+    ref = oracle.BA(n, k)
+    with bdc.Engine(n, k) as e:
+        e.set_p_uniform()
+        stats = e.iterate(iters, bdc.PREC_FP64)
+        assert stats.shape == (iters, 4)
+        for t in range(iters):
+            mx = ref.iterate()
+            assert abs(stats[t, 0] - mx) < 1e-10, (t, stats[t, 0], mx)
+        np.testing.assert_allclose(e.get_p_orbits(), ref.get_q(), rtol=1e-10)
+        cert = e.certify()
+        assert abs(cert["lower"] - ref.rate()) / LN2 < 1e-10
+        assert abs(cert["upper_cert"] - ref.max_kl()) / LN2 < 1e-10
+        assert abs(cert["sum_p"] - 1.0) < 1e-12
+        # sandwich: I(p_t) <= max_x D_t, and I nondecreasing
+        assert (stats[:, 3] <= stats[:, 1] + 1e-12).all()
+        assert (np.diff(stats[:, 3]) >= -1e-12).all()
+
+
+def test_ba_tolerance_stop(bdc, oracle):
+    n, k = 12, 6
+    say("ba tol stop")
+    ref = oracle.BA(n, k)
+    ref_iters = 0
+    while True:
+        ref_iters += 1
+        if ref.iterate() < 1e-3:
+            break
+    with bdc.Engine(n, k) as e:
+        e.set_p_uniform()
+        stats = e.iterate(1000, bdc.PREC_FP64, tol=1e-3)
+    assert len(stats) == ref_iters
+
+
+def test_dist_roundtrip_and_dirichlet(bdc, oracle):
+    n, k = 14, 7
+    say("dist roundtrip")
+    with bdc.Engine(n, k) as e:
+        e.set_p_dirichlet(1.0, 0)
+        q1 = e.get_p_orbits()
+        e.set_p_dirichlet(1.0, 0)
+        assert np.array_equal(q1, e.get_p_orbits())  # deterministic in the seed
+        sizes = np.array([oracle.orbit_size(int(r), n) for r in e.orbits()])
+        assert abs((sizes * q1).sum() - 1.0) < 1e-12
+        assert (q1 > 0).all()
+        e.set_p_orbits(q1)
+        assert np.array_equal(e.get_p_orbits(), q1)
+        with pytest.raises(bdc.BdcError):
+            e.set_p_orbits(q1[:-1])
+
+
+@pytest.mark.parametrize("n,k", [(10, 5), (11, 6)])
+def test_dual_gradient(bdc, oracle, n, k):
+    say(f"dual n={n} k={k}")
+    rng = np.random.default_rng(2)
+    theta = rng.normal(scale=0.5, size=1 << k)
+    with bdc.Engine(n, k) as e:
+        for beta in (64.0, 256.0):
+            U, g = e.dual(theta, beta)
+            Uo, go = oracle.full_dual(n, k, theta, beta)
+            assert abs(U - Uo) < 1e-12
+            np.testing.assert_allclose(g, go, rtol=0, atol=1e-12)
+            h = 1e-6
+            for y in rng.choice(1 << k, 16, replace=False):
+                tp, tm = theta.copy(), theta.copy()
+                tp[y] += h
+                tm[y] -= h
+                fd = (e.dual(tp, beta, want_grad=False) - e.dual(tm, beta, want_grad=False)) / (2 * h)
+                assert abs(fd - g[y]) < 1e-7

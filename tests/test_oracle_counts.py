@@ -1,0 +1,80 @@
+"""The oracle's subsequence counts and cache files against the reference
+generator (golden digests and raw files), brute force and closed forms."""
+import hashlib
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+import dense_ba
+
+
+def test_count_matches_brute_force(oracle):
+    rng = np.random.default_rng(0)
+    for n in range(0, 11):
+        for k in range(0, n + 1):
+            for _ in range(6):
+                x = int(rng.integers(0, 1 << n)) if n else 0
+                y = int(rng.integers(0, 1 << k)) if k else 0
+                assert oracle.count(x, n, y, k) == dense_ba.brute_count(x, n, y, k)
+
+
+def test_count_matrix_helper_matches_brute_force():
+    n, k = 7, 3
+    C = dense_ba.count_matrix(n, k)
+    for x in range(1 << n):
+        for y in range(1 << k):
+            assert C[x, y] == dense_ba.brute_count(x, n, y, k)
+
+
+def test_cache_raw_files_byte_identical(oracle, golden_dir):
+    raw = os.path.join(golden_dir, "raw")
+    files = sorted(os.listdir(raw))
+    assert len(files) == sum(n for n in range(1, 7))
+    for name in files:
+        _, n, k = name.split("_")
+        n, k = int(n), int(k)
+        with open(os.path.join(raw, name), "rb") as f:
+            data = f.read()
+        assert oracle.cache(n, k).tobytes() == data, name
+
+
+def test_cache_digests_match_reference_generator(oracle, golden_dir):
+    with open(os.path.join(golden_dir, "cache_sha256.json")) as f:
+        ref = json.load(f)["sha256"]
+    assert len(ref) == sum(n for n in range(1, 15))
+    for key, digest in ref.items():
+        n, k = map(int, key.split("_"))
+        if n > 12:
+            continue  # n' = 13, 14 are checked on the GPU path and in test_cache_digests_large
+        assert hashlib.sha256(oracle.cache(n, k).tobytes()).hexdigest() == digest, key
+
+
+@pytest.mark.parametrize("n,k", [(13, 12), (14, 7), (14, 13)])
+def test_cache_digests_large(oracle, golden_dir, n, k):
+    with open(os.path.join(golden_dir, "cache_sha256.json")) as f:
+        ref = json.load(f)["sha256"]
+    assert hashlib.sha256(oracle.cache(n, k).tobytes()).hexdigest() == ref[f"{n}_{k}"]
+
+
+@pytest.mark.parametrize("n,k", [(8, 3), (9, 5), (10, 9)])
+def test_cache_invariants(oracle, n, k):
+    c = oracle.cache(n, k).reshape(1 << n, 1 << k).astype(np.int64)
+    assert (c.sum(axis=1) == math.comb(n, k)).all()  # row sums
+    assert (c.sum(axis=0) == math.comb(n, k) * 2 ** (n - k)).all()  # column sums
+    assert c[0, 0] == math.comb(n, k)
+    # complement of both strings, reversal of both strings
+    xs, ys = np.arange(1 << n), np.arange(1 << k)
+    comp_x, comp_y = xs ^ ((1 << n) - 1), ys ^ ((1 << k) - 1)
+    assert (c[np.ix_(comp_x, comp_y)] == c).all()
+    rev = lambda v, m: int(format(v, f"0{m}b")[::-1], 2) if m else 0
+    rx = np.array([rev(v, n) for v in xs])
+    ry = np.array([rev(v, k) for v in ys])
+    assert (c[np.ix_(rx, ry)] == c).all()
+
+
+def test_cache_k0_all_ones(oracle):
+    for n in range(1, 9):
+        assert (oracle.cache(n, 0) == 1).all()
