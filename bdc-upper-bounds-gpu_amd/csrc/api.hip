@@ -334,9 +334,10 @@ int bdc_destroy(bdc_ctx *c) {
   }
   for (auto &b : c->cnt1) b.release();
   for (auto &b : c->cnt2) b.release();
-  for (bdc::DevBuf *b : {&c->abar16, &c->bbar16, &c->abarh, &c->bbarth, &c->repmask, &c->wordbase, &c->orbits, &c->P,
+  for (bdc::DevBuf *b : {&c->abar16, &c->bbar16, &c->abarh, &c->abarTh, &c->bbarh, &c->residA, &c->rowptrA,
+                         &c->entA, &c->rowptrB, &c->entB, &c->repmask, &c->wordbase, &c->orbits, &c->P,
                          &c->Hc, &c->T, &c->U, &c->qpart, &c->qv, &c->g2a, &c->g2b, &c->splitk, &c->part, &c->scal,
-                         &c->hist, &c->plimb, &c->ptexp, &c->vlimb, &c->wlimb, &c->resid, &c->theta, &c->adam_m,
+                         &c->hist, &c->plimb, &c->ptexp, &c->vlimb, &c->wlimb, &c->theta, &c->adam_m,
                          &c->adam_v, &c->grad, &c->dvec})
     b->release();
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -850,3 +851,148 @@ int bdc_work_model(bdc_ctx *c, int prec, double *g1, double *g2, double *bytes) 
 }
 
 }  // extern "C"
+
+// =====================================================================
+// split-FP16 path orchestration
+// =====================================================================
+namespace bdc {
+
+static int env_int(const char *name, int dflt) {
+  const char *v = getenv(name);
+  return v && v[0] ? atoi(v) : dflt;
+}
+
+// Largest limb width b <= 11 with mass * 2^(b-1) <= 2^24 (exact FP32 sums).
+static int exact_bits(double mass) {
+  for (int b = 11; b >= 2; --b)
+    if (mass * std::ldexp(1.0, b - 1) <= 16777216.0) return b;
+  return 0;
+}
+
+int split_prepare(bdc_ctx *c) {
+  if (c->split.ok) return BDC_OK;
+  if (c->N1 % 128 || c->N2 % 128)
+    return fail(BDC_ERR_ARG, "split-FP16 path needs n >= 14 (128 x 128 tiles of P); use FP64 for BDC_{%d,%d}", c->n,
+                c->k);
+  const size_t N1 = c->N1, N2 = c->N2, KLp = c->KLp, KRp = c->KRp;
+  std::vector<uint16_t> a(N1 * KLp), b(N2 * KRp);
+  BDC_HIP(hipMemcpyAsync(a.data(), c->abar16.p, a.size() * 2, hipMemcpyDeviceToHost, c->stream));
+  if (KRp) BDC_HIP(hipMemcpyAsync(b.data(), c->bbar16.p, b.size() * 2, hipMemcpyDeviceToHost, c->stream));
+  BDC_HIP(hipStreamSynchronize(c->stream));
+  auto round16 = [](uint16_t v) -> _Float16 { return (_Float16)(float)v; };
+  std::vector<_Float16> ah(N1 * KLp), aTh(KLp * N1), bh(N2 * KRp);
+  std::vector<int32_t> resA, rowptrA(N1 + 1, 0), entA, rowptrB(N2 + 1, 0), entB;
+  double rowA = 0, rowB = 0, chunk = 0;
+  std::vector<double> csum(KLp, 0.0);
+  for (size_t x1 = 0; x1 < N1; ++x1) {
+    double rs = 0;
+    for (size_t kk = 0; kk < KLp; ++kk) {
+      const uint16_t v = a[x1 * KLp + kk];
+      const _Float16 h = round16(v);
+      ah[x1 * KLp + kk] = h;
+      aTh[kk * N1 + x1] = h;
+      const float hf = (float)h;
+      rs += std::fabs(hf);
+      csum[kk] += std::fabs(hf);
+      const int r = (int)v - (int)hf;
+      if (r) {
+        resA.insert(resA.end(), {(int32_t)x1, (int32_t)kk, r});
+        entA.insert(entA.end(), {(int32_t)kk, r});
+      }
+    }
+    rowptrA[x1 + 1] = (int32_t)(entA.size() / 2);
+    rowA = std::max(rowA, rs);
+    if ((x1 + 1) % 128 == 0) {  // G1 accumulates 128 rows of x1 in FP32
+      for (size_t kk = 0; kk < KLp; ++kk) {
+        chunk = std::max(chunk, csum[kk]);
+        csum[kk] = 0;
+      }
+    }
+  }
+  for (size_t x2 = 0; x2 < N2; ++x2) {
+    double rs = 0;
+    for (size_t kk = 0; kk < KRp; ++kk) {
+      const uint16_t v = b[x2 * KRp + kk];
+      const _Float16 h = round16(v);
+      bh[x2 * KRp + kk] = h;
+      rs += std::fabs((float)h);
+      const int r = (int)v - (int)(float)h;
+      if (r) entB.insert(entB.end(), {(int32_t)kk, r});
+    }
+    rowptrB[x2 + 1] = (int32_t)(entB.size() / 2);
+    rowB = std::max(rowB, rs);
+  }
+  c->n_resid_a = (int)(resA.size() / 3);
+  c->n_resid_b = (int)(entB.size() / 2);
+  c->g1_chunk_sum = chunk;
+  c->g2_row_sum = rowA + rowB;
+  Split &sp = c->split;
+  sp.g1_bits = exact_bits(chunk);
+  sp.g2_bits = exact_bits(rowA + rowB);
+  if (sp.g1_bits < 6 || sp.g2_bits < 6)
+    return fail(BDC_ERR_NUMERIC, "exact FP32 accumulation needs limbs of %d / %d bits, too narrow", sp.g1_bits, sp.g2_bits);
+  sp.g1_bits = std::min(sp.g1_bits, env_int("BDC_SPLIT_G1_BITS", sp.g1_bits));
+  sp.g2_bits = std::min(sp.g2_bits, env_int("BDC_SPLIT_G2_BITS", sp.g2_bits));
+  sp.g1_limbs = env_int("BDC_SPLIT_G1_LIMBS", (36 + sp.g1_bits - 1) / sp.g1_bits);
+  sp.g2_limbs = env_int("BDC_SPLIT_G2_LIMBS", (42 + sp.g2_bits - 1) / sp.g2_bits);
+  if (sp.g1_limbs < 2 || sp.g1_limbs > 4 || sp.g2_limbs < 2 || sp.g2_limbs > 4)
+    return fail(BDC_ERR_ARG, "limb counts must be in [2, 4] (got %d, %d)", sp.g1_limbs, sp.g2_limbs);
+  if (resA.empty()) resA.push_back(0);
+  if (entA.empty()) entA.push_back(0);
+  if (entB.empty()) entB.push_back(0);
+  BDC_TRY(c->abarh.alloc(ah.size() * 2));
+  BDC_TRY(c->abarTh.alloc(aTh.size() * 2));
+  BDC_HIP(hipMemcpyAsync(c->abarh.p, ah.data(), ah.size() * 2, hipMemcpyHostToDevice, c->stream));
+  BDC_HIP(hipMemcpyAsync(c->abarTh.p, aTh.data(), aTh.size() * 2, hipMemcpyHostToDevice, c->stream));
+  if (KRp) {
+    BDC_TRY(c->bbarh.alloc(bh.size() * 2));
+    BDC_HIP(hipMemcpyAsync(c->bbarh.p, bh.data(), bh.size() * 2, hipMemcpyHostToDevice, c->stream));
+  }
+  BDC_TRY(c->residA.alloc(resA.size() * 4));
+  BDC_TRY(c->rowptrA.alloc(rowptrA.size() * 4));
+  BDC_TRY(c->entA.alloc(entA.size() * 4));
+  BDC_TRY(c->rowptrB.alloc(rowptrB.size() * 4));
+  BDC_TRY(c->entB.alloc(entB.size() * 4));
+  BDC_HIP(hipMemcpyAsync(c->residA.p, resA.data(), resA.size() * 4, hipMemcpyHostToDevice, c->stream));
+  BDC_HIP(hipMemcpyAsync(c->rowptrA.p, rowptrA.data(), rowptrA.size() * 4, hipMemcpyHostToDevice, c->stream));
+  BDC_HIP(hipMemcpyAsync(c->entA.p, entA.data(), entA.size() * 4, hipMemcpyHostToDevice, c->stream));
+  BDC_HIP(hipMemcpyAsync(c->rowptrB.p, rowptrB.data(), rowptrB.size() * 4, hipMemcpyHostToDevice, c->stream));
+  BDC_HIP(hipMemcpyAsync(c->entB.p, entB.data(), entB.size() * 4, hipMemcpyHostToDevice, c->stream));
+  BDC_TRY(c->plimb.alloc((size_t)sp.g1_limbs * c->nx * 2));
+  BDC_TRY(c->ptexp.alloc((N1 / 128) * (N2 / 128) * 4));
+  BDC_TRY(c->vlimb.alloc((size_t)sp.g2_limbs * N2 * KLp * 2));
+  if (KRp) BDC_TRY(c->wlimb.alloc((size_t)sp.g2_limbs * N1 * KRp * 2));
+  BDC_HIP(hipStreamSynchronize(c->stream));
+  sp.ok = true;
+  return BDC_OK;
+}
+
+int split_output_law_g1(bdc_ctx *c, const double *X) {
+  const Split &sp = c->split;
+  BDC_LAUNCH(launch_split_p(X, c->scal.as<double>(), c->N1, c->N2, sp.g1_limbs, sp.g1_bits, c->plimb.as<_Float16>(),
+                            c->ptexp.as<int32_t>(), c->stream));
+  BDC_LAUNCH(launch_split_g1(c->KLp, c->N1, c->N2, c->abarTh.as<_Float16>(), c->plimb.as<_Float16>(),
+                             c->ptexp.as<int32_t>(), sp.g1_limbs, sp.g1_bits, c->T.as<double>(), c->stream));
+  BDC_LAUNCH(launch_resid_g1(c->residA.as<int32_t>(), c->n_resid_a, X, c->N2, c->T.as<double>(), c->stream));
+  return BDC_OK;
+}
+
+int split_cross_operands(bdc_ctx *c, const double *L) {
+  BDC_TRY(cross_operands(c, L));  // V, W in FP64 (stage GEMMs)
+  const Split &sp = c->split;
+  BDC_LAUNCH(launch_split_vw(c->g2b.as<double>(), c->KL, c->KLp, c->N2, c->g2a.as<double>(), c->N1, c->KR, c->KRp,
+                             sp.g2_limbs, sp.g2_bits, c->vlimb.as<_Float16>(), c->wlimb.as<_Float16>(),
+                             c->part.as<double>(), c->scal.as<double>(), c->stream));
+  return BDC_OK;
+}
+
+int split_cross_term(bdc_ctx *c, int epi, const EpiArgs &ea, int *nb) {
+  const Split &sp = c->split;
+  BDC_LAUNCH(launch_split_g2(c->N1, c->N2, c->KL, c->KLp, c->KR, c->KRp, c->abarh.as<_Float16>(), c->vlimb.as<_Float16>(),
+                             c->wlimb.as<_Float16>(), c->bbarh.as<_Float16>(), sp.g2_limbs, sp.g2_bits, epi, ea,
+                             c->rowptrA.as<int32_t>(), c->entA.as<int32_t>(), c->rowptrB.as<int32_t>(),
+                             c->entB.as<int32_t>(), c->g2b.as<double>(), c->g2a.as<double>(), c->stream, nb));
+  return BDC_OK;
+}
+
+}  // namespace bdc

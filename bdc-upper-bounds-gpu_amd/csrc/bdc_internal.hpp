@@ -79,7 +79,7 @@ struct Timer {
   int used = 0;
 };
 
-struct Split {  // split-FP16 configuration, fixed at context creation
+struct Split {  // split-FP16 configuration, fixed on first use
   int g1_limbs = 4, g1_bits = 9, g1_chunk = 128;
   int g2_limbs = 4, g2_bits = 11;
   bool ok = false;
@@ -105,8 +105,9 @@ struct bdc_ctx {
   bdc::DevBuf cnt1[bdc::kMaxHalf + 1], cnt2[bdc::kMaxHalf + 1];
   bdc::DevBuf abar16;   // Abar as u16, N1 x KLp (zero padded)
   bdc::DevBuf bbar16;   // Bbar as u16, N2 x KRp
-  bdc::DevBuf abarh;    // Abar as fp16 (exact counts or rounded; residual list below), N1 x KLp
-  bdc::DevBuf bbarth;   // Bbar^T as fp16, KRp x N2
+  bdc::DevBuf abarh;    // Abar as fp16 (counts rounded to fp16; residual lists below), N1 x KLp
+  bdc::DevBuf abarTh;   // Abar^T as fp16, KLp x N1
+  bdc::DevBuf bbarh;    // Bbar as fp16, N2 x KRp
 
   // symmetry / orbits
   bdc::DevBuf repmask;   // 2^n bits (uint64 words), bit x set iff rep(x) == x
@@ -136,12 +137,14 @@ struct bdc_ctx {
 
   // split-FP16 buffers
   bdc::Split split;
-  bdc::DevBuf plimb;   // P limbs, fp16 [L][N1][N2]
+  bdc::DevBuf plimb;   // P limbs transposed, fp16 [L][N2][N1]
   bdc::DevBuf ptexp;   // per-tile exponents of P (int32), [N1/128][N2/128]
-  bdc::DevBuf vlimb;   // Vbar limbs fp16 [L][KLp][N2]
+  bdc::DevBuf vlimb;   // Vbar limbs transposed, fp16 [L][N2][KLp]
   bdc::DevBuf wlimb;   // Wbar limbs fp16 [L][N1][KRp]
-  bdc::DevBuf resid;   // sparse residual corrections of non-representable counts
+  bdc::DevBuf residA;  // (x1, kappa, r) triples of Abar counts fp16 cannot hold
+  bdc::DevBuf rowptrA, entA, rowptrB, entB;  // the same residuals as CSR over x1 (Abar) and x2 (Bbar)
   int n_resid_a = 0, n_resid_b = 0;
+  double g1_chunk_sum = 0, g2_row_sum = 0;  // exactness bounds (max count mass per FP32 accumulation)
 
   // dual refinement state
   bdc::DevBuf theta, adam_m, adam_v, grad, dvec;

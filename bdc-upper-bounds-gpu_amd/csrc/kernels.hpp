@@ -82,13 +82,16 @@ int launch_dot(const double *a, const double *b, uint64_t len, double *part, dou
 int launch_vec_sub(const double *a, const double *b, double scale_b, double *out, uint64_t len, hipStream_t s);
 
 // ---- split-FP16 contractions (kernels_split.hip) ----
-int launch_split_p(const double *P, int N1, int N2, int limbs, int bits, _Float16 *limb, int32_t *texp, hipStream_t s);
-int launch_split_vw(const double *V, int64_t vrows, int64_t vcols, const double *W, int64_t wrows, int64_t wcols,
-                    int limbs, int bits, _Float16 *vl, _Float16 *wl, double *scal, hipStream_t s);
-int split_g1(int KLp, int N1, int N2, const _Float16 *abarh, const _Float16 *plimb, const int32_t *texp, int limbs,
-             int bits, int chunk, double *T, hipStream_t s);
-int split_g2(int N1, int N2, int KL, int KLp, int KR, int KRp, const _Float16 *abarh, const _Float16 *bbarth,
-             const _Float16 *vl, const _Float16 *wl, int limbs, int bits, int epi, const EpiArgs &ea, hipStream_t s,
-             int *nblocks_out);
+int launch_split_p(const double *P, const double *scal, int N1, int N2, int limbs, int bits, _Float16 *limbT,
+                   int32_t *texp, hipStream_t s);
+int launch_split_vw(const double *V, int KL, int KLp, int N2, const double *W, int N1, int KR, int KRp, int limbs,
+                    int bits, _Float16 *vT, _Float16 *wl, double *part, double *scal, hipStream_t s);
+int launch_split_g1(int KLp, int N1, int N2, const _Float16 *aT, const _Float16 *pT, const int32_t *texp, int limbs,
+                    int bits, double *T, hipStream_t s);
+int launch_resid_g1(const int32_t *res, int nres, const double *P, int N2, double *T, hipStream_t s);
+int launch_split_g2(int N1, int N2, int KL, int KLp, int KR, int KRp, const _Float16 *A16, const _Float16 *vT,
+                    const _Float16 *wl, const _Float16 *B16, int limbs, int bits, int epi, const EpiArgs &ea,
+                    const int32_t *rowptrA, const int32_t *entA, const int32_t *rowptrB, const int32_t *entB,
+                    const double *V64, const double *W64, hipStream_t s, int *nb);
 
 }  // namespace bdc

@@ -169,3 +169,37 @@ def test_dual_gradient(bdc, oracle, n, k):
                 tm[y] -= h
                 fd = (e.dual(tp, beta, want_grad=False) - e.dual(tm, beta, want_grad=False)) / (2 * h)
                 assert abs(fd - g[y]) < 1e-7
+
+
+# ---------------------------------------------------------------- split-FP16
+@pytest.mark.parametrize("n,k,iters", [(16, 8, 25), (14, 7, 20), (18, 9, 4)])
+def test_ba_split_matches_oracle(bdc, oracle, n, k, iters):
+    say(f"ba split n={n} k={k} iters={iters}")
+    ref = oracle.BA(n, k)
+    with bdc.Engine(n, k) as e:
+        e.set_p_uniform()
+        stats = e.iterate(iters, bdc.PREC_SPLIT16)
+        for t in range(iters):
+            mx = ref.iterate()
+            assert abs(stats[t, 0] - mx) < 1e-9, (t, stats[t, 0], mx)
+        np.testing.assert_allclose(e.get_p_orbits(), ref.get_q(), rtol=1e-9)
+        cert = e.certify()
+        assert abs(cert["lower"] - ref.rate()) / LN2 < 1e-9
+        assert abs(cert["upper_cert"] - ref.max_kl()) / LN2 < 1e-9
+
+
+@pytest.mark.parametrize("n,k", [(14, 7), (15, 6), (16, 9)])
+def test_split_building_blocks(bdc, oracle, n, k):
+    say(f"split blocks n={n} k={k}")
+    rng = np.random.default_rng(n + 3 * k)
+    w = rng.random(1 << n)
+    q = rng.random(1 << k) + 0.05
+    q /= q.sum()
+    with bdc.Engine(n, k) as e:
+        q16 = e.output_law(w, bdc.PREC_SPLIT16)
+        q64 = e.output_law(w, bdc.PREC_FP64)
+        np.testing.assert_allclose(q16, q64, rtol=1e-10)
+        d16 = e.kl(np.log(q), bdc.PREC_SPLIT16)
+        d64 = e.kl(np.log(q), bdc.PREC_FP64)
+        np.testing.assert_allclose(d16, d64, rtol=0, atol=1e-11)
+    np.testing.assert_allclose(q64, oracle.full_output_law(n, k, w), rtol=1e-13)
