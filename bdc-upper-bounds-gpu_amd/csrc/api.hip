@@ -933,8 +933,8 @@ int split_prepare(bdc_ctx *c) {
     return fail(BDC_ERR_NUMERIC, "exact FP32 accumulation needs limbs of %d / %d bits, too narrow", sp.g1_bits, sp.g2_bits);
   sp.g1_bits = std::min(sp.g1_bits, env_int("BDC_SPLIT_G1_BITS", sp.g1_bits));
   sp.g2_bits = std::min(sp.g2_bits, env_int("BDC_SPLIT_G2_BITS", sp.g2_bits));
-  sp.g1_limbs = env_int("BDC_SPLIT_G1_LIMBS", (36 + sp.g1_bits - 1) / sp.g1_bits);
-  sp.g2_limbs = env_int("BDC_SPLIT_G2_LIMBS", (42 + sp.g2_bits - 1) / sp.g2_bits);
+  sp.g1_limbs = env_int("BDC_SPLIT_G1_LIMBS", std::min(4, (36 + sp.g1_bits - 1) / sp.g1_bits));
+  sp.g2_limbs = env_int("BDC_SPLIT_G2_LIMBS", std::min(4, (42 + sp.g2_bits - 1) / sp.g2_bits));
   if (sp.g1_limbs < 2 || sp.g1_limbs > 4 || sp.g2_limbs < 2 || sp.g2_limbs > 4)
     return fail(BDC_ERR_ARG, "limb counts must be in [2, 4] (got %d, %d)", sp.g1_limbs, sp.g2_limbs);
   if (resA.empty()) resA.push_back(0);

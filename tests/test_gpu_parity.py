@@ -130,7 +130,7 @@ def test_ba_tolerance_stop(bdc, oracle):
             break
     with bdc.Engine(n, k) as e:
         e.set_p_uniform()
-        stats = e.iterate(1000, bdc.PREC_FP64, tol=1e-3)
+        stats = e.iterate(5000, bdc.PREC_FP64, tol=1e-3)
     assert len(stats) == ref_iters
 
 
