@@ -405,8 +405,11 @@ static int output_law(bdc_ctx *c, int prec, const double *X, int scale_slot, boo
     const int j = c->JL[i], m = k - j;
     const double *Tj = c->T.as<double>() + (size_t)c->offL[i] * N2;
     double *dst = qL + i * ny;
-    if (m == c->n2) {
+    const int d = c->n2 - m;  // symbols B_m deletes
+    if (d == 0) {
       BDC_LAUNCH(launch_copy2d(Tj, N2, 1, dst, 1 << m, 1, 1 << j, 1 << m, c->stream));
+    } else if (d <= kMaxGatherDeletions) {
+      BDC_LAUNCH(launch_ins_gather(Tj, N2, 1, dst, 1 << m, 1, 1 << j, c->n2, d, c->stream));
     } else {
       const Opnd A{Tj, N2, 1, 0}, B{c->c2(m), 1 << m, 1, 1};
       BDC_TRY(stage_gemm(c, 1 << j, 1 << m, N2, A, B, dst));
@@ -430,9 +433,12 @@ static int output_law(bdc_ctx *c, int prec, const double *X, int scale_slot, boo
       const int j = c->JR[i], m = k - j;
       const double *Uj = c->U.as<double>() + (size_t)c->offR[i] * N1;
       double *dst = qR + i * ny;
-      if (j == c->n1) {
+      const int d = c->n1 - j;  // symbols A_j deletes
+      if (d == 0) {
         // q_j[y1][z] = U[off + z][y1]
         BDC_LAUNCH(launch_copy2d(Uj, 1, N1, dst, 1 << m, 1, 1 << j, 1 << m, c->stream));
+      } else if (d <= kMaxGatherDeletions) {
+        BDC_LAUNCH(launch_ins_gather(Uj, N1, 1, dst, 1, 1 << m, 1 << m, c->n1, d, c->stream));
       } else {
         const Opnd A{c->c1(j), 1, 1 << j, 1}, B{Uj, 1, N1, 0};
         BDC_TRY(stage_gemm(c, 1 << j, 1 << m, N1, A, B, dst));
@@ -451,8 +457,11 @@ static int cross_operands(bdc_ctx *c, const double *L) {
   for (size_t i = 0; i < c->JL.size(); ++i) {
     const int j = c->JL[i], m = k - j;
     double *dst = c->g2b.as<double>() + (size_t)c->offL[i] * N2;
-    if (m == c->n2) {
+    const int d = c->n2 - m;
+    if (d == 0) {
       BDC_LAUNCH(launch_copy2d(L, 1 << m, 1, dst, N2, 1, 1 << j, N2, c->stream));
+    } else if (d <= kMaxGatherDeletions) {
+      BDC_LAUNCH(launch_del_gather(L, 1 << m, 1, dst, N2, 1, 1 << j, c->n2, d, c->stream));
     } else {
       const Opnd A{L, 1 << m, 1, 0}, B{c->c2(m), 1, 1 << m, 1};
       const Opnd none{nullptr, 0, 0, 0};
@@ -462,8 +471,11 @@ static int cross_operands(bdc_ctx *c, const double *L) {
   for (size_t i = 0; i < c->JR.size(); ++i) {
     const int j = c->JR[i], m = k - j;
     double *dst = c->g2a.as<double>() + c->offR[i];
-    if (j == c->n1) {
+    const int d = c->n1 - j;
+    if (d == 0) {
       BDC_LAUNCH(launch_copy2d(L, 1 << m, 1, dst, c->KRp, 1, N1, 1 << m, c->stream));
+    } else if (d <= kMaxGatherDeletions) {
+      BDC_LAUNCH(launch_del_gather(L, 1, 1 << m, dst, 1, c->KRp, 1 << m, c->n1, d, c->stream));
     } else {
       const Opnd A{c->c1(j), 1 << j, 1, 1}, B{L, 1 << m, 1, 0};
       const Opnd none{nullptr, 0, 0, 0};

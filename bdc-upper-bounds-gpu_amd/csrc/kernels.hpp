@@ -81,6 +81,13 @@ int launch_copy2d(const double *src, int64_t srs, int64_t scs, double *dst, int6
 int launch_dot(const double *a, const double *b, uint64_t len, double *part, double *out, hipStream_t s);
 int launch_vec_sub(const double *a, const double *b, double scale_b, double *out, uint64_t len, hipStream_t s);
 
+// ---- few-deletion stage contractions (kernels_stage.hip) ----
+int launch_del_gather(const double *in, int64_t irs, int64_t ics, double *out, int64_t ors, int64_t ocs, int R, int nb,
+                      int d, hipStream_t s);
+int launch_ins_gather(const double *in, int64_t irs, int64_t ics, double *out, int64_t ors, int64_t ocs, int R, int nb,
+                      int d, hipStream_t s);
+constexpr int kMaxGatherDeletions = 3;
+
 // ---- split-FP16 contractions (kernels_split.hip) ----
 int launch_split_p(const double *P, const double *scal, int N1, int N2, int limbs, int bits, _Float16 *limbT,
                    int32_t *texp, hipStream_t s);

@@ -51,6 +51,8 @@ __global__ void __launch_bounds__(256) k_gemm64(int M, int N, Opnd A1, Opnd B1, 
 #pragma unroll
     for (int j = 0; j < SN; ++j) acc[i][j] = (double4_){0.0, 0.0, 0.0, 0.0};
 
+  constexpr int AE = BM * BK / 256, BE = BN * BK / 256;  // elements per thread per tile
+  double ra[AE], rb[BE];
 #pragma unroll
   for (int ph = 0; ph < 2; ++ph) {
     const Opnd A = ph ? A2 : A1;
@@ -63,11 +65,12 @@ __global__ void __launch_bounds__(256) k_gemm64(int M, int N, Opnd A1, Opnd B1, 
       kb = 0;
       ke = K2;
     }
-    for (int k0 = kb; k0 < ke; k0 += BK) {
-      // A tile BM x BK
+    if (kb >= ke) continue;
+    // global -> registers for the tile at k0 (zero outside the operand)
+    auto gload = [&](int k0) {
 #pragma unroll
-      for (int e0 = 0; e0 < BM * BK; e0 += 256) {
-        const int e = e0 + tid;
+      for (int q = 0; q < AE; ++q) {
+        const int e = tid + 256 * q;
         int m, kk;
         if (A.cs == 1) {
           kk = e % BK;
@@ -77,14 +80,11 @@ __global__ void __launch_bounds__(256) k_gemm64(int M, int N, Opnd A1, Opnd B1, 
           kk = e / BM;
         }
         const int gm = m0 + m, gk = k0 + kk;
-        double v = 0.0;
-        if (gm < M && gk < ke) v = ph ? ld_elem<TA2>(A.p, gm * A.rs + gk * A.cs) : ld_elem<TA1>(A.p, gm * A.rs + gk * A.cs);
-        As[kk][m] = v;
+        ra[q] = (gm < M && gk < ke) ? (ph ? ld_elem<TA2>(A.p, gm * A.rs + gk * A.cs) : ld_elem<TA1>(A.p, gm * A.rs + gk * A.cs)) : 0.0;
       }
-      // B tile BK x BN
 #pragma unroll
-      for (int e0 = 0; e0 < BN * BK; e0 += 256) {
-        const int e = e0 + tid;
+      for (int q = 0; q < BE; ++q) {
+        const int e = tid + 256 * q;
         int nn, kk;
         if (B.cs == 1) {
           nn = e % BN;
@@ -94,11 +94,33 @@ __global__ void __launch_bounds__(256) k_gemm64(int M, int N, Opnd A1, Opnd B1, 
           nn = e / BK;
         }
         const int gn = n0 + nn, gk = k0 + kk;
-        double v = 0.0;
-        if (gn < N && gk < ke) v = ph ? ld_elem<TB2>(B.p, gk * B.rs + gn * B.cs) : ld_elem<TB1>(B.p, gk * B.rs + gn * B.cs);
-        Bs[kk][nn] = v;
+        rb[q] = (gn < N && gk < ke) ? (ph ? ld_elem<TB2>(B.p, gk * B.rs + gn * B.cs) : ld_elem<TB1>(B.p, gk * B.rs + gn * B.cs)) : 0.0;
       }
+    };
+    auto sstore = [&]() {
+#pragma unroll
+      for (int q = 0; q < AE; ++q) {
+        const int e = tid + 256 * q;
+        if (A.cs == 1)
+          As[e % BK][e / BK] = ra[q];
+        else
+          As[e / BM][e % BM] = ra[q];
+      }
+#pragma unroll
+      for (int q = 0; q < BE; ++q) {
+        const int e = tid + 256 * q;
+        if (B.cs == 1)
+          Bs[e / BN][e % BN] = rb[q];
+        else
+          Bs[e % BK][e / BK] = rb[q];
+      }
+    };
+    gload(kb);
+    for (int k0 = kb; k0 < ke; k0 += BK) {
+      __syncthreads();  // previous tile fully consumed
+      sstore();
       __syncthreads();
+      if (k0 + BK < ke) gload(k0 + BK);  // prefetch overlaps the MFMAs below
 #pragma unroll
       for (int ks = 0; ks < BK / 4; ++ks) {
         const int kr = ks * 4 + (lane >> 4);
@@ -112,8 +134,8 @@ __global__ void __launch_bounds__(256) k_gemm64(int M, int N, Opnd A1, Opnd B1, 
 #pragma unroll
           for (int j = 0; j < SN; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
       }
-      __syncthreads();
     }
+    __syncthreads();
   }
 
   if constexpr (EPI == EPI_STORE) {
@@ -185,6 +207,9 @@ static int dispatch_shape(int M, int N, const Opnd &A1, const Opnd &B1, int K1, 
   if (N <= 16 && EPI == EPI_STORE)
     return run<64, 16, 4, 1, TA1, TB1, TA2, TB2, EPI>(M, N, A1, B1, K1, A2, B2, K2, C, crs, ccs, splits, split_stride,
                                                       ea, s, nb);
+  if (M >= 128 && N >= 128 && (int64_t)M * N >= (int64_t)128 * 128 * 256)
+    return run<128, 128, 2, 2, TA1, TB1, TA2, TB2, EPI>(M, N, A1, B1, K1, A2, B2, K2, C, crs, ccs, splits, split_stride,
+                                                        ea, s, nb);
   return run<64, 64, 2, 2, TA1, TB1, TA2, TB2, EPI>(M, N, A1, B1, K1, A2, B2, K2, C, crs, ccs, splits, split_stride,
                                                     ea, s, nb);
 }
@@ -217,14 +242,12 @@ int gemm64(int M, int N, const Opnd &A1, const Opnd &B1, int K1, const Opnd &A2,
   }
   if (code != 0b1001 && code != 0b1000) return BDC_ERR_ARG;
   if (epi == EPI_BA) {
-    if (code == 0b1001)
-      return run<64, 64, 2, 2, 1, 0, 0, 1, EPI_BA>(M, N, A1, B1, K1, A2, B2, K2, C, crs, ccs, 1, 0, ea, s, nb);
-    return run<64, 64, 2, 2, 1, 0, 0, 0, EPI_BA>(M, N, A1, B1, K1, A2, B2, K2, C, crs, ccs, 1, 0, ea, s, nb);
+    if (code == 0b1001) return dispatch_shape<1, 0, 0, 1, EPI_BA>(M, N, A1, B1, K1, A2, B2, K2, C, crs, ccs, 1, 0, ea, s, nb);
+    return dispatch_shape<1, 0, 0, 0, EPI_BA>(M, N, A1, B1, K1, A2, B2, K2, C, crs, ccs, 1, 0, ea, s, nb);
   }
   if (epi == EPI_D) {
-    if (code == 0b1001)
-      return run<64, 64, 2, 2, 1, 0, 0, 1, EPI_D>(M, N, A1, B1, K1, A2, B2, K2, C, crs, ccs, 1, 0, ea, s, nb);
-    return run<64, 64, 2, 2, 1, 0, 0, 0, EPI_D>(M, N, A1, B1, K1, A2, B2, K2, C, crs, ccs, 1, 0, ea, s, nb);
+    if (code == 0b1001) return dispatch_shape<1, 0, 0, 1, EPI_D>(M, N, A1, B1, K1, A2, B2, K2, C, crs, ccs, 1, 0, ea, s, nb);
+    return dispatch_shape<1, 0, 0, 0, EPI_D>(M, N, A1, B1, K1, A2, B2, K2, C, crs, ccs, 1, 0, ea, s, nb);
   }
   return BDC_ERR_ARG;
 }

@@ -1,0 +1,89 @@
+// Stage contractions with a few-deletion count factor, done as gathers.
+//
+// When a half-length count matrix B_m = cache_{nb,m} removes only d = nb - m
+// symbols (d <= 3), its rows hold at most C(nb, d) non-zeros and the dense
+// stage GEMM wastes both MFMA work and the 2^(nb+m)-entry read of the cache.
+// The same numbers follow from the definition of the count:
+//   c(x, z) = #{ S subset of positions, |S| = d : x with S deleted == z }
+// so for any array F
+//   sum_z F[z] c(x, z)            = sum_S F[del(x, S)]                 (deletion form)
+//   sum_x F[x] c(x, z)            = sum_S sum_{b in {0,1}^d} F[ins(z, S, b)]  (insertion form)
+// where ins(z, S, b) places the bits b at the positions S of an nb-bit word.
+// Both are plain FP64 sums of the same products the GEMM forms.
+#include "device_common.hpp"
+#include "kernels.hpp"
+
+namespace bdc {
+
+// remove the bits of x at the set positions of mask (bit 0 = least significant)
+__device__ inline uint32_t del_bits(uint32_t x, uint32_t mask) {
+  // highest positions first so lower positions keep their index
+  while (mask) {
+    const int s = 31 - __clz(mask);
+    x = ((x >> (s + 1)) << s) | (x & ((1u << s) - 1u));
+    mask &= ~(1u << s);
+  }
+  return x;
+}
+// insert the bits of b (lowest first) at the ascending set positions of mask
+__device__ inline uint32_t ins_bits(uint32_t z, uint32_t mask, uint32_t b) {
+  while (mask) {
+    const int s = __ffs(mask) - 1;
+    z = ((z >> s) << (s + 1)) | ((b & 1u) << s) | (z & ((1u << s) - 1u));
+    b >>= 1;
+    mask &= mask - 1u;
+  }
+  return z;
+}
+// next d-subset of the low nb bits after mask (Gosper), 0 when exhausted
+__device__ inline uint32_t next_subset(uint32_t v, int nb) {
+  if (v == 0) return 0;
+  const uint32_t t = v | (v - 1u);
+  const uint32_t w = (t + 1u) | (((~t & -~t) - 1u) >> (__ffs(v)));
+  return w < (1u << nb) ? w : 0u;
+}
+
+// out[r*ors + x*ocs] = sum_{|S| = d} in[r*irs + del(x, S)*ics]   for r < R, x < 2^nb
+__global__ void k_del_gather(const double *__restrict__ in, int64_t irs, int64_t ics, double *__restrict__ out,
+                             int64_t ors, int64_t ocs, int R, int nb, int d) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t nx = 1ll << nb;
+  if (i >= (int64_t)R * nx) return;
+  const int r = (int)(i >> nb);
+  const uint32_t x = (uint32_t)(i & (nx - 1));
+  double s = 0.0;
+  for (uint32_t m = (1u << d) - 1u; m; m = next_subset(m, nb)) s += in[r * irs + (int64_t)del_bits(x, m) * ics];
+  out[r * ors + (int64_t)x * ocs] = s;
+}
+
+// out[r*ors + z*ocs] = sum_{|S| = d} sum_b in[r*irs + ins(z, S, b)*ics]   for r < R, z < 2^(nb-d)
+__global__ void k_ins_gather(const double *__restrict__ in, int64_t irs, int64_t ics, double *__restrict__ out,
+                             int64_t ors, int64_t ocs, int R, int nb, int d) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int nz = nb - d;
+  const int64_t cnt = 1ll << nz;
+  if (i >= (int64_t)R * cnt) return;
+  const int r = (int)(i >> nz);
+  const uint32_t z = (uint32_t)(i & (cnt - 1));
+  double s = 0.0;
+  for (uint32_t m = (1u << d) - 1u; m; m = next_subset(m, nb))
+    for (uint32_t b = 0; b < (1u << d); ++b) s += in[r * irs + (int64_t)ins_bits(z, m, b) * ics];
+  out[r * ors + (int64_t)z * ocs] = s;
+}
+
+int launch_del_gather(const double *in, int64_t irs, int64_t ics, double *out, int64_t ors, int64_t ocs, int R, int nb,
+                      int d, hipStream_t s) {
+  const int64_t tot = (int64_t)R << nb;
+  hipLaunchKernelGGL(k_del_gather, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, in, irs, ics, out, ors, ocs, R,
+                     nb, d);
+  return hipGetLastError() == hipSuccess ? BDC_OK : BDC_ERR_HIP;
+}
+int launch_ins_gather(const double *in, int64_t irs, int64_t ics, double *out, int64_t ors, int64_t ocs, int R, int nb,
+                      int d, hipStream_t s) {
+  const int64_t tot = (int64_t)R << (nb - d);
+  hipLaunchKernelGGL(k_ins_gather, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, in, irs, ics, out, ors, ocs, R,
+                     nb, d);
+  return hipGetLastError() == hipSuccess ? BDC_OK : BDC_ERR_HIP;
+}
+
+}  // namespace bdc
