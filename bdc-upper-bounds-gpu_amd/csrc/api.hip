@@ -335,7 +335,7 @@ int bdc_destroy(bdc_ctx *c) {
   for (auto &b : c->cnt1) b.release();
   for (auto &b : c->cnt2) b.release();
   for (bdc::DevBuf *b : {&c->abar16, &c->bbar16, &c->abarh, &c->abarTh, &c->bbarh, &c->residA, &c->rowptrA,
-                         &c->entA, &c->rowptrB, &c->entB, &c->repmask, &c->wordbase, &c->orbits, &c->P,
+                         &c->entA, &c->rowptrB, &c->entB, &c->blkA, &c->blkB, &c->g1part, &c->repmask, &c->wordbase, &c->orbits, &c->P,
                          &c->Hc, &c->T, &c->U, &c->qpart, &c->qv, &c->g2a, &c->g2b, &c->splitk, &c->part, &c->scal,
                          &c->hist, &c->plimb, &c->ptexp, &c->vlimb, &c->wlimb, &c->theta, &c->adam_m,
                          &c->adam_v, &c->grad, &c->dvec})
@@ -451,8 +451,9 @@ static int output_law(bdc_ctx *c, int prec, const double *X, int scale_slot, boo
   return BDC_OK;
 }
 
-// stage 1: Vbar rows (j in JL) and Wbar columns (j in JR) from L (FP64)
-static int cross_operands(bdc_ctx *c, const double *L) {
+// stage 1: Vbar rows (j in JL) and Wbar columns (j in JR) from L (FP64).
+// mirror: L is reversal-symmetric and n1 = n2, so Wbar is a permutation of Vbar.
+static int cross_operands(bdc_ctx *c, const double *L, bool mirror = false) {
   const int k = c->k, N1 = c->N1, N2 = c->N2;
   for (size_t i = 0; i < c->JL.size(); ++i) {
     const int j = c->JL[i], m = k - j;
@@ -467,6 +468,21 @@ static int cross_operands(bdc_ctx *c, const double *L) {
       const Opnd none{nullptr, 0, 0, 0};
       BDC_LAUNCH(gemm64(1 << j, N2, A, B, 1 << m, none, none, 0, dst, N2, 1, 1, 0, EPI_STORE, nullptr, c->stream, nullptr));
     }
+  }
+  if (mirror && c->n1 == c->n2 && !c->JR.empty()) {
+    MirrorW mw{};
+    mw.count = (int)c->JR.size();
+    for (size_t i = 0; i < c->JR.size(); ++i) {
+      const int m = k - c->JR[i];
+      mw.offR[i] = c->offR[i];
+      mw.m[i] = m;
+      mw.offL[i] = -1;
+      for (size_t q = 0; q < c->JL.size(); ++q)
+        if (c->JL[q] == m) mw.offL[i] = c->offL[q];
+      if (mw.offL[i] < 0) return fail(BDC_ERR_STATE, "mirror slice for j=%d missing", c->JR[i]);
+    }
+    BDC_LAUNCH(launch_w_from_v(mw, c->g2b.as<double>(), N2, c->n2, c->g2a.as<double>(), N1, c->KR, c->KRp, c->stream));
+    return BDC_OK;
   }
   for (size_t i = 0; i < c->JR.size(); ++i) {
     const int j = c->JR[i], m = k - j;
@@ -525,9 +541,9 @@ static int ba_step(bdc_ctx *c, int prec, int idx) {
   double *q = c->qv.as<double>(), *L = q + ((size_t)1 << c->k);
   BDC_TRY(output_law(c, prec, c->P.as<double>(), kScalPScale, true, q, L));
   if (prec == BDC_PREC_FP64) {
-    BDC_TRY(cross_operands(c, L));
+    BDC_TRY(cross_operands(c, L, true));
   } else {
-    BDC_TRY(split_cross_operands(c, L));
+    BDC_TRY(split_cross_operands(c, L, true));
   }
   int nb = 0;
   BDC_TRY(cross_term(c, prec, EPI_BA, nullptr, &nb));
@@ -949,6 +965,23 @@ int split_prepare(bdc_ctx *c) {
   sp.g2_limbs = env_int("BDC_SPLIT_G2_LIMBS", std::min(4, (42 + sp.g2_bits - 1) / sp.g2_bits));
   if (sp.g1_limbs < 2 || sp.g1_limbs > 4 || sp.g2_limbs < 2 || sp.g2_limbs > 4)
     return fail(BDC_ERR_ARG, "limb counts must be in [2, 4] (got %d, %d)", sp.g1_limbs, sp.g2_limbs);
+  std::vector<uint8_t> fA(N1 / 128, 0), fB(N2 / 128, 0);
+  for (size_t x1 = 0; x1 < N1; ++x1)
+    if (rowptrA[x1 + 1] > rowptrA[x1]) fA[x1 / 128] = 1;
+  for (size_t x2 = 0; x2 < N2; ++x2)
+    if (rowptrB[x2 + 1] > rowptrB[x2]) fB[x2 / 128] = 1;
+  BDC_TRY(c->blkA.alloc(fA.size()));
+  BDC_TRY(c->blkB.alloc(fB.size()));
+  BDC_HIP(hipMemcpyAsync(c->blkA.p, fA.data(), fA.size(), hipMemcpyHostToDevice, c->stream));
+  BDC_HIP(hipMemcpyAsync(c->blkB.p, fB.data(), fB.size(), hipMemcpyHostToDevice, c->stream));
+  // split-K of G1 so that at least ~512 blocks run
+  {
+    const int blocks = (int)(N2 / 64) * (int)((KLp + 255) / 256);
+    int ks = 1;
+    while (blocks * ks < 512 && (N1 / 128) % (ks * 2) == 0 && N1 / (ks * 2) >= 1024) ks *= 2;
+    c->g1_ksplit = ks;
+    if (ks > 1) BDC_TRY(c->g1part.alloc((size_t)ks * KLp * N2 * 8));
+  }
   if (resA.empty()) resA.push_back(0);
   if (entA.empty()) entA.push_back(0);
   if (entB.empty()) entB.push_back(0);
@@ -983,14 +1016,19 @@ int split_output_law_g1(bdc_ctx *c, const double *X) {
   const Split &sp = c->split;
   BDC_LAUNCH(launch_split_p(X, c->scal.as<double>(), c->N1, c->N2, sp.g1_limbs, sp.g1_bits, c->plimb.as<_Float16>(),
                             c->ptexp.as<int32_t>(), c->stream));
+  const int ks = c->g1_ksplit;
+  double *dstT = ks > 1 ? c->g1part.as<double>() : c->T.as<double>();
   BDC_LAUNCH(launch_split_g1(c->KLp, c->N1, c->N2, c->abarTh.as<_Float16>(), c->plimb.as<_Float16>(),
-                             c->ptexp.as<int32_t>(), sp.g1_limbs, sp.g1_bits, c->T.as<double>(), c->stream));
+                             c->ptexp.as<int32_t>(), sp.g1_limbs, sp.g1_bits, ks, dstT, c->stream));
+  if (ks > 1)
+    BDC_LAUNCH(launch_reduce_splits(dstT, ks, (int64_t)c->KLp * c->N2, (uint64_t)c->KLp * c->N2, c->T.as<double>(),
+                                    c->stream));
   BDC_LAUNCH(launch_resid_g1(c->residA.as<int32_t>(), c->n_resid_a, X, c->N2, c->T.as<double>(), c->stream));
   return BDC_OK;
 }
 
-int split_cross_operands(bdc_ctx *c, const double *L) {
-  BDC_TRY(cross_operands(c, L));  // V, W in FP64 (stage GEMMs)
+int split_cross_operands(bdc_ctx *c, const double *L, bool mirror) {
+  BDC_TRY(cross_operands(c, L, mirror));  // V, W in FP64 (stage GEMMs)
   const Split &sp = c->split;
   BDC_LAUNCH(launch_split_vw(c->g2b.as<double>(), c->KL, c->KLp, c->N2, c->g2a.as<double>(), c->N1, c->KR, c->KRp,
                              sp.g2_limbs, sp.g2_bits, c->vlimb.as<_Float16>(), c->wlimb.as<_Float16>(),
@@ -1002,7 +1040,7 @@ int split_cross_term(bdc_ctx *c, int epi, const EpiArgs &ea, int *nb) {
   const Split &sp = c->split;
   BDC_LAUNCH(launch_split_g2(c->N1, c->N2, c->KL, c->KLp, c->KR, c->KRp, c->abarh.as<_Float16>(), c->vlimb.as<_Float16>(),
                              c->wlimb.as<_Float16>(), c->bbarh.as<_Float16>(), sp.g2_limbs, sp.g2_bits, epi, ea,
-                             c->rowptrA.as<int32_t>(), c->entA.as<int32_t>(), c->rowptrB.as<int32_t>(),
+                             c->blkA.as<uint8_t>(), c->blkB.as<uint8_t>(), c->rowptrA.as<int32_t>(), c->entA.as<int32_t>(), c->rowptrB.as<int32_t>(),
                              c->entB.as<int32_t>(), c->g2b.as<double>(), c->g2a.as<double>(), c->stream, nb));
   return BDC_OK;
 }

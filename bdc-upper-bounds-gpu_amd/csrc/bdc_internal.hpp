@@ -143,6 +143,9 @@ struct bdc_ctx {
   bdc::DevBuf wlimb;   // Wbar limbs fp16 [L][N1][KRp]
   bdc::DevBuf residA;  // (x1, kappa, r) triples of Abar counts fp16 cannot hold
   bdc::DevBuf rowptrA, entA, rowptrB, entB;  // the same residuals as CSR over x1 (Abar) and x2 (Bbar)
+  bdc::DevBuf blkA, blkB;  // per 128-block flags: any residual in these rows
+  bdc::DevBuf g1part;      // split-K partial slabs of Tbar
+  int g1_ksplit = 1;
   int n_resid_a = 0, n_resid_b = 0;
   double g1_chunk_sum = 0, g2_row_sum = 0;  // exactness bounds (max count mass per FP32 accumulation)
 
@@ -162,6 +165,6 @@ struct EpiArgs;
 // split-FP16 path (split.hip)
 int split_prepare(bdc_ctx *c);
 int split_output_law_g1(bdc_ctx *c, const double *X);
-int split_cross_operands(bdc_ctx *c, const double *L);
+int split_cross_operands(bdc_ctx *c, const double *L, bool mirror = false);
 int split_cross_term(bdc_ctx *c, int epi, const EpiArgs &ea, int *nblocks);
 }  // namespace bdc

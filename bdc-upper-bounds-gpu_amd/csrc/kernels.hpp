@@ -87,6 +87,12 @@ int launch_del_gather(const double *in, int64_t irs, int64_t ics, double *out, i
 int launch_ins_gather(const double *in, int64_t irs, int64_t ics, double *out, int64_t ors, int64_t ocs, int R, int nb,
                       int d, hipStream_t s);
 constexpr int kMaxGatherDeletions = 3;
+struct MirrorW {  // for each j in JR: column offset in Wbar, m = k - j, row offset of V_{k-j} in Vbar
+  int count;
+  int offR[18], m[18], offL[18];
+};
+int launch_w_from_v(const MirrorW &mw, const double *V, int N2, int n2, double *W, int N1, int KR, int KRp,
+                    hipStream_t s);
 
 // ---- split-FP16 contractions (kernels_split.hip) ----
 int launch_split_p(const double *P, const double *scal, int N1, int N2, int limbs, int bits, _Float16 *limbT,
@@ -94,11 +100,12 @@ int launch_split_p(const double *P, const double *scal, int N1, int N2, int limb
 int launch_split_vw(const double *V, int KL, int KLp, int N2, const double *W, int N1, int KR, int KRp, int limbs,
                     int bits, _Float16 *vT, _Float16 *wl, double *part, double *scal, hipStream_t s);
 int launch_split_g1(int KLp, int N1, int N2, const _Float16 *aT, const _Float16 *pT, const int32_t *texp, int limbs,
-                    int bits, double *T, hipStream_t s);
+                    int bits, int ksplit, double *T, hipStream_t s);
 int launch_resid_g1(const int32_t *res, int nres, const double *P, int N2, double *T, hipStream_t s);
 int launch_split_g2(int N1, int N2, int KL, int KLp, int KR, int KRp, const _Float16 *A16, const _Float16 *vT,
                     const _Float16 *wl, const _Float16 *B16, int limbs, int bits, int epi, const EpiArgs &ea,
-                    const int32_t *rowptrA, const int32_t *entA, const int32_t *rowptrB, const int32_t *entB,
-                    const double *V64, const double *W64, hipStream_t s, int *nb);
+                    const uint8_t *blkA, const uint8_t *blkB, const int32_t *rowptrA, const int32_t *entA,
+                    const int32_t *rowptrB, const int32_t *entB, const double *V64, const double *W64, hipStream_t s,
+                    int *nb);
 
 }  // namespace bdc

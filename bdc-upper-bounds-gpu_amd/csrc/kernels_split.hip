@@ -36,7 +36,8 @@ template <int L>
 __global__ void __launch_bounds__(256) k_split_p(const double *__restrict__ P, const double *__restrict__ scal, int N1,
                                                  int N2, int bits, _Float16 *__restrict__ limbT,
                                                  int32_t *__restrict__ texp) {
-  __shared__ _Float16 tile[128][130];
+  constexpr int PITCH = 136;  // halves per LDS row: 16-byte aligned rows
+  __shared__ __attribute__((aligned(16))) _Float16 tile[128][PITCH];  // [x2][x1]
   __shared__ double red[16];
   const int tid = threadIdx.x;
   const int m0 = blockIdx.y * 128, n0 = blockIdx.x * 128;
@@ -54,20 +55,22 @@ __global__ void __launch_bounds__(256) k_split_p(const double *__restrict__ P, c
 #pragma unroll
   for (int r = 0; r < 64; ++r) v[r] = ldexp(v[r], -et);
   const double radix = ldexp(1.0, bits);
+#pragma unroll
   for (int l = 0; l < L; ++l) {
 #pragma unroll
     for (int r = 0; r < 64; ++r) {
       const int e = tid + 256 * r, row = e >> 7, col = e & 127;
       const double li = rint(v[r]);
-      tile[row][col] = (_Float16)(float)li;
+      tile[col][row] = (_Float16)(float)li;
       v[r] = (v[r] - li) * radix;
     }
     __syncthreads();
     _Float16 *dst = limbT + (int64_t)l * N1 * N2;
-#pragma unroll 8
-    for (int r = 0; r < 64; ++r) {
-      const int e = tid + 256 * r, c2 = e >> 7, r1 = e & 127;
-      dst[(int64_t)(n0 + c2) * N1 + m0 + r1] = tile[r1][c2];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int ch = tid + 256 * q, c2 = ch >> 4, g = ch & 15;
+      *reinterpret_cast<uint4 *>(dst + (int64_t)(n0 + c2) * N1 + m0 + 8 * g) =
+          *reinterpret_cast<const uint4 *>(&tile[c2][8 * g]);
     }
     __syncthreads();
   }
@@ -150,7 +153,7 @@ __global__ void k_split_w(const double *__restrict__ W, int64_t len, int KR, int
 template <int L>
 __global__ void __launch_bounds__(512) k_split_g1(int KLp, int N1, int N2, const _Float16 *__restrict__ aT,
                                                   const _Float16 *__restrict__ pT, const int32_t *__restrict__ texp,
-                                                  int bits, double *__restrict__ T) {
+                                                  int bits, int krows, double *__restrict__ T) {
   constexpr int BM = 256, BN = 64, BK = 32, PADK = BK + 8;
   __shared__ _Float16 As[2][BM][PADK];
   __shared__ _Float16 Bs[2][L][BN][PADK];
@@ -206,13 +209,16 @@ __global__ void __launch_bounds__(512) k_split_g1(int KLp, int N1, int N2, const
     }
   };
 
-  const int nsteps = N1 / BK;
-  gload(0);
+  // this block's K range: rows [kbase, kbase + krows) of x1 (multiple of 128)
+  const int kbase = blockIdx.z * krows;
+  T += (int64_t)blockIdx.z * KLp * N2;
+  const int nsteps = krows / BK;
+  gload(kbase);
   sstore(0);
   __syncthreads();
   for (int s = 0; s < nsteps; ++s) {
     const int buf = s & 1;
-    if (s + 1 < nsteps) gload((s + 1) * BK);
+    if (s + 1 < nsteps) gload(kbase + (s + 1) * BK);
 #pragma unroll
     for (int kk = 0; kk < BK / 16; ++kk) {
       const int kof = kk * 16 + 8 * (lane >> 5);
@@ -228,7 +234,7 @@ __global__ void __launch_bounds__(512) k_split_g1(int KLp, int N1, int N2, const
     }
     // promote after every 128 rows of x1 (4 steps): exact integers -> FP64
     if (((s + 1) * BK) % 128 == 0) {
-      const int et = texp[(s * BK / 128) * ntile2 + n0 / 128];
+      const int et = texp[((kbase + s * BK) / 128) * ntile2 + n0 / 128];
       double sc[L];
 #pragma unroll
       for (int l = 0; l < L; ++l) sc[l] = ldexp(1.0, et - bits * l);
@@ -275,6 +281,8 @@ __global__ void k_resid_g1(const int32_t *__restrict__ res, int nres, const doub
 
 // ----------------------------------------------------------------- G2
 struct G2Resid {
+  const uint8_t *blkA;     // N1/128: block of x1 rows has a residual
+  const uint8_t *blkB;     // N2/128: block of x2 rows (of Bbar) has a residual
   const int32_t *rowptrA;  // N1 + 1
   const int32_t *entA;     // (kappa, r) pairs
   const int32_t *rowptrB;  // N2 + 1
@@ -405,6 +413,7 @@ __global__ void __launch_bounds__(512) k_split_g2(int N1, int N2, int KL, int KL
 #pragma unroll
   for (int l = 0; l < L; ++l) sc[l] = ldexp(1.0, ev - bits * l);
   const double mu = ea.scal[kScalMu], dref = ea.scal[kScalDRef];
+  const bool hasA = rs.blkA[blockIdx.y] != 0, hasB = rs.blkB[blockIdx.x] != 0;
   double s0 = 0.0, m1 = -INFINITY, s2 = 0.0;
 #pragma unroll
   for (int j = 0; j < 2; ++j)
@@ -415,10 +424,12 @@ __global__ void __launch_bounds__(512) k_split_g2(int N1, int N2, int KL, int KL
       double S = 0.0;
 #pragma unroll
       for (int l = 0; l < L; ++l) S += (double)acc[l][j][r] * sc[l];
-      for (int e = rs.rowptrA[x1]; e < rs.rowptrA[x1 + 1]; ++e)
-        S += (double)rs.entA[2 * e + 1] * rs.V64[(int64_t)rs.entA[2 * e] * N2 + x2];
-      for (int e = rs.rowptrB[x2]; e < rs.rowptrB[x2 + 1]; ++e)
-        S += rs.W64[(int64_t)x1 * rs.KRp + rs.entB[2 * e]] * (double)rs.entB[2 * e + 1];
+      if (hasA)
+        for (int e = rs.rowptrA[x1]; e < rs.rowptrA[x1 + 1]; ++e)
+          S += (double)rs.entA[2 * e + 1] * rs.V64[(int64_t)rs.entA[2 * e] * N2 + x2];
+      if (hasB)
+        for (int e = rs.rowptrB[x2]; e < rs.rowptrB[x2 + 1]; ++e)
+          S += rs.W64[(int64_t)x1 * rs.KRp + rs.entB[2 * e]] * (double)rs.entB[2 * e + 1];
       const int64_t x = (int64_t)x1 * ea.ldx + x2;
       const double D = ea.Hc[x] - mu - S * ea.inv_c;
       m1 = fmax(m1, D);
@@ -481,10 +492,11 @@ int launch_split_vw(const double *V, int KL, int KLp, int N2, const double *W, i
 }
 
 int launch_split_g1(int KLp, int N1, int N2, const _Float16 *aT, const _Float16 *pT, const int32_t *texp, int limbs,
-                    int bits, double *T, hipStream_t s) {
-  if (N1 % 128 || N2 % 128) return BDC_ERR_ARG;
-  dim3 grid(N2 / 64, (KLp + 255) / 256);
-#define LG(LL) hipLaunchKernelGGL(k_split_g1<LL>, grid, dim3(512), 0, s, KLp, N1, N2, aT, pT, texp, bits, T)
+                    int bits, int ksplit, double *T, hipStream_t s) {
+  if (N1 % 128 || N2 % 128 || ksplit < 1 || (N1 / 128) % ksplit) return BDC_ERR_ARG;
+  const int krows = N1 / ksplit;
+  dim3 grid(N2 / 64, (KLp + 255) / 256, ksplit);
+#define LG(LL) hipLaunchKernelGGL(k_split_g1<LL>, grid, dim3(512), 0, s, KLp, N1, N2, aT, pT, texp, bits, krows, T)
   switch (limbs) {
     case 2: LG(2); break;
     case 3: LG(3); break;
@@ -503,10 +515,11 @@ int launch_resid_g1(const int32_t *res, int nres, const double *P, int N2, doubl
 
 int launch_split_g2(int N1, int N2, int KL, int KLp, int KR, int KRp, const _Float16 *A16, const _Float16 *vT,
                     const _Float16 *wl, const _Float16 *B16, int limbs, int bits, int epi, const EpiArgs &ea,
-                    const int32_t *rowptrA, const int32_t *entA, const int32_t *rowptrB, const int32_t *entB,
-                    const double *V64, const double *W64, hipStream_t s, int *nb) {
+                    const uint8_t *blkA, const uint8_t *blkB, const int32_t *rowptrA, const int32_t *entA,
+                    const int32_t *rowptrB, const int32_t *entB, const double *V64, const double *W64, hipStream_t s,
+                    int *nb) {
   if (N1 % 128 || N2 % 128) return BDC_ERR_ARG;
-  G2Resid rs{rowptrA, entA, rowptrB, entB, V64, W64, KRp};
+  G2Resid rs{blkA, blkB, rowptrA, entA, rowptrB, entB, V64, W64, KRp};
   dim3 grid(N2 / 128, N1 / 128);
   if (nb) *nb = (int)(grid.x * grid.y);
 #define LG2(LL, EE) hipLaunchKernelGGL((k_split_g2<LL, EE>), grid, dim3(512), 0, s, N1, N2, KL, KLp, KR, KRp, A16, vT, wl, B16, bits, ea, rs)

@@ -86,4 +86,26 @@ int launch_ins_gather(const double *in, int64_t irs, int64_t ics, double *out, i
   return hipGetLastError() == hipSuccess ? BDC_OK : BDC_ERR_HIP;
 }
 
+// Wbar from Vbar by reversal (n1 = n2, L reversal-symmetric):
+//   W_j[x1][z] = V_{k-j}[rev z][rev x1]
+// (substitute y1 -> rev y1 in sum_y1 c(x1,y1) L(y1 z) and use c(rev u, rev v) = c(u, v),
+//  L(y1 z) = L(rev z rev y1)).  One launch covers every j in JR.
+__global__ void k_w_from_v(MirrorW mw, const double *__restrict__ V, int N2, int n2, double *__restrict__ W, int N1,
+                           int KR, int KRp) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)N1 * KR) return;
+  const int x1 = (int)(i / KR), kap = (int)(i % KR);
+  int s = 0;
+  while (s + 1 < mw.count && kap >= mw.offR[s + 1]) ++s;
+  const int z = kap - mw.offR[s], m = mw.m[s];
+  const uint32_t rz = rev_bits((uint32_t)z, m), rx = rev_bits((uint32_t)x1, n2);
+  W[(int64_t)x1 * KRp + kap] = V[(int64_t)(mw.offL[s] + rz) * N2 + rx];
+}
+int launch_w_from_v(const MirrorW &mw, const double *V, int N2, int n2, double *W, int N1, int KR, int KRp,
+                    hipStream_t s) {
+  const int64_t tot = (int64_t)N1 * KR;
+  hipLaunchKernelGGL(k_w_from_v, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, mw, V, N2, n2, W, N1, KR, KRp);
+  return hipGetLastError() == hipSuccess ? BDC_OK : BDC_ERR_HIP;
+}
+
 }  // namespace bdc
