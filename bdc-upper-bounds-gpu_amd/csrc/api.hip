@@ -408,7 +408,7 @@ static int output_law(bdc_ctx *c, int prec, const double *X, int scale_slot, boo
     const int d = c->n2 - m;  // symbols B_m deletes
     if (d == 0) {
       BDC_LAUNCH(launch_copy2d(Tj, N2, 1, dst, 1 << m, 1, 1 << j, 1 << m, c->stream));
-    } else if (d <= kMaxGatherDeletions) {
+    } else if (d <= kMaxGatherInsertions) {
       BDC_LAUNCH(launch_ins_gather(Tj, N2, 1, dst, 1 << m, 1, 1 << j, c->n2, d, c->stream));
     } else {
       const Opnd A{Tj, N2, 1, 0}, B{c->c2(m), 1 << m, 1, 1};
@@ -437,7 +437,7 @@ static int output_law(bdc_ctx *c, int prec, const double *X, int scale_slot, boo
       if (d == 0) {
         // q_j[y1][z] = U[off + z][y1]
         BDC_LAUNCH(launch_copy2d(Uj, 1, N1, dst, 1 << m, 1, 1 << j, 1 << m, c->stream));
-      } else if (d <= kMaxGatherDeletions) {
+      } else if (d <= kMaxGatherInsertions) {
         BDC_LAUNCH(launch_ins_gather(Uj, N1, 1, dst, 1, 1 << m, 1 << m, c->n1, d, c->stream));
       } else {
         const Opnd A{c->c1(j), 1, 1 << j, 1}, B{Uj, 1, N1, 0};
@@ -547,6 +547,7 @@ static int ba_step(bdc_ctx *c, int prec, int idx) {
   }
   int nb = 0;
   BDC_TRY(cross_term(c, prec, EPI_BA, nullptr, &nb));
+  if (prec == BDC_PREC_FP64) c->plimb_valid = false;  // P changed without new limbs
   BDC_LAUNCH(launch_ba_finalize(c->part.as<double>(), nb, c->scal.as<double>(), c->hist.as<double>(), idx, c->stream));
   return BDC_OK;
 }
@@ -582,6 +583,7 @@ int bdc_set_p_uniform(bdc_ctx *c) {
   BDC_LAUNCH(launch_fill(c->P.as<double>(), c->nx, 1.0 / (double)c->nx, c->stream));  // main_BAA.cu:64
   BDC_TRY(reset_scalars(c, 1.0));
   c->have_p = true;
+  c->plimb_valid = false;
   return BDC_OK;
 }
 
@@ -604,6 +606,7 @@ int bdc_set_p_orbits(bdc_ctx *c, const double *q, uint64_t len) {
   d.release();
   BDC_TRY(reset_scalars(c, 1.0));
   c->have_p = true;
+  c->plimb_valid = false;
   return BDC_OK;
 }
 
@@ -639,6 +642,7 @@ int bdc_set_p_dirichlet(bdc_ctx *c, double alpha, uint64_t seed) {
   BDC_HIP(hipStreamSynchronize(c->stream));
   BDC_TRY(reset_scalars(c, 1.0));
   c->have_p = true;
+  c->plimb_valid = false;
   return BDC_OK;
 }
 
@@ -976,7 +980,7 @@ int split_prepare(bdc_ctx *c) {
   BDC_HIP(hipMemcpyAsync(c->blkB.p, fB.data(), fB.size(), hipMemcpyHostToDevice, c->stream));
   // split-K of G1 so that at least ~512 blocks run
   {
-    const int blocks = (int)(N2 / 64) * (int)((KLp + 255) / 256);
+    const int blocks = (int)(N2 / 32) * (int)((KLp + 255) / 256);
     int ks = 1;
     while (blocks * ks < 512 && (N1 / 128) % (ks * 2) == 0 && N1 / (ks * 2) >= 1024) ks *= 2;
     c->g1_ksplit = ks;
@@ -1014,8 +1018,10 @@ int split_prepare(bdc_ctx *c) {
 
 int split_output_law_g1(bdc_ctx *c, const double *X) {
   const Split &sp = c->split;
-  BDC_LAUNCH(launch_split_p(X, c->scal.as<double>(), c->N1, c->N2, sp.g1_limbs, sp.g1_bits, c->plimb.as<_Float16>(),
-                            c->ptexp.as<int32_t>(), c->stream));
+  if (!(X == c->P.as<double>() && c->plimb_valid))
+    BDC_LAUNCH(launch_split_p(X, c->scal.as<double>(), c->N1, c->N2, sp.g1_limbs, sp.g1_bits, c->plimb.as<_Float16>(),
+                              c->ptexp.as<int32_t>(), c->stream));
+  c->plimb_valid = (X == c->P.as<double>());
   const int ks = c->g1_ksplit;
   double *dstT = ks > 1 ? c->g1part.as<double>() : c->T.as<double>();
   BDC_LAUNCH(launch_split_g1(c->KLp, c->N1, c->N2, c->abarTh.as<_Float16>(), c->plimb.as<_Float16>(),
@@ -1038,10 +1044,15 @@ int split_cross_operands(bdc_ctx *c, const double *L, bool mirror) {
 
 int split_cross_term(bdc_ctx *c, int epi, const EpiArgs &ea, int *nb) {
   const Split &sp = c->split;
+  // the BA update writes the next step's P limbs in its epilogue
+  const bool fuse = (epi == EPI_BA) && ea.P == c->P.as<double>();
   BDC_LAUNCH(launch_split_g2(c->N1, c->N2, c->KL, c->KLp, c->KR, c->KRp, c->abarh.as<_Float16>(), c->vlimb.as<_Float16>(),
                              c->wlimb.as<_Float16>(), c->bbarh.as<_Float16>(), sp.g2_limbs, sp.g2_bits, epi, ea,
-                             c->blkA.as<uint8_t>(), c->blkB.as<uint8_t>(), c->rowptrA.as<int32_t>(), c->entA.as<int32_t>(), c->rowptrB.as<int32_t>(),
-                             c->entB.as<int32_t>(), c->g2b.as<double>(), c->g2a.as<double>(), c->stream, nb));
+                             c->blkA.as<uint8_t>(), c->blkB.as<uint8_t>(), c->rowptrA.as<int32_t>(), c->entA.as<int32_t>(),
+                             c->rowptrB.as<int32_t>(), c->entB.as<int32_t>(), c->g2b.as<double>(), c->g2a.as<double>(),
+                             fuse ? c->plimb.as<_Float16>() : nullptr, fuse ? c->ptexp.as<int32_t>() : nullptr,
+                             sp.g1_limbs, sp.g1_bits, c->stream, nb));
+  if (epi == EPI_BA) c->plimb_valid = fuse;
   return BDC_OK;
 }
 

@@ -146,6 +146,7 @@ struct bdc_ctx {
   bdc::DevBuf blkA, blkB;  // per 128-block flags: any residual in these rows
   bdc::DevBuf g1part;      // split-K partial slabs of Tbar
   int g1_ksplit = 1;
+  bool plimb_valid = false;  // plimb/ptexp hold the limbs of the current P
   int n_resid_a = 0, n_resid_b = 0;
   double g1_chunk_sum = 0, g2_row_sum = 0;  // exactness bounds (max count mass per FP32 accumulation)
 

@@ -86,7 +86,8 @@ int launch_del_gather(const double *in, int64_t irs, int64_t ics, double *out, i
                       int d, hipStream_t s);
 int launch_ins_gather(const double *in, int64_t irs, int64_t ics, double *out, int64_t ors, int64_t ocs, int R, int nb,
                       int d, hipStream_t s);
-constexpr int kMaxGatherDeletions = 3;
+constexpr int kMaxGatherDeletions = 3;   // deletion form: C(n,d) terms per output
+constexpr int kMaxGatherInsertions = 2;  // insertion form: 2^d C(n,d) terms per output
 struct MirrorW {  // for each j in JR: column offset in Wbar, m = k - j, row offset of V_{k-j} in Vbar
   int count;
   int offR[18], m[18], offL[18];
@@ -105,7 +106,7 @@ int launch_resid_g1(const int32_t *res, int nres, const double *P, int N2, doubl
 int launch_split_g2(int N1, int N2, int KL, int KLp, int KR, int KRp, const _Float16 *A16, const _Float16 *vT,
                     const _Float16 *wl, const _Float16 *B16, int limbs, int bits, int epi, const EpiArgs &ea,
                     const uint8_t *blkA, const uint8_t *blkB, const int32_t *rowptrA, const int32_t *entA,
-                    const int32_t *rowptrB, const int32_t *entB, const double *V64, const double *W64, hipStream_t s,
-                    int *nb);
+                    const int32_t *rowptrB, const int32_t *entB, const double *V64, const double *W64,
+                    _Float16 *limbT_next, int32_t *texp_next, int limbs_p, int bits_p, hipStream_t s, int *nb);
 
 }  // namespace bdc

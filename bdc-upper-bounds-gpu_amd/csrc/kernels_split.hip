@@ -33,19 +33,19 @@ __device__ inline int tile_exponent(double mx, int bits) {
 // transposed ([l][x2][x1]) so that the output-law contraction reads K = x1
 // contiguously.
 template <int L>
-__global__ void __launch_bounds__(256) k_split_p(const double *__restrict__ P, const double *__restrict__ scal, int N1,
-                                                 int N2, int bits, _Float16 *__restrict__ limbT,
-                                                 int32_t *__restrict__ texp) {
+__global__ void __launch_bounds__(1024) k_split_p(const double *__restrict__ P, const double *__restrict__ scal, int N1,
+                                                  int N2, int bits, _Float16 *__restrict__ limbT,
+                                                  int32_t *__restrict__ texp) {
   constexpr int PITCH = 136;  // halves per LDS row: 16-byte aligned rows
   __shared__ __attribute__((aligned(16))) _Float16 tile[128][PITCH];  // [x2][x1]
   __shared__ double red[16];
   const int tid = threadIdx.x;
   const int m0 = blockIdx.y * 128, n0 = blockIdx.x * 128;
-  double v[64];
+  double v[16];
   double mx = 0.0;
 #pragma unroll
-  for (int r = 0; r < 64; ++r) {
-    const int e = tid + 256 * r, row = e >> 7, col = e & 127;
+  for (int r = 0; r < 16; ++r) {
+    const int e = tid + 1024 * r, row = e >> 7, col = e & 127;
     v[r] = P[(int64_t)(m0 + row) * N2 + n0 + col];
     mx = fmax(mx, fabs(v[r]));
   }
@@ -53,13 +53,12 @@ __global__ void __launch_bounds__(256) k_split_p(const double *__restrict__ P, c
   const int et = tile_exponent(mx, bits);
   if (tid == 0) texp[blockIdx.y * gridDim.x + blockIdx.x] = et;
 #pragma unroll
-  for (int r = 0; r < 64; ++r) v[r] = ldexp(v[r], -et);
+  for (int r = 0; r < 16; ++r) v[r] = ldexp(v[r], -et);
   const double radix = ldexp(1.0, bits);
-#pragma unroll
   for (int l = 0; l < L; ++l) {
 #pragma unroll
-    for (int r = 0; r < 64; ++r) {
-      const int e = tid + 256 * r, row = e >> 7, col = e & 127;
+    for (int r = 0; r < 16; ++r) {
+      const int e = tid + 1024 * r, row = e >> 7, col = e & 127;
       const double li = rint(v[r]);
       tile[col][row] = (_Float16)(float)li;
       v[r] = (v[r] - li) * radix;
@@ -67,8 +66,8 @@ __global__ void __launch_bounds__(256) k_split_p(const double *__restrict__ P, c
     __syncthreads();
     _Float16 *dst = limbT + (int64_t)l * N1 * N2;
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const int ch = tid + 256 * q, c2 = ch >> 4, g = ch & 15;
+    for (int q = 0; q < 2; ++q) {
+      const int ch = tid + 1024 * q, c2 = ch >> 4, g = ch & 15;
       *reinterpret_cast<uint4 *>(dst + (int64_t)(n0 + c2) * N1 + m0 + 8 * g) =
           *reinterpret_cast<const uint4 *>(&tile[c2][8 * g]);
     }
@@ -147,35 +146,31 @@ __global__ void k_split_w(const double *__restrict__ W, int64_t len, int KR, int
 
 // ----------------------------------------------------------------- G1
 // Tbar[kappa][x2] = sum_x1 Abar[x1][kappa] P[x1][x2] with P in L limbs.
-// Block: 256 (kappa) x 64 (x2), 8 waves of 64 x 32, K step 32 through LDS
-// (double-buffered), FP32 accumulators promoted to FP64 every 128 rows of x1
-// (one P tile row -> one exponent).
+// Block: 256 (kappa) x 32 (x2); 8 waves, each 32 kappa x 32 x2 and all L limbs
+// (64 FP32 accumulators + 16 FP64 per lane).  K step 32 through LDS,
+// double-buffered; FP32 sums promoted to FP64 every 128 rows of x1 (one P tile
+// row -> one exponent).  blockIdx.z splits K.
 template <int L>
 __global__ void __launch_bounds__(512) k_split_g1(int KLp, int N1, int N2, const _Float16 *__restrict__ aT,
                                                   const _Float16 *__restrict__ pT, const int32_t *__restrict__ texp,
                                                   int bits, int krows, double *__restrict__ T) {
-  constexpr int BM = 256, BN = 64, BK = 32, PADK = BK + 8;
-  __shared__ _Float16 As[2][BM][PADK];
-  __shared__ _Float16 Bs[2][L][BN][PADK];
+  constexpr int BM = 256, BN = 32, BK = 32, PADK = BK + 8;
+  __shared__ __attribute__((aligned(16))) _Float16 As[2][BM][PADK];
+  __shared__ __attribute__((aligned(16))) _Float16 Bs[2][L][BN][PADK];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = wave >> 1, wn = wave & 1;
   const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
   const int ntile2 = N2 / 128;
 
-  float16_ acc[L][2];
-  double acc64[2][16];
+  float16_ acc[L];
+  double acc64[16];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int r = 0; r < 16; ++r) acc64[r] = 0.0;
 #pragma unroll
-    for (int r = 0; r < 16; ++r) acc64[i][r] = 0.0;
-#pragma unroll
-  for (int l = 0; l < L; ++l)
-#pragma unroll
-    for (int i = 0; i < 2; ++i) acc[l][i] = (float16_){0};
+  for (int l = 0; l < L; ++l) acc[l] = (float16_){0};
 
-  // global -> register staging: A 1024 chunks of 16 B, B L*256 chunks
-  constexpr int ACH = BM * BK / 8 / 512;          // 2
-  constexpr int BCH = (L * BN * BK / 8 + 511) / 512;  // L/2
+  constexpr int ACH = BM * BK / 8 / 512;              // 2 chunks of 16 B per thread
+  constexpr int BTOT = L * BN * BK / 8;               // L * 128 chunks
+  constexpr int BCH = (BTOT + 511) / 512;             // 1
   uint4 ra[ACH], rb[BCH];
   auto gload = [&](int k0) {
 #pragma unroll
@@ -187,7 +182,7 @@ __global__ void __launch_bounds__(512) k_split_g1(int KLp, int N1, int N2, const
 #pragma unroll
     for (int c = 0; c < BCH; ++c) {
       const int ch = tid + 512 * c;
-      if (ch < L * BN * BK / 8) {
+      if (ch < BTOT) {
         const int l = ch / (BN * BK / 8), rem = ch % (BN * BK / 8), row = rem >> 2, kq = rem & 3;
         rb[c] = *reinterpret_cast<const uint4 *>(pT + (int64_t)l * N1 * N2 + (int64_t)(n0 + row) * N1 + k0 + kq * 8);
       }
@@ -202,14 +197,13 @@ __global__ void __launch_bounds__(512) k_split_g1(int KLp, int N1, int N2, const
 #pragma unroll
     for (int c = 0; c < BCH; ++c) {
       const int ch = tid + 512 * c;
-      if (ch < L * BN * BK / 8) {
+      if (ch < BTOT) {
         const int l = ch / (BN * BK / 8), rem = ch % (BN * BK / 8), row = rem >> 2, kq = rem & 3;
         *reinterpret_cast<uint4 *>(&Bs[buf][l][row][kq * 8]) = rb[c];
       }
     }
   };
 
-  // this block's K range: rows [kbase, kbase + krows) of x1 (multiple of 128)
   const int kbase = blockIdx.z * krows;
   T += (int64_t)blockIdx.z * KLp * N2;
   const int nsteps = krows / BK;
@@ -222,35 +216,28 @@ __global__ void __launch_bounds__(512) k_split_g1(int KLp, int N1, int N2, const
 #pragma unroll
     for (int kk = 0; kk < BK / 16; ++kk) {
       const int kof = kk * 16 + 8 * (lane >> 5);
-      half8 a[2], b[L];
+      const half8 a = *reinterpret_cast<const half8 *>(&As[buf][wave * 32 + (lane & 31)][kof]);
 #pragma unroll
-      for (int i = 0; i < 2; ++i) a[i] = *reinterpret_cast<const half8 *>(&As[buf][wm * 64 + i * 32 + (lane & 31)][kof]);
-#pragma unroll
-      for (int l = 0; l < L; ++l) b[l] = *reinterpret_cast<const half8 *>(&Bs[buf][l][wn * 32 + (lane & 31)][kof]);
-#pragma unroll
-      for (int l = 0; l < L; ++l)
-#pragma unroll
-        for (int i = 0; i < 2; ++i) acc[l][i] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[i], b[l], acc[l][i], 0, 0, 0);
+      for (int l = 0; l < L; ++l) {
+        const half8 b = *reinterpret_cast<const half8 *>(&Bs[buf][l][lane & 31][kof]);
+        acc[l] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, acc[l], 0, 0, 0);
+      }
     }
-    // promote after every 128 rows of x1 (4 steps): exact integers -> FP64
+    // promote after every 128 rows of x1: exact integers -> FP64
     if (((s + 1) * BK) % 128 == 0) {
       const int et = texp[((kbase + s * BK) / 128) * ntile2 + n0 / 128];
       double sc[L];
 #pragma unroll
       for (int l = 0; l < L; ++l) sc[l] = ldexp(1.0, et - bits * l);
 #pragma unroll
-      for (int i = 0; i < 2; ++i)
+      for (int r = 0; r < 16; ++r) {
+        double t = 0.0;
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-          double t = 0.0;
+        for (int l = 0; l < L; ++l) t += (double)acc[l][r] * sc[l];
+        acc64[r] += t;
+      }
 #pragma unroll
-          for (int l = 0; l < L; ++l) t += (double)acc[l][i][r] * sc[l];
-          acc64[i][r] += t;
-        }
-#pragma unroll
-      for (int l = 0; l < L; ++l)
-#pragma unroll
-        for (int i = 0; i < 2; ++i) acc[l][i] = (float16_){0};
+      for (int l = 0; l < L; ++l) acc[l] = (float16_){0};
     }
     if (s + 1 < nsteps) {
       sstore(buf ^ 1);
@@ -258,13 +245,11 @@ __global__ void __launch_bounds__(512) k_split_g1(int KLp, int N1, int N2, const
     }
   }
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      const int row = m0 + wm * 64 + i * 32 + mfma32_row(lane, r);
-      const int col = n0 + wn * 32 + (lane & 31);
-      if (row < KLp) T[(int64_t)row * N2 + col] = acc64[i][r];
-    }
+  for (int r = 0; r < 16; ++r) {
+    const int row = m0 + wave * 32 + mfma32_row(lane, r);
+    const int col = n0 + (lane & 31);
+    if (row < KLp) T[(int64_t)row * N2 + col] = acc64[r];
+  }
 }
 
 // T[kappa][x2] += r * P[x1][x2] for the residuals of the rounded counts; one
@@ -293,21 +278,43 @@ struct G2Resid {
 };
 
 // S[x1][x2] = sum_kappa A16[x1][kappa] V[kappa][x2] + sum_kappa' W[x1][kappa'] B16[x2][kappa']
-// with V, W in L limbs of one global scale.  Block 128 x 128, 8 waves of 32 x 64.
+// with V, W in L limbs of one global scale.  Block 128 x 128 (one P tile),
+// 8 waves of 32 x 64.  Blocks are mapped to tiles XCD by XCD: the 8 XCDs each
+// sweep their own 8 x 8 patches of tiles so the operand rows a patch needs
+// (about 3.75 MiB) stay in that XCD's L2.
+// Epilogue: the BA update (or D), and with fuse_limbs the next step's P limbs
+// and tile exponent, written transposed through LDS.
+struct G2Out {
+  _Float16 *limbT;  // next P limbs [L][N2][N1] (fused BA epilogue) or null
+  int32_t *texp;    // tile exponents
+  int limbs_p, bits_p;
+  int patches;      // 1: XCD patch mapping
+};
 template <int L, int EPI>
 __global__ void __launch_bounds__(512) k_split_g2(int N1, int N2, int KL, int KLp, int KR, int KRp,
                                                   const _Float16 *__restrict__ A16, const _Float16 *__restrict__ vT,
                                                   const _Float16 *__restrict__ wl, const _Float16 *__restrict__ B16,
-                                                  int bits, EpiArgs ea, G2Resid rs) {
+                                                  int bits, EpiArgs ea, G2Resid rs, G2Out go) {
   constexpr int BM = 128, BN = 128, BK = 32, PADK = BK + 8;
   static_assert(BM == BN, "slabs are shared between the A and B roles");
   // L + 1 slabs of 128 x 32 per stage: phase 1 uses slab 0 for A16 and slabs
   // 1..L for the V limbs; phase 2 uses slabs 0..L-1 for the W limbs and slab L for B16.
-  __shared__ _Float16 Sl[2][L + 1][BM][PADK];
+  __shared__ __attribute__((aligned(16))) _Float16 Sl[2][L + 1][BM][PADK];
   __shared__ double red[48];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
-  const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+  const int TX = N2 / BN;
+  int ty, tx;
+  if (go.patches) {
+    const int b = blockIdx.x, xcd = b & 7, sq = b >> 3;
+    const int patch = (sq >> 6) * 8 + xcd, w = sq & 63;
+    ty = (patch / (TX >> 3)) * 8 + (w >> 3);
+    tx = (patch % (TX >> 3)) * 8 + (w & 7);
+  } else {
+    ty = blockIdx.x / TX;
+    tx = blockIdx.x % TX;
+  }
+  const int m0 = ty * BM, n0 = tx * BN;
 
   float16_ acc[L][2];
 #pragma unroll
@@ -360,12 +367,12 @@ __global__ void __launch_bounds__(512) k_split_g2(int N1, int N2, int KL, int KL
             acc[l][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, acc[l][j], 0, 0, 0);
           }
       }
-      __syncthreads();
       if (s + 1 < nst) {
         st(buf ^ 1);
         __syncthreads();
       }
     }
+    __syncthreads();
   }
   // ---- phase 2: W limbs x B16 (shared)
   if (KR > 0) {
@@ -399,39 +406,53 @@ __global__ void __launch_bounds__(512) k_split_g2(int N1, int N2, int KL, int KL
           for (int j = 0; j < 2; ++j) acc[l][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b[j], acc[l][j], 0, 0, 0);
         }
       }
-      __syncthreads();
       if (s + 1 < nst) {
         st(buf ^ 1);
         __syncthreads();
       }
     }
+    __syncthreads();
   }
 
-  // ---- epilogue: combine limbs in FP64, residual corrections, BA update / D
+  // ---- epilogue pass 1: combine limbs into S (FP64) and apply residuals
   const int ev = (int)ea.scal[kScalVScale];
   double sc[L];
 #pragma unroll
   for (int l = 0; l < L; ++l) sc[l] = ldexp(1.0, ev - bits * l);
+  double Sv[2][16];
+#pragma unroll
+  for (int j = 0; j < 2; ++j)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      double S = 0.0;
+#pragma unroll
+      for (int l = 0; l < L; ++l) S += (double)acc[l][j][r] * sc[l];
+      Sv[j][r] = S;
+    }
+  if (rs.blkA[ty] || rs.blkB[tx]) {
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int x1 = m0 + wm * 32 + mfma32_row(lane, r);
+        const int x2 = n0 + wn * 64 + j * 32 + (lane & 31);
+        for (int e = rs.rowptrA[x1]; e < rs.rowptrA[x1 + 1]; ++e)
+          Sv[j][r] += (double)rs.entA[2 * e + 1] * rs.V64[(int64_t)rs.entA[2 * e] * N2 + x2];
+        for (int e = rs.rowptrB[x2]; e < rs.rowptrB[x2 + 1]; ++e)
+          Sv[j][r] += rs.W64[(int64_t)x1 * rs.KRp + rs.entB[2 * e]] * (double)rs.entB[2 * e + 1];
+      }
+  }
+  // ---- epilogue pass 2: D, the update, block partials
   const double mu = ea.scal[kScalMu], dref = ea.scal[kScalDRef];
-  const bool hasA = rs.blkA[blockIdx.y] != 0, hasB = rs.blkB[blockIdx.x] != 0;
-  double s0 = 0.0, m1 = -INFINITY, s2 = 0.0;
+  double s0 = 0.0, m1 = -INFINITY, s2 = 0.0, wmax = 0.0;
 #pragma unroll
   for (int j = 0; j < 2; ++j)
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
       const int x1 = m0 + wm * 32 + mfma32_row(lane, r);
       const int x2 = n0 + wn * 64 + j * 32 + (lane & 31);
-      double S = 0.0;
-#pragma unroll
-      for (int l = 0; l < L; ++l) S += (double)acc[l][j][r] * sc[l];
-      if (hasA)
-        for (int e = rs.rowptrA[x1]; e < rs.rowptrA[x1 + 1]; ++e)
-          S += (double)rs.entA[2 * e + 1] * rs.V64[(int64_t)rs.entA[2 * e] * N2 + x2];
-      if (hasB)
-        for (int e = rs.rowptrB[x2]; e < rs.rowptrB[x2 + 1]; ++e)
-          S += rs.W64[(int64_t)x1 * rs.KRp + rs.entB[2 * e]] * (double)rs.entB[2 * e + 1];
       const int64_t x = (int64_t)x1 * ea.ldx + x2;
-      const double D = ea.Hc[x] - mu - S * ea.inv_c;
+      const double D = ea.Hc[x] - mu - Sv[j][r] * ea.inv_c;
       m1 = fmax(m1, D);
       if constexpr (EPI == EPI_BA) {
         const double p = ea.P[x];
@@ -439,16 +460,54 @@ __global__ void __launch_bounds__(512) k_split_g2(int N1, int N2, int KL, int KL
         ea.P[x] = w;
         s0 += w;
         s2 += p * D;
+        Sv[j][r] = w;
+        wmax = fmax(wmax, w);
       } else {
         ea.D[x] = D;
       }
     }
   block_reduce3(s0, m1, s2, red);
   if (tid == 0) {
-    double *pp = ea.part + (int64_t)(blockIdx.y * gridDim.x + blockIdx.x) * kPartStrideDev;
+    double *pp = ea.part + (int64_t)(ty * TX + tx) * kPartStrideDev;
     pp[0] = s0;
     pp[1] = m1;
     pp[2] = s2;
+  }
+  // ---- epilogue pass 3 (BA, fused): limbs of the new P tile for the next output law
+  if constexpr (EPI == EPI_BA) {
+    if (go.limbT) {
+      wmax = block_max(wmax, red + 16);
+      const int et = tile_exponent(wmax, go.bits_p);
+      if (tid == 0) go.texp[ty * TX + tx] = et;
+      constexpr int PITCH = 136;
+      _Float16(*tileT)[PITCH] = reinterpret_cast<_Float16(*)[PITCH]>(&Sl[0][0][0][0]);  // 128 x 136, 34 KB
+      const double radix = ldexp(1.0, go.bits_p);
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) Sv[j][r] = ldexp(Sv[j][r], -et);
+      for (int l = 0; l < go.limbs_p; ++l) {
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+          for (int r = 0; r < 16; ++r) {
+            const int xl1 = wm * 32 + mfma32_row(lane, r);
+            const int xl2 = wn * 64 + j * 32 + (lane & 31);
+            const double li = rint(Sv[j][r]);
+            tileT[xl2][xl1] = (_Float16)(float)li;
+            Sv[j][r] = (Sv[j][r] - li) * radix;
+          }
+        __syncthreads();
+        _Float16 *dst = go.limbT + (int64_t)l * N1 * N2;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int ch = tid + 512 * q, c2 = ch >> 4, g = ch & 15;
+          *reinterpret_cast<uint4 *>(dst + (int64_t)(n0 + c2) * N1 + m0 + 8 * g) =
+              *reinterpret_cast<const uint4 *>(&tileT[c2][8 * g]);
+        }
+        __syncthreads();
+      }
+    }
   }
 }
 
@@ -466,7 +525,7 @@ int launch_split_p(const double *P, const double *scal, int N1, int N2, int limb
                    int32_t *texp, hipStream_t s) {
   if (N1 % 128 || N2 % 128) return BDC_ERR_ARG;
   dim3 grid(N2 / 128, N1 / 128);
-#define LP(LL) hipLaunchKernelGGL(k_split_p<LL>, grid, dim3(256), 0, s, P, scal, N1, N2, bits, limbT, texp)
+#define LP(LL) hipLaunchKernelGGL(k_split_p<LL>, grid, dim3(1024), 0, s, P, scal, N1, N2, bits, limbT, texp)
   BDC_SPLIT_L(limbs, LP)
 #undef LP
   return hipGetLastError() == hipSuccess ? BDC_OK : BDC_ERR_HIP;
@@ -495,7 +554,7 @@ int launch_split_g1(int KLp, int N1, int N2, const _Float16 *aT, const _Float16 
                     int bits, int ksplit, double *T, hipStream_t s) {
   if (N1 % 128 || N2 % 128 || ksplit < 1 || (N1 / 128) % ksplit) return BDC_ERR_ARG;
   const int krows = N1 / ksplit;
-  dim3 grid(N2 / 64, (KLp + 255) / 256, ksplit);
+  dim3 grid(N2 / 32, (KLp + 255) / 256, ksplit);
 #define LG(LL) hipLaunchKernelGGL(k_split_g1<LL>, grid, dim3(512), 0, s, KLp, N1, N2, aT, pT, texp, bits, krows, T)
   switch (limbs) {
     case 2: LG(2); break;
@@ -516,13 +575,16 @@ int launch_resid_g1(const int32_t *res, int nres, const double *P, int N2, doubl
 int launch_split_g2(int N1, int N2, int KL, int KLp, int KR, int KRp, const _Float16 *A16, const _Float16 *vT,
                     const _Float16 *wl, const _Float16 *B16, int limbs, int bits, int epi, const EpiArgs &ea,
                     const uint8_t *blkA, const uint8_t *blkB, const int32_t *rowptrA, const int32_t *entA,
-                    const int32_t *rowptrB, const int32_t *entB, const double *V64, const double *W64, hipStream_t s,
-                    int *nb) {
+                    const int32_t *rowptrB, const int32_t *entB, const double *V64, const double *W64,
+                    _Float16 *limbT_next, int32_t *texp_next, int limbs_p, int bits_p, hipStream_t s, int *nb) {
   if (N1 % 128 || N2 % 128) return BDC_ERR_ARG;
   G2Resid rs{blkA, blkB, rowptrA, entA, rowptrB, entB, V64, W64, KRp};
-  dim3 grid(N2 / 128, N1 / 128);
-  if (nb) *nb = (int)(grid.x * grid.y);
-#define LG2(LL, EE) hipLaunchKernelGGL((k_split_g2<LL, EE>), grid, dim3(512), 0, s, N1, N2, KL, KLp, KR, KRp, A16, vT, wl, B16, bits, ea, rs)
+  const int TX = N2 / 128, TY = N1 / 128;
+  G2Out go{limbT_next, texp_next, limbs_p, bits_p, (TX % 8 == 0 && TY % 8 == 0 && ((TX / 8) * (TY / 8)) % 8 == 0) ? 1 : 0};
+  if (limbT_next && (limbs_p < 1 || limbs_p > 4)) return BDC_ERR_ARG;
+  const int blocks = TX * TY;
+  if (nb) *nb = blocks;
+#define LG2(LL, EE) hipLaunchKernelGGL((k_split_g2<LL, EE>), dim3(blocks), dim3(512), 0, s, N1, N2, KL, KLp, KR, KRp, A16, vT, wl, B16, bits, ea, rs, go)
   if (epi == EPI_BA) {
     switch (limbs) {
       case 2: LG2(2, EPI_BA); break;

@@ -203,3 +203,20 @@ def test_split_building_blocks(bdc, oracle, n, k):
         d64 = e.kl(np.log(q), bdc.PREC_FP64)
         np.testing.assert_allclose(d16, d64, rtol=0, atol=1e-11)
     np.testing.assert_allclose(q64, oracle.full_output_law(n, k, w), rtol=1e-13)
+
+
+@pytest.mark.parametrize("n,k,iters", [(24, 12, 6), (22, 10, 6)])
+def test_split_matches_fp64_large(bdc, n, k, iters):
+    """Larger sizes (XCD patch mapping, fused limb epilogue) against the FP64 path."""
+    say(f"split vs fp64 n={n} k={k}")
+    with bdc.Engine(n, k) as e:
+        e.set_p_uniform()
+        s64 = e.iterate(iters, bdc.PREC_FP64)
+        c64 = e.certify()
+        e.set_p_uniform()
+        s16 = e.iterate(iters, bdc.PREC_SPLIT16)
+        c16 = e.certify()
+    assert np.abs(s16[:, 0] - s64[:, 0]).max() < 1e-9
+    assert np.abs(s16[:, 3] - s64[:, 3]).max() / LN2 < 1e-9
+    assert abs(c16["upper_cert"] - c64["upper_cert"]) / LN2 < 1e-9
+    assert abs(c16["lower"] - c64["lower"]) / LN2 < 1e-9
