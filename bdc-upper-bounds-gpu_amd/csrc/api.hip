@@ -299,6 +299,13 @@ int bdc_create(int n, int k, const char *cache_dir, bdc_ctx **out) {
   c->part_blocks = std::max<size_t>(4096, ((size_t)c->N1 / 64) * ((size_t)c->N2 / 64));
   BDC_TRY_C(c->part.alloc(c->part_blocks * kPartStride * 8));
   BDC_TRY_C(c->scal.alloc(kScalCount * 8));
+  BDC_TRY_C(c->exptab.alloc(32 * 8));
+  {
+    double t[32];
+    for (int j = 0; j < 32; ++j) t[j] = (double)exp2l((long double)j / 32.0L);
+    if (hipMemcpy(c->exptab.p, t, sizeof t, hipMemcpyHostToDevice) != hipSuccess)
+      return cleanup(fail(BDC_ERR_HIP, "exp table"));
+  }
   {
     double h[kScalCount] = {0};
     h[kScalPScale] = 1.0;
@@ -335,8 +342,8 @@ int bdc_destroy(bdc_ctx *c) {
   for (auto &b : c->cnt1) b.release();
   for (auto &b : c->cnt2) b.release();
   for (bdc::DevBuf *b : {&c->abar16, &c->bbar16, &c->abarh, &c->abarTh, &c->bbarh, &c->residA, &c->rowptrA,
-                         &c->entA, &c->rowptrB, &c->entB, &c->blkA, &c->blkB, &c->g1part, &c->repmask, &c->wordbase, &c->orbits, &c->P,
-                         &c->Hc, &c->T, &c->U, &c->qpart, &c->qv, &c->g2a, &c->g2b, &c->splitk, &c->part, &c->scal,
+                         &c->entA, &c->rowptrB, &c->entB, &c->blkA, &c->blkB, &c->restiles[0], &c->restiles[1], &c->g1part, &c->repmask, &c->wordbase, &c->orbits, &c->P,
+                         &c->Hc, &c->T, &c->U, &c->qpart, &c->qv, &c->g2a, &c->g2b, &c->splitk, &c->part, &c->scal, &c->exptab,
                          &c->hist, &c->plimb, &c->ptexp, &c->vlimb, &c->wlimb, &c->theta, &c->adam_m,
                          &c->adam_v, &c->grad, &c->dvec})
     b->release();
@@ -375,21 +382,25 @@ static int stage_gemm(bdc_ctx *c, int M, int N, int K, const Opnd &A, const Opnd
   return BDC_OK;
 }
 
-// q(y) and L(y) = ln q - mu from full-space weights X (p = scal[scale_slot] * X).
+// q(y) and L(y) = ln q - mu from weights X (p = scal[scale_slot] * X).
 // mirror: use q_j(y) = q_{k-j}(rev y) for j in JR (X reversal-symmetric, n even).
-static int output_law(bdc_ctx *c, int prec, const double *X, int scale_slot, bool mirror, double *qout, double *Lout) {
+// rows = N1/2: X holds only the half space x1 < N1/2 of a complement-symmetric
+// p; the contractions run over those rows and q_j(y) = R_j(y) + R_j(~y).
+static int output_law(bdc_ctx *c, int prec, const double *X, int scale_slot, bool mirror, int rows, double *qout,
+                      double *Lout) {
   const int k = c->k, N1 = c->N1, N2 = c->N2;
   const size_t ny = (size_t)1 << k;
   const Opnd none{nullptr, 0, 0, 0};
   if (c->n1 != c->n2) mirror = false;
-  // G1: Tbar = Abar^T X   (KL x N2, K = N1)
+  if (rows != N1 && !(mirror && rows == N1 / 2)) return fail(BDC_ERR_STATE, "half-space output law needs the mirror rule");
+  // G1: Tbar = Abar[rows]^T X   (KL x N2, K = rows)
   BDC_TRY(timer_begin(c, c->t_g1));
   if (prec == BDC_PREC_FP64) {
     const Opnd A{c->abar16.p, 1, c->KLp, 1}, B{X, N2, 1, 0};
-    BDC_LAUNCH(gemm64(c->KL, N2, A, B, N1, none, none, 0, c->T.as<double>(), N2, 1, 1, 0, EPI_STORE, nullptr, c->stream,
-                      nullptr));
+    BDC_LAUNCH(gemm64(c->KL, N2, A, B, rows, none, none, 0, c->T.as<double>(), N2, 1, 1, 0, EPI_STORE, nullptr,
+                      c->stream, nullptr));
   } else {
-    BDC_TRY(split_output_law_g1(c, X));
+    BDC_TRY(split_output_law_g1(c, X, rows));
   }
   BDC_TRY(timer_end(c, c->t_g1));
   if (!mirror && c->KR) {
@@ -420,6 +431,7 @@ static int output_law(bdc_ctx *c, int prec, const double *X, int scale_slot, boo
   qc.nL = (int)c->JL.size();
   qc.qL = qL;
   qc.scale_slot = scale_slot;
+  qc.comp = rows < N1 ? 1 : 0;
   if (mirror) {
     qc.mirror = 1;
     for (int j : c->JR) {
@@ -447,13 +459,14 @@ static int output_law(bdc_ctx *c, int prec, const double *X, int scale_slot, boo
     qc.nR = (int)c->JR.size();
     qc.qR = qR;
   }
-  BDC_LAUNCH(launch_q_combine(qc, c->inv_c, c->scal.as<double>(), qout, Lout, c->stream));
+  BDC_LAUNCH(launch_q_combine(qc, c->inv_c, c->scal.as<double>(), qout, Lout, c->part.as<double>(), c->stream));
   return BDC_OK;
 }
 
 // stage 1: Vbar rows (j in JL) and Wbar columns (j in JR) from L (FP64).
-// mirror: L is reversal-symmetric and n1 = n2, so Wbar is a permutation of Vbar.
-static int cross_operands(bdc_ctx *c, const double *L, bool mirror = false) {
+// mirror: L is reversal-symmetric and n1 = n2, so Wbar is a permutation of Vbar;
+// only the first `rows` rows of Wbar are formed then.
+static int cross_operands(bdc_ctx *c, const double *L, bool mirror = false, int rows = 0) {
   const int k = c->k, N1 = c->N1, N2 = c->N2;
   for (size_t i = 0; i < c->JL.size(); ++i) {
     const int j = c->JL[i], m = k - j;
@@ -481,9 +494,11 @@ static int cross_operands(bdc_ctx *c, const double *L, bool mirror = false) {
         if (c->JL[q] == m) mw.offL[i] = c->offL[q];
       if (mw.offL[i] < 0) return fail(BDC_ERR_STATE, "mirror slice for j=%d missing", c->JR[i]);
     }
-    BDC_LAUNCH(launch_w_from_v(mw, c->g2b.as<double>(), N2, c->n2, c->g2a.as<double>(), N1, c->KR, c->KRp, c->stream));
+    BDC_LAUNCH(launch_w_from_v(mw, c->g2b.as<double>(), N2, c->n2, c->g2a.as<double>(), rows > 0 ? rows : N1, c->KR,
+                               c->KRp, c->stream));
     return BDC_OK;
   }
+  if (rows > 0 && rows != N1) return fail(BDC_ERR_STATE, "half-space cross operands need the mirror rule");
   for (size_t i = 0; i < c->JR.size(); ++i) {
     const int j = c->JR[i], m = k - j;
     double *dst = c->g2a.as<double>() + c->offR[i];
@@ -502,8 +517,10 @@ static int cross_operands(bdc_ctx *c, const double *L, bool mirror = false) {
   return BDC_OK;
 }
 
-// G2 with the given epilogue; returns the number of partial blocks
-static int cross_term(bdc_ctx *c, int prec, int epi, double *Dout, int *nblocks) {
+// G2 with the given epilogue over the rows x1 < rows (N1 unless the half-space
+// step); returns the number of partial blocks
+static int cross_term(bdc_ctx *c, int prec, int epi, double *Dout, int *nblocks, int rows = 0) {
+  if (rows <= 0) rows = c->N1;
   EpiArgs ea{};
   ea.P = c->P.as<double>();
   ea.Hc = c->Hc.as<double>();
@@ -512,13 +529,14 @@ static int cross_term(bdc_ctx *c, int prec, int epi, double *Dout, int *nblocks)
   ea.scal = c->scal.as<double>();
   ea.inv_c = c->inv_c;
   ea.ldx = c->N2;
+  ea.exptab = c->exptab.as<double>();
   BDC_TRY(timer_begin(c, c->t_g2));
   if (prec == BDC_PREC_FP64) {
     const Opnd A1{c->abar16.p, c->KLp, 1, 1}, B1{c->g2b.p, c->N2, 1, 0};
     const Opnd A2{c->g2a.p, c->KRp, 1, 0}, B2{c->bbar16.p, 1, c->KRp, 1};
-    BDC_LAUNCH(gemm64(c->N1, c->N2, A1, B1, c->KL, A2, B2, c->KR, nullptr, 0, 0, 1, 0, epi, &ea, c->stream, nblocks));
+    BDC_LAUNCH(gemm64(rows, c->N2, A1, B1, c->KL, A2, B2, c->KR, nullptr, 0, 0, 1, 0, epi, &ea, c->stream, nblocks));
   } else {
-    BDC_TRY(split_cross_term(c, epi, ea, nblocks));
+    BDC_TRY(split_cross_term(c, epi, ea, nblocks, rows));
   }
   BDC_TRY(timer_end(c, c->t_g2));
   return BDC_OK;
@@ -536,19 +554,33 @@ static int ensure_dvec(bdc_ctx *c) {
   return c->dvec.alloc(c->nx * 8);
 }
 
-// one BA step on the device; stats land in hist[idx]
-static int ba_step(bdc_ctx *c, int prec, int idx) {
+// Rows of P a BA step updates.  p is complement-symmetric (every setter
+// expands orbits, and the update commutes with x -> ~x), so when the reversal
+// mirror applies (n1 = n2) the step runs on the half space x1 < N1/2 (the top
+// bit of x is 0) and the rows beyond are their complement images.  The split
+// path needs 128-row tiles of that half.  BDC_HALF_SPACE=0 forces the full space.
+static int ba_rows(bdc_ctx *c, int prec) {
+  const char *e = getenv("BDC_HALF_SPACE");
+  if (e && e[0] == '0') return c->N1;
+  if (c->n1 != c->n2 || c->N1 < 2) return c->N1;
+  if (prec == BDC_PREC_SPLIT16 && (c->N1 / 2) % 128) return c->N1;
+  return c->N1 / 2;
+}
+
+// one BA step on the device over the rows x1 < rows; stats land in hist[idx]
+static int ba_step(bdc_ctx *c, int prec, int idx, int rows) {
   double *q = c->qv.as<double>(), *L = q + ((size_t)1 << c->k);
-  BDC_TRY(output_law(c, prec, c->P.as<double>(), kScalPScale, true, q, L));
+  BDC_TRY(output_law(c, prec, c->P.as<double>(), kScalPScale, true, rows, q, L));
   if (prec == BDC_PREC_FP64) {
-    BDC_TRY(cross_operands(c, L, true));
+    BDC_TRY(cross_operands(c, L, true, rows));
   } else {
-    BDC_TRY(split_cross_operands(c, L, true));
+    BDC_TRY(split_cross_operands(c, L, true, rows));
   }
   int nb = 0;
-  BDC_TRY(cross_term(c, prec, EPI_BA, nullptr, &nb));
+  BDC_TRY(cross_term(c, prec, EPI_BA, nullptr, &nb, rows));
   if (prec == BDC_PREC_FP64) c->plimb_valid = false;  // P changed without new limbs
-  BDC_LAUNCH(launch_ba_finalize(c->part.as<double>(), nb, c->scal.as<double>(), c->hist.as<double>(), idx, c->stream));
+  BDC_LAUNCH(launch_ba_finalize(c->part.as<double>(), nb, (double)c->N1 / rows, c->scal.as<double>(),
+                                c->hist.as<double>(), idx, c->stream));
   return BDC_OK;
 }
 
@@ -653,9 +685,10 @@ int bdc_iterate(bdc_ctx *c, int prec, int iters, double tol, bdc_iter_stats *sta
   if (!c->have_p) return fail(BDC_ERR_STATE, "p is not set");
   if (prec == BDC_PREC_SPLIT16) BDC_TRY(split_prepare(c));
   BDC_TRY(ensure_hist(c, std::max(iters, 1)));
+  const int rows = ba_rows(c, prec);
   int ran = 0;
   for (int it = 0; it < iters; ++it) {
-    BDC_TRY(ba_step(c, prec, it));
+    BDC_TRY(ba_step(c, prec, it, rows));
     ++ran;
     if (tol > 0.0) {
       double mx = 0.0;
@@ -664,6 +697,7 @@ int bdc_iterate(bdc_ctx *c, int prec, int iters, double tol, bdc_iter_stats *sta
       if (mx < tol) break;  // main_BAA.cu:73 (loop runs while mx >= a)
     }
   }
+  if (ran && rows < c->N1) BDC_LAUNCH(launch_reflect_half(c->P.as<double>(), c->nx, c->stream));
   if (stats && ran) BDC_HIP(hipMemcpyAsync(stats, c->hist.p, (size_t)ran * sizeof(bdc_iter_stats), hipMemcpyDeviceToHost, c->stream));
   BDC_HIP(hipStreamSynchronize(c->stream));
   if (done) *done = ran;
@@ -676,7 +710,7 @@ int bdc_certify(bdc_ctx *c, bdc_certificate *out) {
   BDC_TRY(ensure_dvec(c));
   double *q = c->qv.as<double>(), *L = q + ((size_t)1 << c->k);
   // FP64 throughout, no symmetry assumption
-  BDC_TRY(output_law(c, BDC_PREC_FP64, c->P.as<double>(), kScalPScale, false, q, L));
+  BDC_TRY(output_law(c, BDC_PREC_FP64, c->P.as<double>(), kScalPScale, false, c->N1, q, L));
   BDC_TRY(cross_operands(c, L));
   int nb = 0;
   BDC_TRY(cross_term(c, BDC_PREC_FP64, EPI_D, c->dvec.as<double>(), &nb));
@@ -697,7 +731,7 @@ int bdc_output_law(bdc_ctx *c, int prec, const double *w, double *qo) {
   BDC_TRY(ensure_dvec(c));
   BDC_HIP(hipMemcpyAsync(c->dvec.p, w, c->nx * 8, hipMemcpyHostToDevice, c->stream));
   double *q = c->qv.as<double>() + 2 * ((size_t)1 << c->k), *L = c->qv.as<double>() + 3 * ((size_t)1 << c->k);
-  BDC_TRY(output_law(c, prec, c->dvec.as<double>(), kScalOne, false, q, L));
+  BDC_TRY(output_law(c, prec, c->dvec.as<double>(), kScalOne, false, c->N1, q, L));
   BDC_HIP(hipMemcpyAsync(qo, q, ((size_t)1 << c->k) * 8, hipMemcpyDeviceToHost, c->stream));
   BDC_HIP(hipStreamSynchronize(c->stream));
   return BDC_OK;
@@ -720,7 +754,7 @@ int bdc_kl(bdc_ctx *c, int prec, const double *log_q, double *d) {
   if (prec == BDC_PREC_FP64)
     BDC_TRY(cross_operands(c, L));
   else
-    BDC_TRY(split_cross_operands(c, L));
+    BDC_TRY(split_cross_operands(c, L, false, c->N1));
   int nb = 0;
   BDC_TRY(cross_term(c, prec, EPI_D, c->dvec.as<double>(), &nb));
   BDC_HIP(hipMemcpyAsync(d, c->dvec.p, c->nx * 8, hipMemcpyDeviceToHost, c->stream));
@@ -747,14 +781,14 @@ static int dual_device(bdc_ctx *c, int prec, double beta, double *devU, bool wan
   if (prec == BDC_PREC_FP64)
     BDC_TRY(cross_operands(c, L));
   else
-    BDC_TRY(split_cross_operands(c, L));
+    BDC_TRY(split_cross_operands(c, L, false, c->N1));
   int nb = 0;
   BDC_TRY(cross_term(c, prec, EPI_D, c->dvec.as<double>(), &nb));
   BDC_LAUNCH(launch_dual_weights(c->dvec.as<double>(), c->nx, beta, c->part.as<double>(), nb, c->scal.as<double>(), devU,
                                  c->stream));
   if (!want_grad) return BDC_OK;
   // W^T pi through the output-law contraction (adjoint of the cross term)
-  BDC_TRY(output_law(c, prec, c->dvec.as<double>(), kScalDualScale, false, wq, wl));
+  BDC_TRY(output_law(c, prec, c->dvec.as<double>(), kScalDualScale, false, c->N1, wq, wl));
   BDC_LAUNCH(launch_vec_sub(q, wq, 1.0, c->grad.as<double>(), ny, c->stream));
   return BDC_OK;
 }
@@ -829,7 +863,7 @@ int bdc_log_output(bdc_ctx *c, double *theta) {
   if (!c->have_p) return fail(BDC_ERR_STATE, "p is not set");
   const size_t ny = (size_t)1 << c->k;
   double *q = c->qv.as<double>(), *L = q + ny;
-  BDC_TRY(output_law(c, BDC_PREC_FP64, c->P.as<double>(), kScalPScale, false, q, L));
+  BDC_TRY(output_law(c, BDC_PREC_FP64, c->P.as<double>(), kScalPScale, false, c->N1, q, L));
   std::vector<double> h(ny);
   BDC_HIP(hipMemcpyAsync(h.data(), q, ny * 8, hipMemcpyDeviceToHost, c->stream));
   BDC_HIP(hipStreamSynchronize(c->stream));
@@ -872,7 +906,8 @@ int bdc_kernel_time(bdc_ctx *c, const char *name, double *avg_ms, int *count) {
 
 int bdc_work_model(bdc_ctx *c, int prec, double *g1, double *g2, double *bytes) {
   if (!c) return fail(BDC_ERR_ARG, "null context");
-  const double nx = (double)c->nx;
+  // a BA step covers the rows x1 < ba_rows of P
+  const double nx = (double)c->nx * ((double)ba_rows(c, prec) / c->N1);
   // G1 with the mirror rule contracts only Abar (KL columns); G2 has K = KL + KR.
   if (g1) *g1 = 2.0 * c->KL * nx;
   if (g2) *g2 = 2.0 * (c->KL + c->KR) * nx;
@@ -969,23 +1004,27 @@ int split_prepare(bdc_ctx *c) {
   sp.g2_limbs = env_int("BDC_SPLIT_G2_LIMBS", std::min(4, (42 + sp.g2_bits - 1) / sp.g2_bits));
   if (sp.g1_limbs < 2 || sp.g1_limbs > 4 || sp.g2_limbs < 2 || sp.g2_limbs > 4)
     return fail(BDC_ERR_ARG, "limb counts must be in [2, 4] (got %d, %d)", sp.g1_limbs, sp.g2_limbs);
-  std::vector<uint8_t> fA(N1 / 128, 0), fB(N2 / 128, 0);
+  std::vector<uint8_t> fA(N1 / 128, 0), fB(N2 / 64, 0);
   for (size_t x1 = 0; x1 < N1; ++x1)
     if (rowptrA[x1 + 1] > rowptrA[x1]) fA[x1 / 128] = 1;
   for (size_t x2 = 0; x2 < N2; ++x2)
-    if (rowptrB[x2 + 1] > rowptrB[x2]) fB[x2 / 128] = 1;
+    if (rowptrB[x2 + 1] > rowptrB[x2]) fB[x2 / 64] = 1;
+  for (int h = 0; h < 2; ++h) {  // residual tile lists for the full and the half space
+    const size_t rows = h ? N1 / 2 : N1;
+    std::vector<int32_t> t;
+    if (rows % 128 == 0)
+      for (size_t ty = 0; ty < rows / 128; ++ty)
+        for (size_t tx = 0; tx < N2 / 64; ++tx)
+          if (fA[ty] || fB[tx]) t.push_back((int32_t)(ty * (N2 / 64) + tx));
+    c->nrestiles[h] = (int)t.size();
+    if (t.empty()) t.push_back(0);
+    BDC_TRY(c->restiles[h].alloc(t.size() * 4));
+    BDC_HIP(hipMemcpyAsync(c->restiles[h].p, t.data(), t.size() * 4, hipMemcpyHostToDevice, c->stream));
+  }
   BDC_TRY(c->blkA.alloc(fA.size()));
   BDC_TRY(c->blkB.alloc(fB.size()));
   BDC_HIP(hipMemcpyAsync(c->blkA.p, fA.data(), fA.size(), hipMemcpyHostToDevice, c->stream));
   BDC_HIP(hipMemcpyAsync(c->blkB.p, fB.data(), fB.size(), hipMemcpyHostToDevice, c->stream));
-  // split-K of G1 so that at least ~512 blocks run
-  {
-    const int blocks = (int)(N2 / 32) * (int)((KLp + 255) / 256);
-    int ks = 1;
-    while (blocks * ks < 512 && (N1 / 128) % (ks * 2) == 0 && N1 / (ks * 2) >= 1024) ks *= 2;
-    c->g1_ksplit = ks;
-    if (ks > 1) BDC_TRY(c->g1part.alloc((size_t)ks * KLp * N2 * 8));
-  }
   if (resA.empty()) resA.push_back(0);
   if (entA.empty()) entA.push_back(0);
   if (entB.empty()) entB.push_back(0);
@@ -1008,7 +1047,7 @@ int split_prepare(bdc_ctx *c) {
   BDC_HIP(hipMemcpyAsync(c->rowptrB.p, rowptrB.data(), rowptrB.size() * 4, hipMemcpyHostToDevice, c->stream));
   BDC_HIP(hipMemcpyAsync(c->entB.p, entB.data(), entB.size() * 4, hipMemcpyHostToDevice, c->stream));
   BDC_TRY(c->plimb.alloc((size_t)sp.g1_limbs * c->nx * 2));
-  BDC_TRY(c->ptexp.alloc((N1 / 128) * (N2 / 128) * 4));
+  BDC_TRY(c->ptexp.alloc((N1 / 128) * (N2 / 32) * 4));
   BDC_TRY(c->vlimb.alloc((size_t)sp.g2_limbs * N2 * KLp * 2));
   if (KRp) BDC_TRY(c->wlimb.alloc((size_t)sp.g2_limbs * N1 * KRp * 2));
   BDC_HIP(hipStreamSynchronize(c->stream));
@@ -1016,43 +1055,83 @@ int split_prepare(bdc_ctx *c) {
   return BDC_OK;
 }
 
-int split_output_law_g1(bdc_ctx *c, const double *X) {
+// G1 split-K: one K range per XCD (8) when every range keeps >= 1024 rows
+static int g1_splits(int rows) {
+  int ks = 1;
+  while (ks < 8 && (rows / 128) % (ks * 2) == 0 && rows / (ks * 2) >= 1024) ks *= 2;
+  return env_int("BDC_G1_KSPLIT", ks);
+}
+
+int split_output_law_g1(bdc_ctx *c, const double *X, int rows) {
   const Split &sp = c->split;
-  if (!(X == c->P.as<double>() && c->plimb_valid))
-    BDC_LAUNCH(launch_split_p(X, c->scal.as<double>(), c->N1, c->N2, sp.g1_limbs, sp.g1_bits, c->plimb.as<_Float16>(),
-                              c->ptexp.as<int32_t>(), c->stream));
-  c->plimb_valid = (X == c->P.as<double>());
-  const int ks = c->g1_ksplit;
+  if (rows % 128) return fail(BDC_ERR_ARG, "split output law needs a multiple of 128 rows (%d)", rows);
+  const bool isP = X == c->P.as<double>();
+  if (!(isP && c->plimb_valid && c->plimb_rows == rows))
+    BDC_LAUNCH(launch_split_p(X, rows, c->N2, sp.g1_limbs, sp.g1_bits, c->plimb.as<_Float16>(), c->ptexp.as<int32_t>(),
+                              c->stream));
+  c->plimb_valid = isP;
+  c->plimb_rows = rows;
+  const int ks = g1_splits(rows);
+  if (ks < 1 || rows % (ks * 128)) return fail(BDC_ERR_ARG, "G1 split-K %d does not divide %d rows in 128s", ks, rows);
+  const size_t slab = (size_t)c->KLp * c->N2;
+  if (ks > 1 && c->g1part.bytes < ks * slab * 8) BDC_TRY(c->g1part.alloc(ks * slab * 8));
   double *dstT = ks > 1 ? c->g1part.as<double>() : c->T.as<double>();
-  BDC_LAUNCH(launch_split_g1(c->KLp, c->N1, c->N2, c->abarTh.as<_Float16>(), c->plimb.as<_Float16>(),
+  BDC_LAUNCH(launch_split_g1(c->KLp, rows, c->N1, c->N2, c->abarTh.as<_Float16>(), c->plimb.as<_Float16>(),
                              c->ptexp.as<int32_t>(), sp.g1_limbs, sp.g1_bits, ks, dstT, c->stream));
-  if (ks > 1)
-    BDC_LAUNCH(launch_reduce_splits(dstT, ks, (int64_t)c->KLp * c->N2, (uint64_t)c->KLp * c->N2, c->T.as<double>(),
-                                    c->stream));
-  BDC_LAUNCH(launch_resid_g1(c->residA.as<int32_t>(), c->n_resid_a, X, c->N2, c->T.as<double>(), c->stream));
+  if (ks > 1) BDC_LAUNCH(launch_reduce_splits(dstT, ks, (int64_t)slab, (uint64_t)slab, c->T.as<double>(), c->stream));
+  BDC_LAUNCH(launch_resid_g1(c->residA.as<int32_t>(), c->n_resid_a, X, rows, c->N2, c->T.as<double>(), c->stream));
   return BDC_OK;
 }
 
-int split_cross_operands(bdc_ctx *c, const double *L, bool mirror) {
-  BDC_TRY(cross_operands(c, L, mirror));  // V, W in FP64 (stage GEMMs)
+int split_cross_operands(bdc_ctx *c, const double *L, bool mirror, int rows) {
+  BDC_TRY(cross_operands(c, L, mirror, rows));  // V, W in FP64 (stage GEMMs)
   const Split &sp = c->split;
-  BDC_LAUNCH(launch_split_vw(c->g2b.as<double>(), c->KL, c->KLp, c->N2, c->g2a.as<double>(), c->N1, c->KR, c->KRp,
+  BDC_LAUNCH(launch_split_vw(c->g2b.as<double>(), c->KL, c->KLp, c->N2, c->g2a.as<double>(), rows, c->KR, c->KRp,
                              sp.g2_limbs, sp.g2_bits, c->vlimb.as<_Float16>(), c->wlimb.as<_Float16>(),
                              c->part.as<double>(), c->scal.as<double>(), c->stream));
   return BDC_OK;
 }
 
-int split_cross_term(bdc_ctx *c, int epi, const EpiArgs &ea, int *nb) {
+int split_cross_term(bdc_ctx *c, int epi, const EpiArgs &ea, int *nb, int rows) {
   const Split &sp = c->split;
   // the BA update writes the next step's P limbs in its epilogue
   const bool fuse = (epi == EPI_BA) && ea.P == c->P.as<double>();
-  BDC_LAUNCH(launch_split_g2(c->N1, c->N2, c->KL, c->KLp, c->KR, c->KRp, c->abarh.as<_Float16>(), c->vlimb.as<_Float16>(),
-                             c->wlimb.as<_Float16>(), c->bbarh.as<_Float16>(), sp.g2_limbs, sp.g2_bits, epi, ea,
-                             c->blkA.as<uint8_t>(), c->blkB.as<uint8_t>(), c->rowptrA.as<int32_t>(), c->entA.as<int32_t>(),
-                             c->rowptrB.as<int32_t>(), c->entB.as<int32_t>(), c->g2b.as<double>(), c->g2a.as<double>(),
-                             fuse ? c->plimb.as<_Float16>() : nullptr, fuse ? c->ptexp.as<int32_t>() : nullptr,
-                             sp.g1_limbs, sp.g1_bits, c->stream, nb));
-  if (epi == EPI_BA) c->plimb_valid = fuse;
+  SplitG2Args a{};
+  a.N1 = c->N1;
+  a.N2 = c->N2;
+  a.M = rows;
+  a.KL = c->KL;
+  a.KLp = c->KLp;
+  a.KR = c->KR;
+  a.KRp = c->KRp;
+  a.A16 = c->abarh.as<_Float16>();
+  a.vT = c->vlimb.as<_Float16>();
+  a.wl = c->wlimb.as<_Float16>();
+  a.B16 = c->bbarh.as<_Float16>();
+  a.limbs = sp.g2_limbs;
+  a.bits = sp.g2_bits;
+  a.epi = epi;
+  a.blkA = c->blkA.as<uint8_t>();
+  a.blkB = c->blkB.as<uint8_t>();
+  a.rowptrA = c->rowptrA.as<int32_t>();
+  a.entA = c->entA.as<int32_t>();
+  a.rowptrB = c->rowptrB.as<int32_t>();
+  a.entB = c->entB.as<int32_t>();
+  a.V64 = c->g2b.as<double>();
+  a.W64 = c->g2a.as<double>();
+  a.limbT_next = fuse ? c->plimb.as<_Float16>() : nullptr;
+  a.texp_next = fuse ? c->ptexp.as<int32_t>() : nullptr;
+  a.limbs_p = sp.g1_limbs;
+  a.bits_p = sp.g1_bits;
+  a.exptab = c->exptab.as<double>();
+  const int h = rows == c->N1 ? 0 : 1;
+  a.res_tiles = c->restiles[h].as<int32_t>();
+  a.nres_tiles = c->nrestiles[h];
+  BDC_LAUNCH(launch_split_g2(a, ea, c->stream, nb));
+  if (epi == EPI_BA) {
+    c->plimb_valid = fuse;
+    c->plimb_rows = rows;
+  }
   return BDC_OK;
 }
 

@@ -132,21 +132,24 @@ struct bdc_ctx {
   bdc::DevBuf part;   // per-block partials of the fused epilogue
   size_t part_blocks = 0;
   bdc::DevBuf scal;   // device scalars (see kScal* in kernels)
+  bdc::DevBuf exptab; // 2^(j/32), j < 32, for the update's exp
   bdc::DevBuf hist;   // per-iteration stats (device)
   int hist_cap = 0;
 
   // split-FP16 buffers
   bdc::Split split;
   bdc::DevBuf plimb;   // P limbs transposed, fp16 [L][N2][N1]
-  bdc::DevBuf ptexp;   // per-tile exponents of P (int32), [N1/128][N2/128]
+  bdc::DevBuf ptexp;   // per-tile exponents of P (int32), one per 128 x 64 tile
   bdc::DevBuf vlimb;   // Vbar limbs transposed, fp16 [L][N2][KLp]
   bdc::DevBuf wlimb;   // Wbar limbs fp16 [L][N1][KRp]
   bdc::DevBuf residA;  // (x1, kappa, r) triples of Abar counts fp16 cannot hold
   bdc::DevBuf rowptrA, entA, rowptrB, entB;  // the same residuals as CSR over x1 (Abar) and x2 (Bbar)
-  bdc::DevBuf blkA, blkB;  // per 128-block flags: any residual in these rows
+  bdc::DevBuf blkA, blkB;  // flags per 128 rows of Abar / 64 rows of Bbar: any residual there
+  bdc::DevBuf restiles[2];  // G2 tiles (128 x 64) touched by a residual: over N1 rows, over N1/2 rows
+  int nrestiles[2] = {0, 0};
   bdc::DevBuf g1part;      // split-K partial slabs of Tbar
-  int g1_ksplit = 1;
-  bool plimb_valid = false;  // plimb/ptexp hold the limbs of the current P
+  bool plimb_valid = false;  // plimb/ptexp hold the limbs of the current P (rows x1 < plimb_rows)
+  int plimb_rows = 0;
   int n_resid_a = 0, n_resid_b = 0;
   double g1_chunk_sum = 0, g2_row_sum = 0;  // exactness bounds (max count mass per FP32 accumulation)
 
@@ -165,7 +168,7 @@ namespace bdc {
 struct EpiArgs;
 // split-FP16 path (split.hip)
 int split_prepare(bdc_ctx *c);
-int split_output_law_g1(bdc_ctx *c, const double *X);
-int split_cross_operands(bdc_ctx *c, const double *L, bool mirror = false);
-int split_cross_term(bdc_ctx *c, int epi, const EpiArgs &ea, int *nblocks);
+int split_output_law_g1(bdc_ctx *c, const double *X, int rows);
+int split_cross_operands(bdc_ctx *c, const double *L, bool mirror, int rows);
+int split_cross_term(bdc_ctx *c, int epi, const EpiArgs &ea, int *nblocks, int rows);
 }  // namespace bdc

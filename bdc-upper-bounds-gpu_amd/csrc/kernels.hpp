@@ -44,6 +44,7 @@ struct EpiArgs {
   const double *scal; // device scalars
   double inv_c;
   int64_t ldx;        // row stride of the x-indexed arrays (N2)
+  const double *exptab;  // 2^(j/32), j < 32
 };
 // C[M x N] (=) sum_phase A_ph[M x K_ph] B_ph[K_ph x N].  Phase 2 is skipped when K2 == 0.
 // splits > 1: C + z*split_stride receives the partial over the z-th K range of phase 1
@@ -55,8 +56,11 @@ int gemm64(int M, int N, const Opnd &A1, const Opnd &B1, int K1, const Opnd &A2,
 int launch_reduce_splits(const double *part, int splits, int64_t stride, uint64_t len, double *C, hipStream_t s);
 
 // ---- streaming (kernels_stream.hip) ----
-// finalize the fused BA epilogue: reduce block partials, update scalars, write history
-int launch_ba_finalize(const double *part, int nblocks, double *scal, double *hist, int hist_idx, hipStream_t s);
+// finalize the fused BA epilogue: reduce block partials (times mult), update scalars, write history
+int launch_ba_finalize(const double *part, int nblocks, double mult, double *scal, double *hist, int hist_idx,
+                       hipStream_t s);
+// P[2^n - 1 - x] = P[x] for x < 2^n / 2
+int launch_reflect_half(double *P, uint64_t nx, hipStream_t s);
 // q(y) = pscale * inv_c * (sum_{j in JL} qpart[jl][y] + sum_{j in JR} ...) ; L = ln q - mu
 struct QCombine {
   int k;
@@ -66,8 +70,11 @@ struct QCombine {
   int mirror;                  // 1: JR part = sum over qL slices selected by mirrorSel at rev(y)
   unsigned long long mirrorSel; // bit i set: slice i of qL contributes to the mirrored part
   int scale_slot;              // device scalar holding the scale of the weights (p = scale * P)
+  int comp;                    // 1: slices are half-space sums R_j; q_j(y) = R_j(y) + R_j(~y)
 };
-int launch_q_combine(const QCombine &qc, double inv_c, double *scal, double *qout, double *Lout, hipStream_t s);
+// part: 2^k / 256 doubles of scratch
+int launch_q_combine(const QCombine &qc, double inv_c, double *scal, double *qout, double *Lout, double *part,
+                     hipStream_t s);
 // theta -> q = softmax(theta), L = ln q - mu (dual bound)
 int launch_softmax_log(const double *theta, int k, double *q, double *L, double *scal, hipStream_t s);
 // D-pass reductions for the dual: max D, then pi = softmax(beta D) and U
@@ -96,17 +103,29 @@ int launch_w_from_v(const MirrorW &mw, const double *V, int N2, int n2, double *
                     hipStream_t s);
 
 // ---- split-FP16 contractions (kernels_split.hip) ----
-int launch_split_p(const double *P, const double *scal, int N1, int N2, int limbs, int bits, _Float16 *limbT,
-                   int32_t *texp, hipStream_t s);
-int launch_split_vw(const double *V, int KL, int KLp, int N2, const double *W, int N1, int KR, int KRp, int limbs,
+// P-like array (rows x1 < M, N2 columns) -> transposed limbs [l][N2][M] + 128x64 tile exponents
+int launch_split_p(const double *P, int M, int N2, int limbs, int bits, _Float16 *limbT, int32_t *texp, hipStream_t s);
+int launch_split_vw(const double *V, int KL, int KLp, int N2, const double *W, int M, int KR, int KRp, int limbs,
                     int bits, _Float16 *vT, _Float16 *wl, double *part, double *scal, hipStream_t s);
-int launch_split_g1(int KLp, int N1, int N2, const _Float16 *aT, const _Float16 *pT, const int32_t *texp, int limbs,
-                    int bits, int ksplit, double *T, hipStream_t s);
-int launch_resid_g1(const int32_t *res, int nres, const double *P, int N2, double *T, hipStream_t s);
-int launch_split_g2(int N1, int N2, int KL, int KLp, int KR, int KRp, const _Float16 *A16, const _Float16 *vT,
-                    const _Float16 *wl, const _Float16 *B16, int limbs, int bits, int epi, const EpiArgs &ea,
-                    const uint8_t *blkA, const uint8_t *blkB, const int32_t *rowptrA, const int32_t *entA,
-                    const int32_t *rowptrB, const int32_t *entB, const double *V64, const double *W64,
-                    _Float16 *limbT_next, int32_t *texp_next, int limbs_p, int bits_p, hipStream_t s, int *nb);
+// ks partial slabs (ks * KLp * ncols FP64) of Tbar = aT X over the first Kdim rows (aT row stride lda)
+int launch_split_g1(int KLp, int Kdim, int lda, int ncols, const _Float16 *aT, const _Float16 *pT, const int32_t *texp,
+                    int limbs, int bits, int ks, double *Tpart, hipStream_t s);
+// residual corrections of G1 from the rows x1 < rows
+int launch_resid_g1(const int32_t *res, int nres, const double *X, int rows, int N2, double *T, hipStream_t s);
+struct SplitG2Args {
+  int N1, N2, M, KL, KLp, KR, KRp;  // M: rows x1 computed (< N1 in the half-space step)
+  const _Float16 *A16, *vT, *wl, *B16;
+  int limbs, bits, epi;
+  const uint8_t *blkA, *blkB;
+  const int32_t *rowptrA, *entA, *rowptrB, *entB;
+  const double *V64, *W64;
+  _Float16 *limbT_next;  // fused next-step limbs (BA) or null
+  int32_t *texp_next;
+  int limbs_p, bits_p;
+  const double *exptab;
+  const int32_t *res_tiles;  // tiles (ty * N2/64 + tx) holding a residual, run by a second launch
+  int nres_tiles;
+};
+int launch_split_g2(const SplitG2Args &a, const EpiArgs &ea, hipStream_t s, int *nb);
 
 }  // namespace bdc

@@ -12,6 +12,7 @@
 // D(x||q) = H(x)/C - ln C - mu - S(x)/C, writes p~(x) exp(D - dref) in place,
 // and emits per-block partials (sum of the new p~, max D, sum p~ D).
 #include "device_common.hpp"
+#include "fast_exp.hpp"
 #include "kernels.hpp"
 
 namespace bdc {
@@ -40,7 +41,9 @@ __global__ void __launch_bounds__(256) k_gemm64(int M, int N, Opnd A1, Opnd B1, 
   __shared__ double As[BK][BM + 1];
   __shared__ double Bs[BK][BN + 1];
   __shared__ double red[48];
+  __shared__ double etab[32];  // exp table in LDS (a global one could alias the P stores)
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if (EPI == EPI_BA && tid < 32) etab[tid] = ea.exptab[tid];
   const int wm = wave / WN, wn = wave % WN;
   const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
   const int z = blockIdx.z;
@@ -151,10 +154,25 @@ __global__ void __launch_bounds__(256) k_gemm64(int M, int N, Opnd A1, Opnd B1, 
           if (row < M && col < N) Cz[row * crs + col * ccs] = acc[i][j][r];
         }
   } else {
+    __syncthreads();  // etab visible even when no K loop ran
     const double mu = ea.scal[kScalMu], dref = ea.scal[kScalDRef];
     double s0 = 0.0, m1 = -INFINITY, s2 = 0.0;
 #pragma unroll
-    for (int i = 0; i < SM; ++i)
+    for (int i = 0; i < SM; ++i) {
+      // issue the H (and P) loads of 16 elements before any store to P, so
+      // their latencies overlap instead of ordering behind the stores
+      double hv[SN][4], pv[SN][4];
+#pragma unroll
+      for (int j = 0; j < SN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = m0 + wm * TM + i * 16 + mfma64_row(lane, r);
+          const int col = n0 + wn * TN + j * 16 + (lane & 15);
+          const int64_t x = (int64_t)row * ea.ldx + col;
+          const bool in = row < M && col < N;
+          hv[j][r] = in ? ea.Hc[x] : 0.0;
+          if constexpr (EPI == EPI_BA) pv[j][r] = in ? ea.P[x] : 0.0;
+        }
 #pragma unroll
       for (int j = 0; j < SN; ++j)
 #pragma unroll
@@ -163,11 +181,11 @@ __global__ void __launch_bounds__(256) k_gemm64(int M, int N, Opnd A1, Opnd B1, 
           const int col = n0 + wn * TN + j * 16 + (lane & 15);
           if (row < M && col < N) {
             const int64_t x = (int64_t)row * ea.ldx + col;
-            const double D = ea.Hc[x] - mu - acc[i][j][r] * ea.inv_c;
+            const double D = hv[j][r] - mu - acc[i][j][r] * ea.inv_c;
             m1 = fmax(m1, D);
             if constexpr (EPI == EPI_BA) {
-              const double p = ea.P[x];
-              const double w = p * exp(D - dref);
+              const double p = pv[j][r];
+              const double w = p * exp_tang(D - dref, etab);
               ea.P[x] = w;
               s0 += w;
               s2 += p * D;
@@ -176,6 +194,7 @@ __global__ void __launch_bounds__(256) k_gemm64(int M, int N, Opnd A1, Opnd B1, 
             }
           }
         }
+    }
     block_reduce3(s0, m1, s2, red);
     if (tid == 0) {
       double *pp = ea.part + (int64_t)(blockIdx.y * gridDim.x + blockIdx.x) * kPartStrideDev;

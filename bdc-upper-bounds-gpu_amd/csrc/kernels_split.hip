@@ -11,12 +11,19 @@
 // partial sum stays below 2^24 in magnitude — the FP32 accumulator then adds
 // integers exactly, whatever order the matrix core uses.  The only error left
 // is the truncation of the FP64 operand to L*b bits.
+//
+// P tiles are 128 (x1) x 32 (x2) with one exponent each.  The output-law
+// contraction reads P limbs transposed ([l][x2][x1]) so that its K index x1 is
+// contiguous.  The BA step keeps only the rows x1 < N1/2 of P (the other half
+// is its complement image), so K runs over those rows.
 #include "device_common.hpp"
+#include "fast_exp.hpp"
 #include "kernels.hpp"
 
 namespace bdc {
 
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+typedef _Float16 half4 __attribute__((ext_vector_type(4)));
 typedef float float16_ __attribute__((ext_vector_type(16)));
 
 // row of the 32x32 MFMA result held by `lane` in register r (verified layout)
@@ -28,60 +35,74 @@ __device__ inline int tile_exponent(double mx, int bits) {
   return ilogb(mx) + 1 - (bits - 1);
 }
 
+// Transposed limb layout of a P-like array: limb l, column x2, row x1 at
+// l * ncols * kdim + x2 * kdim + x1; exponents per 128 x 32 tile, row-major.
+struct LimbMap {
+  int kdim;   // rows x1 held (the K extent of the output-law contraction)
+  int ncols;  // columns x2
+};
+
+// write a [64][136] LDS tile (x2 local, x1 local) of one limb to its place
+__device__ inline void store_limb_tile(const _Float16 (*t)[136], _Float16 *dst, const LimbMap &mp, int m0, int n0,
+                                       int nthreads) {
+  for (int ch = threadIdx.x; ch < 64 * 16; ch += nthreads) {
+    const int r2 = ch >> 4, g = ch & 15;
+    *reinterpret_cast<uint4 *>(dst + (int64_t)(n0 + r2) * mp.kdim + m0 + 8 * g) =
+        *reinterpret_cast<const uint4 *>(&t[r2][8 * g]);
+  }
+}
+__device__ inline int tile_index(const LimbMap &mp, int m0, int n0) { return (m0 / 128) * (mp.ncols / 32) + n0 / 32; }
+
 // ----------------------------------------------------------------- P limbs
-// One 128x128 tile of P per block: tile max -> exponent, limbs written
-// transposed ([l][x2][x1]) so that the output-law contraction reads K = x1
-// contiguously.
+// One 128 x 64 block of a P-like array (rows x1 < kdim, N2 columns) per
+// workgroup: two 128 x 32 exponent tiles.  Thread tid always holds column
+// tid % 64, so lanes 0-31 of a wave cover the first tile and 32-63 the second.
 template <int L>
-__global__ void __launch_bounds__(1024) k_split_p(const double *__restrict__ P, const double *__restrict__ scal, int N1,
-                                                  int N2, int bits, _Float16 *__restrict__ limbT,
-                                                  int32_t *__restrict__ texp) {
-  constexpr int PITCH = 136;  // halves per LDS row: 16-byte aligned rows
-  __shared__ __attribute__((aligned(16))) _Float16 tile[128][PITCH];  // [x2][x1]
-  __shared__ double red[16];
-  const int tid = threadIdx.x;
-  const int m0 = blockIdx.y * 128, n0 = blockIdx.x * 128;
-  double v[16];
+__global__ void __launch_bounds__(256) k_split_p(const double *__restrict__ P, int N2, int bits, LimbMap mp,
+                                                 _Float16 *__restrict__ limbT, int32_t *__restrict__ texp) {
+  __shared__ __attribute__((aligned(16))) _Float16 tile[64][136];  // [x2][x1]
+  __shared__ double red[8];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, half = lane >> 5;
+  const int m0 = blockIdx.y * 128, n0 = blockIdx.x * 64;
+  double v[32];
   double mx = 0.0;
 #pragma unroll
-  for (int r = 0; r < 16; ++r) {
-    const int e = tid + 1024 * r, row = e >> 7, col = e & 127;
+  for (int r = 0; r < 32; ++r) {
+    const int e = tid + 256 * r, row = e >> 6, col = e & 63;
     v[r] = P[(int64_t)(m0 + row) * N2 + n0 + col];
     mx = fmax(mx, fabs(v[r]));
   }
-  mx = block_max(mx, red);
-  const int et = tile_exponent(mx, bits);
-  if (tid == 0) texp[blockIdx.y * gridDim.x + blockIdx.x] = et;
 #pragma unroll
-  for (int r = 0; r < 16; ++r) v[r] = ldexp(v[r], -et);
+  for (int o = 16; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o));  // within each 32-lane half
+  if ((lane & 31) == 0) red[wave * 2 + half] = mx;
+  __syncthreads();
+  mx = fmax(fmax(red[half], red[2 + half]), fmax(red[4 + half], red[6 + half]));
+  const int et = tile_exponent(mx, bits);
+  if (wave == 0 && (lane & 31) == 0) texp[tile_index(mp, m0, n0 + 32 * half)] = et;
+#pragma unroll
+  for (int r = 0; r < 32; ++r) v[r] = ldexp(v[r], -et);
   const double radix = ldexp(1.0, bits);
   for (int l = 0; l < L; ++l) {
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      const int e = tid + 1024 * r, row = e >> 7, col = e & 127;
+    for (int r = 0; r < 32; ++r) {
+      const int e = tid + 256 * r, row = e >> 6, col = e & 63;
       const double li = rint(v[r]);
       tile[col][row] = (_Float16)(float)li;
       v[r] = (v[r] - li) * radix;
     }
     __syncthreads();
-    _Float16 *dst = limbT + (int64_t)l * N1 * N2;
-#pragma unroll
-    for (int q = 0; q < 2; ++q) {
-      const int ch = tid + 1024 * q, c2 = ch >> 4, g = ch & 15;
-      *reinterpret_cast<uint4 *>(dst + (int64_t)(n0 + c2) * N1 + m0 + 8 * g) =
-          *reinterpret_cast<const uint4 *>(&tile[c2][8 * g]);
-    }
+    store_limb_tile(tile, limbT + (int64_t)l * mp.ncols * mp.kdim, mp, m0, n0, 256);
     __syncthreads();
   }
 }
 
 // ----------------------------------------------------------------- V/W limbs
-// max |V| over the KL x N2 block and |W| over the N1 x KR block
+// max |V| over the KL x N2 block and |W| over the M x KR block
 __global__ void __launch_bounds__(256) k_absmax_vw(const double *__restrict__ V, int64_t vlen, const double *__restrict__ W,
-                                                   int N1, int KR, int KRp, double *__restrict__ part) {
+                                                   int M, int KR, int KRp, double *__restrict__ part) {
   __shared__ double red[16];
   double m = 0.0;
-  const int64_t wlen = (int64_t)N1 * KR;
+  const int64_t wlen = (int64_t)M * KR;
   const int64_t tot = vlen + wlen;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < tot; i += (int64_t)gridDim.x * blockDim.x) {
     double a;
@@ -127,7 +148,7 @@ __global__ void __launch_bounds__(256) k_split_v(const double *__restrict__ V, i
     }
   }
 }
-// W (N1 x KRp) -> wl[l][x1][kappa']
+// W (M x KRp) -> wl[l][x1][kappa']
 template <int L>
 __global__ void k_split_w(const double *__restrict__ W, int64_t len, int KR, int KRp, int bits,
                           const double *__restrict__ scal, _Float16 *__restrict__ wl) {
@@ -145,21 +166,30 @@ __global__ void k_split_w(const double *__restrict__ W, int64_t len, int KR, int
 }
 
 // ----------------------------------------------------------------- G1
-// Tbar[kappa][x2] = sum_x1 Abar[x1][kappa] P[x1][x2] with P in L limbs.
-// Block: 256 (kappa) x 32 (x2); 8 waves, each 32 kappa x 32 x2 and all L limbs
-// (64 FP32 accumulators + 16 FP64 per lane).  K step 32 through LDS,
-// double-buffered; FP32 sums promoted to FP64 every 128 rows of x1 (one P tile
-// row -> one exponent).  blockIdx.z splits K.
+// Tbar[kappa][c] = sum_k aT[kappa][k] X[k][c] with X in L limbs stored
+// transposed (pT[l][c][k]), Kdim = K extent, ncols columns, lda = row stride
+// of aT (>= Kdim: the half-space step reads the leading rows of Abar).
+// Block: 256 (kappa) x 32 (columns); 8 waves of 32 x 32 with all L limbs (64
+// FP32 + 16 FP64 accumulators per lane).  K step 32 through LDS,
+// double-buffered; FP32 sums promoted to FP64 every 128 K rows (one P tile).
+// K is split in ks (8 for large problems) and with ks = 8 the split index is
+// the XCD the block runs on (blocks are dealt round-robin to the 8 XCDs), so
+// each XCD streams one eighth of the count operand, which stays in its L2; the
+// ks partial slabs are reduced afterwards.
 template <int L>
-__global__ void __launch_bounds__(512) k_split_g1(int KLp, int N1, int N2, const _Float16 *__restrict__ aT,
+__global__ void __launch_bounds__(512) k_split_g1(int KLp, int Kdim, int lda, int ncols, const _Float16 *__restrict__ aT,
                                                   const _Float16 *__restrict__ pT, const int32_t *__restrict__ texp,
-                                                  int bits, int krows, double *__restrict__ T) {
+                                                  int bits, int ks, double *__restrict__ Tpart) {
   constexpr int BM = 256, BN = 32, BK = 32, PADK = BK + 8;
   __shared__ __attribute__((aligned(16))) _Float16 As[2][BM][PADK];
   __shared__ __attribute__((aligned(16))) _Float16 Bs[2][L][BN][PADK];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
-  const int ntile2 = N2 / 128;
+  const int kz = blockIdx.x % ks;
+  const int tile = blockIdx.x / ks;
+  const int ntn = ncols / BN;
+  const int n0 = (tile % ntn) * BN, m0 = (tile / ntn) * BM;
+  const int tcols = ncols / 32;
+  const int krows = Kdim / ks;
 
   float16_ acc[L];
   double acc64[16];
@@ -168,23 +198,23 @@ __global__ void __launch_bounds__(512) k_split_g1(int KLp, int N1, int N2, const
 #pragma unroll
   for (int l = 0; l < L; ++l) acc[l] = (float16_){0};
 
-  constexpr int ACH = BM * BK / 8 / 512;              // 2 chunks of 16 B per thread
-  constexpr int BTOT = L * BN * BK / 8;               // L * 128 chunks
-  constexpr int BCH = (BTOT + 511) / 512;             // 1
+  constexpr int ACH = BM * BK / 8 / 512;   // 2 chunks of 16 B per thread
+  constexpr int BTOT = L * BN * BK / 8;    // L * 128 chunks
+  constexpr int BCH = (BTOT + 511) / 512;  // 1
   uint4 ra[ACH], rb[BCH];
   auto gload = [&](int k0) {
 #pragma unroll
     for (int c = 0; c < ACH; ++c) {
       const int ch = tid + 512 * c, row = ch >> 2, kq = ch & 3;
       const int gr = m0 + row;
-      ra[c] = gr < KLp ? *reinterpret_cast<const uint4 *>(aT + (int64_t)gr * N1 + k0 + kq * 8) : make_uint4(0, 0, 0, 0);
+      ra[c] = gr < KLp ? *reinterpret_cast<const uint4 *>(aT + (int64_t)gr * lda + k0 + kq * 8) : make_uint4(0, 0, 0, 0);
     }
 #pragma unroll
     for (int c = 0; c < BCH; ++c) {
       const int ch = tid + 512 * c;
       if (ch < BTOT) {
         const int l = ch / (BN * BK / 8), rem = ch % (BN * BK / 8), row = rem >> 2, kq = rem & 3;
-        rb[c] = *reinterpret_cast<const uint4 *>(pT + (int64_t)l * N1 * N2 + (int64_t)(n0 + row) * N1 + k0 + kq * 8);
+        rb[c] = *reinterpret_cast<const uint4 *>(pT + (int64_t)l * Kdim * ncols + (int64_t)(n0 + row) * Kdim + k0 + kq * 8);
       }
     }
   };
@@ -204,8 +234,7 @@ __global__ void __launch_bounds__(512) k_split_g1(int KLp, int N1, int N2, const
     }
   };
 
-  const int kbase = blockIdx.z * krows;
-  T += (int64_t)blockIdx.z * KLp * N2;
+  const int kbase = kz * krows;
   const int nsteps = krows / BK;
   gload(kbase);
   sstore(0);
@@ -223,9 +252,9 @@ __global__ void __launch_bounds__(512) k_split_g1(int KLp, int N1, int N2, const
         acc[l] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, acc[l], 0, 0, 0);
       }
     }
-    // promote after every 128 rows of x1: exact integers -> FP64
+    // promote after every 128 K rows: exact integers -> FP64
     if (((s + 1) * BK) % 128 == 0) {
-      const int et = texp[((kbase + s * BK) / 128) * ntile2 + n0 / 128];
+      const int et = texp[((kbase + s * BK) / 128) * tcols + n0 / 32];
       double sc[L];
 #pragma unroll
       for (int l = 0; l < L; ++l) sc[l] = ldexp(1.0, et - bits * l);
@@ -244,68 +273,96 @@ __global__ void __launch_bounds__(512) k_split_g1(int KLp, int N1, int N2, const
       __syncthreads();
     }
   }
+  double *T = Tpart + (int64_t)kz * KLp * ncols;
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
     const int row = m0 + wave * 32 + mfma32_row(lane, r);
     const int col = n0 + (lane & 31);
-    if (row < KLp) T[(int64_t)row * N2 + col] = acc64[r];
+    if (row < KLp) T[(int64_t)row * ncols + col] = acc64[r];
   }
 }
 
-// T[kappa][x2] += r * P[x1][x2] for the residuals of the rounded counts; one
-// thread owns a column x2 and walks every residual, so updates never race.
-__global__ void k_resid_g1(const int32_t *__restrict__ res, int nres, const double *__restrict__ P, int N2,
+// T[kappa][c] += r * X(x1, c) for the residuals of the rounded counts in the
+// rows x1 < rows.  One thread owns a column and walks every residual, so
+// updates never race.
+__global__ void k_resid_g1(const int32_t *__restrict__ res, int nres, const double *__restrict__ X, int rows, int N2,
                            double *__restrict__ T) {
-  const int x2 = blockIdx.x * blockDim.x + threadIdx.x;
-  if (x2 >= N2) return;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= N2) return;
   for (int e = 0; e < nres; ++e) {
     const int x1 = res[3 * e], kap = res[3 * e + 1], r = res[3 * e + 2];
-    T[(int64_t)kap * N2 + x2] += (double)r * P[(int64_t)x1 * N2 + x2];
+    if (x1 < rows) T[(int64_t)kap * N2 + c] += (double)r * X[(int64_t)x1 * N2 + c];
   }
 }
 
 // ----------------------------------------------------------------- G2
 struct G2Resid {
   const uint8_t *blkA;     // N1/128: block of x1 rows has a residual
-  const uint8_t *blkB;     // N2/128: block of x2 rows (of Bbar) has a residual
+  const uint8_t *blkB;     // N2/64: block of x2 rows (of Bbar) has a residual
   const int32_t *rowptrA;  // N1 + 1
   const int32_t *entA;     // (kappa, r) pairs
   const int32_t *rowptrB;  // N2 + 1
   const int32_t *entB;     // (kappa', r) pairs
   const double *V64;       // KLp x N2
-  const double *W64;       // N1 x KRp
+  const double *W64;       // M x KRp
   int KRp;
+};
+struct G2Out {
+  _Float16 *limbT;  // next P limbs (BA epilogue)
+  int32_t *texp;    // their tile exponents
+  int bits_p;
+  LimbMap map;
+  int patches;  // 1: XCD patch mapping of tiles
+  int M;        // rows x1 computed (N1/2 in the half-space BA step, else N1)
+  const double *exptab;
 };
 
 // S[x1][x2] = sum_kappa A16[x1][kappa] V[kappa][x2] + sum_kappa' W[x1][kappa'] B16[x2][kappa']
-// with V, W in L limbs of one global scale.  Block 128 x 128 (one P tile),
-// 8 waves of 32 x 64.  Blocks are mapped to tiles XCD by XCD: the 8 XCDs each
-// sweep their own 8 x 8 patches of tiles so the operand rows a patch needs
-// (about 3.75 MiB) stay in that XCD's L2.
-// Epilogue: the BA update (or D), and with fuse_limbs the next step's P limbs
-// and tile exponent, written transposed through LDS.
-struct G2Out {
-  _Float16 *limbT;  // next P limbs [L][N2][N1] (fused BA epilogue) or null
-  int32_t *texp;    // tile exponents
-  int limbs_p, bits_p;
-  int patches;      // 1: XCD patch mapping
+// with V, W in L limbs of one global scale.  Block 128 x 64, 4 waves of
+// 32 x 64 (two 32 x 32 MFMA tiles per limb).
+// Every operand is K-contiguous, so a K stage of 64 is one 128-byte line per
+// operand row: each line is fetched by 8 lanes with 16-byte loads, one L2
+// request per line.  A stage is staged through registers into one LDS buffer
+// whose rows are padded to 144 bytes (conflict-free for the 16-byte writes and
+// for the MFMA operand reads); the next stage's loads are in flight while the
+// current one is multiplied.
+//   phase 1 rows: A16 (128), then V limbs (L x 64)
+//   phase 2 rows: W limbs (L x 128), then B16 (64)
+// The H and P loads of the epilogue are issued before the last stage's
+// multiplications, which hide part of their HBM latency.  Tiles are dealt XCD by XCD
+// in 8 x 8 patches so the operand rows of a patch stay in that XCD's L2.
+// Epilogue: D, and for the BA step the update p~ exp(D - dref), the block
+// partials and the next step's P limbs (LP of them) with one exponent per
+// 128 x 32 tile.
+template <int L>
+struct G2Rows {
+  static constexpr int P1 = 128 + 64 * L;
+  static constexpr int P2 = 128 * L + 64;
+  static constexpr int MAXR = P1 > P2 ? P1 : P2;
+  static constexpr int LD = 72;  // halves per LDS row (64 + 8 pad)
 };
-template <int L, int EPI>
-__global__ void __launch_bounds__(512) k_split_g2(int N1, int N2, int KL, int KLp, int KR, int KRp,
-                                                  const _Float16 *__restrict__ A16, const _Float16 *__restrict__ vT,
-                                                  const _Float16 *__restrict__ wl, const _Float16 *__restrict__ B16,
-                                                  int bits, EpiArgs ea, G2Resid rs, G2Out go) {
-  constexpr int BM = 128, BN = 128, BK = 32, PADK = BK + 8;
-  static_assert(BM == BN, "slabs are shared between the A and B roles");
-  // L + 1 slabs of 128 x 32 per stage: phase 1 uses slab 0 for A16 and slabs
-  // 1..L for the V limbs; phase 2 uses slabs 0..L-1 for the W limbs and slab L for B16.
-  __shared__ __attribute__((aligned(16))) _Float16 Sl[2][L + 1][BM][PADK];
+
+template <int L, int LP, int EPI, bool RES>
+__global__ void __launch_bounds__(256) k_split_g2(int N2, int KLp, int KRp, const _Float16 *__restrict__ A16,
+                                                 const _Float16 *__restrict__ vT, const _Float16 *__restrict__ wl,
+                                                 const _Float16 *__restrict__ B16, int bits, EpiArgs ea, G2Resid rs,
+                                                 G2Out go, const int32_t *__restrict__ tiles) {
+  using G = G2Rows<L>;
+  constexpr int BM = 128, BN = 64, BK = 64, LD = G::LD;
+  constexpr int NR1 = G::P1 / 32, NR2 = G::P2 / 32, NRM = NR1 > NR2 ? NR1 : NR2;  // rows per thread
+  constexpr int SMH = G::MAXR * LD > 64 * 130 * 4 ? G::MAXR * LD : 64 * 130 * 4;  // halves: stages, or the update tile
+  __shared__ __attribute__((aligned(16))) _Float16 sm[SMH];
   __shared__ double red[48];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = wave >> 1, wn = wave & 1;
+  // the exp table lives in LDS: a global table could alias the P stores, which
+  // would order every lookup behind the previous element's store
+  __shared__ double etab[32];
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int TX = N2 / BN;
   int ty, tx;
-  if (go.patches) {
+  if constexpr (RES) {  // tiles with residuals, from a list
+    ty = tiles[blockIdx.x] / TX;
+    tx = tiles[blockIdx.x] % TX;
+  } else if (go.patches) {
     const int b = blockIdx.x, xcd = b & 7, sq = b >> 3;
     const int patch = (sq >> 6) * 8 + xcd, w = sq & 63;
     ty = (patch / (TX >> 3)) * 8 + (w >> 3);
@@ -315,6 +372,48 @@ __global__ void __launch_bounds__(512) k_split_g2(int N1, int N2, int KL, int KL
     tx = blockIdx.x % TX;
   }
   const int m0 = ty * BM, n0 = tx * BN;
+  // tiles touched by a residual of the rounded counts run in the second launch
+  if (!RES && (rs.blkA[m0 / 128] || rs.blkB[n0 / 64])) return;
+  if (EPI == EPI_BA && tid < 32) etab[tid] = go.exptab[tid];
+  const int nst1 = KLp / BK, nst2 = KRp / BK, S = nst1 + nst2;
+  const int q8 = (tid & 7) * 8, rbase = tid >> 3;  // 16-byte piece of a row, first row of this thread
+  const int lr = lane & 31, lk = 8 * (lane >> 5);
+
+  half8 rg[NRM];
+  auto gload = [&](int s) {
+    if (s < nst1) {
+      const int k0 = s * BK + q8;
+#pragma unroll
+      for (int i = 0; i < NR1; ++i) {
+        const int rr = rbase + 32 * i;
+        const _Float16 *src;
+        if (rr < 128)
+          src = A16 + (int64_t)(m0 + rr) * KLp + k0;
+        else
+          src = vT + (int64_t)((rr - 128) >> 6) * N2 * KLp + (int64_t)(n0 + ((rr - 128) & 63)) * KLp + k0;
+        rg[i] = *reinterpret_cast<const half8 *>(src);
+      }
+    } else {
+      const int k0 = (s - nst1) * BK + q8;
+#pragma unroll
+      for (int i = 0; i < NR2; ++i) {
+        const int rr = rbase + 32 * i;
+        const _Float16 *src;
+        if (rr < 128 * L)
+          src = wl + (int64_t)(rr >> 7) * go.M * KRp + (int64_t)(m0 + (rr & 127)) * KRp + k0;
+        else
+          src = B16 + (int64_t)(n0 + rr - 128 * L) * KRp + k0;
+        rg[i] = *reinterpret_cast<const half8 *>(src);
+      }
+    }
+  };
+  auto sstore = [&](int s) {
+    const int nr = s < nst1 ? NR1 : NR2;
+#pragma unroll
+    for (int i = 0; i < NRM; ++i)
+      if (i < nr) *reinterpret_cast<half8 *>(&sm[(rbase + 32 * i) * LD + q8]) = rg[i];
+  };
+  auto opnd = [&](int row, int ks) -> half8 { return *reinterpret_cast<const half8 *>(&sm[row * LD + ks * 16 + lk]); };
 
   float16_ acc[L][2];
 #pragma unroll
@@ -322,148 +421,104 @@ __global__ void __launch_bounds__(512) k_split_g2(int N1, int N2, int KL, int KL
 #pragma unroll
     for (int j = 0; j < 2; ++j) acc[l][j] = (float16_){0};
 
-  // one 16-byte chunk per thread covers a 128 x 32 slab
-  auto load_slab = [&](const _Float16 *base, int64_t ld, int row0, int k0, int kmax) -> uint4 {
-    const int row = tid >> 2, kq = tid & 3;
-    const int kk = k0 + kq * 8;
-    if (kk >= kmax) return make_uint4(0, 0, 0, 0);
-    return *reinterpret_cast<const uint4 *>(base + (int64_t)(row0 + row) * ld + kk);
-  };
-  auto store_slab = [&](int buf, int slab, uint4 v) {
-    const int row = tid >> 2, kq = tid & 3;
-    *reinterpret_cast<uint4 *>(&Sl[buf][slab][row][kq * 8]) = v;
-  };
+  // lane element (j, r) of the tile: row m0 + 32 wave + mfma32_row(lane, r), column n0 + 32 j + lane % 32,
+  // i.e. a wave-uniform base plus the lane offset lo plus a uniform offset per element
+  const int64_t ldx = ea.ldx;
+  const int64_t ubase = (int64_t)(m0 + wave * 32) * ldx + n0;
+  const int lo = (lane >> 5) * 4 * (int)ldx + lr;
+  const double *Hu = ea.Hc + ubase;
+  double *Pu = EPI == EPI_BA ? ea.P + ubase : ea.D + ubase;
+  double hv[2][16], pv[2][16];
 
-  // ---- phase 1: A16 (shared) x V limbs
-  {
-    const int nst = (KL + BK - 1) / BK;
-    uint4 r[L + 1];
-    auto gl = [&](int k0) {
-      r[0] = load_slab(A16, KLp, m0, k0, KLp);
+  gload(0);
+  auto mult = [&](int s) {
+    if (s < nst1) {
 #pragma unroll
-      for (int l = 0; l < L; ++l) r[l + 1] = load_slab(vT + (int64_t)l * N2 * KLp, KLp, n0, k0, KLp);
-    };
-    auto st = [&](int buf) {
+      for (int ks = 0; ks < BK / 16; ++ks) {
+        const half8 a = opnd(wave * 32 + lr, ks);
 #pragma unroll
-      for (int q = 0; q <= L; ++q) store_slab(buf, q, r[q]);
-    };
-    if (nst > 0) {
-      gl(0);
-      st(0);
-      __syncthreads();
-    }
-    for (int s = 0; s < nst; ++s) {
-      const int buf = s & 1;
-      if (s + 1 < nst) gl((s + 1) * BK);
+        for (int l = 0; l < L; ++l)
 #pragma unroll
-      for (int kk = 0; kk < BK / 16; ++kk) {
-        const int kof = kk * 16 + 8 * (lane >> 5);
-        const half8 a = *reinterpret_cast<const half8 *>(&Sl[buf][0][wm * 32 + (lane & 31)][kof]);
-#pragma unroll
-        for (int j = 0; j < 2; ++j)
-#pragma unroll
-          for (int l = 0; l < L; ++l) {
-            const half8 b = *reinterpret_cast<const half8 *>(&Sl[buf][l + 1][wn * 64 + j * 32 + (lane & 31)][kof]);
-            acc[l][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, acc[l][j], 0, 0, 0);
-          }
+          for (int j = 0; j < 2; ++j)
+            acc[l][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, opnd(128 + l * 64 + j * 32 + lr, ks), acc[l][j], 0, 0,
+                                                               0);
       }
-      if (s + 1 < nst) {
-        st(buf ^ 1);
-        __syncthreads();
-      }
-    }
-    __syncthreads();
-  }
-  // ---- phase 2: W limbs x B16 (shared)
-  if (KR > 0) {
-    const int nst = (KR + BK - 1) / BK;
-    uint4 r[L + 1];
-    auto gl = [&](int k0) {
+    } else {
 #pragma unroll
-      for (int l = 0; l < L; ++l) r[l] = load_slab(wl + (int64_t)l * N1 * KRp, KRp, m0, k0, KRp);
-      r[L] = load_slab(B16, KRp, n0, k0, KRp);
-    };
-    auto st = [&](int buf) {
+      for (int ks = 0; ks < BK / 16; ++ks) {
+        half8 bb[2];
 #pragma unroll
-      for (int q = 0; q <= L; ++q) store_slab(buf, q, r[q]);
-    };
-    gl(0);
-    st(0);
-    __syncthreads();
-    for (int s = 0; s < nst; ++s) {
-      const int buf = s & 1;
-      if (s + 1 < nst) gl((s + 1) * BK);
-#pragma unroll
-      for (int kk = 0; kk < BK / 16; ++kk) {
-        const int kof = kk * 16 + 8 * (lane >> 5);
-        half8 b[2];
-#pragma unroll
-        for (int j = 0; j < 2; ++j) b[j] = *reinterpret_cast<const half8 *>(&Sl[buf][L][wn * 64 + j * 32 + (lane & 31)][kof]);
+        for (int j = 0; j < 2; ++j) bb[j] = opnd(128 * L + j * 32 + lr, ks);
 #pragma unroll
         for (int l = 0; l < L; ++l) {
-          const half8 a = *reinterpret_cast<const half8 *>(&Sl[buf][l][wm * 32 + (lane & 31)][kof]);
+          const half8 a = opnd(l * 128 + wave * 32 + lr, ks);
 #pragma unroll
-          for (int j = 0; j < 2; ++j) acc[l][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b[j], acc[l][j], 0, 0, 0);
+          for (int j = 0; j < 2; ++j) acc[l][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, bb[j], acc[l][j], 0, 0, 0);
         }
       }
-      if (s + 1 < nst) {
-        st(buf ^ 1);
-        __syncthreads();
-      }
     }
+  };
+  for (int s = 0; s + 1 < S; ++s) {
+    if (s > 0) __syncthreads();  // the previous stage has been read
+    sstore(s);
     __syncthreads();
+    gload(s + 1);
+    mult(s);
   }
+  if (S > 1) __syncthreads();
+  sstore(S - 1);
+  __syncthreads();
+  // the H and P loads of the epilogue overlap the last stage's multiplications
+#pragma unroll
+  for (int j = 0; j < 2; ++j)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int64_t off = (int64_t)((r >> 2) * 8 + (r & 3)) * ldx + j * 32;
+      hv[j][r] = __builtin_nontemporal_load(Hu + off + lo);
+      if constexpr (EPI == EPI_BA) pv[j][r] = __builtin_nontemporal_load(Pu + off + lo);
+    }
+  mult(S - 1);
+  __syncthreads();  // etab visible; the stage buffer is free from here on
 
-  // ---- epilogue pass 1: combine limbs into S (FP64) and apply residuals
+  // ---- epilogue, one element at a time: S from the limb sums (plus the
+  // residuals of rounded counts), D, and for the BA step the update, whose
+  // result also goes to LDS (column-major, 130-double columns) for the limbs.
   const int ev = (int)ea.scal[kScalVScale];
   double sc[L];
 #pragma unroll
   for (int l = 0; l < L; ++l) sc[l] = ldexp(1.0, ev - bits * l);
-  double Sv[2][16];
+  const double mu = ea.scal[kScalMu], dref = ea.scal[kScalDRef];
+  constexpr int WLD = 130;
+  double *wt = reinterpret_cast<double *>(sm);  // [64 columns][WLD]
+  double s0 = 0.0, m1 = -INFINITY, s2 = 0.0, wmax[2] = {0.0, 0.0};
 #pragma unroll
   for (int j = 0; j < 2; ++j)
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
       double S = 0.0;
 #pragma unroll
-      for (int l = 0; l < L; ++l) S += (double)acc[l][j][r] * sc[l];
-      Sv[j][r] = S;
-    }
-  if (rs.blkA[ty] || rs.blkB[tx]) {
-#pragma unroll
-    for (int j = 0; j < 2; ++j)
-#pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const int x1 = m0 + wm * 32 + mfma32_row(lane, r);
-        const int x2 = n0 + wn * 64 + j * 32 + (lane & 31);
+      for (int l = 0; l < L; ++l) S = fma((double)acc[l][j][r], sc[l], S);
+      const int xl1 = wave * 32 + mfma32_row(lane, r), xl2 = j * 32 + lr;
+      if constexpr (RES) {
+        const int x1 = m0 + xl1, x2 = n0 + xl2;
         for (int e = rs.rowptrA[x1]; e < rs.rowptrA[x1 + 1]; ++e)
-          Sv[j][r] += (double)rs.entA[2 * e + 1] * rs.V64[(int64_t)rs.entA[2 * e] * N2 + x2];
+          S += (double)rs.entA[2 * e + 1] * rs.V64[(int64_t)rs.entA[2 * e] * N2 + x2];
         for (int e = rs.rowptrB[x2]; e < rs.rowptrB[x2 + 1]; ++e)
-          Sv[j][r] += rs.W64[(int64_t)x1 * rs.KRp + rs.entB[2 * e]] * (double)rs.entB[2 * e + 1];
+          S += rs.W64[(int64_t)x1 * rs.KRp + rs.entB[2 * e]] * (double)rs.entB[2 * e + 1];
       }
-  }
-  // ---- epilogue pass 2: D, the update, block partials
-  const double mu = ea.scal[kScalMu], dref = ea.scal[kScalDRef];
-  double s0 = 0.0, m1 = -INFINITY, s2 = 0.0, wmax = 0.0;
-#pragma unroll
-  for (int j = 0; j < 2; ++j)
-#pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      const int x1 = m0 + wm * 32 + mfma32_row(lane, r);
-      const int x2 = n0 + wn * 64 + j * 32 + (lane & 31);
-      const int64_t x = (int64_t)x1 * ea.ldx + x2;
-      const double D = ea.Hc[x] - mu - Sv[j][r] * ea.inv_c;
+      const int64_t off = (int64_t)((r >> 2) * 8 + (r & 3)) * ldx + j * 32;
+      const double D = fma(-S, ea.inv_c, hv[j][r] - mu);
       m1 = fmax(m1, D);
       if constexpr (EPI == EPI_BA) {
-        const double p = ea.P[x];
-        const double w = p * exp(D - dref);
-        ea.P[x] = w;
+        const double p = pv[j][r];
+        const double w = p * exp_tang(D - dref, etab);
+        __builtin_nontemporal_store(w, Pu + off + lo);
         s0 += w;
-        s2 += p * D;
-        Sv[j][r] = w;
-        wmax = fmax(wmax, w);
+        s2 = fma(p, D, s2);
+        wmax[j] = fmax(wmax[j], w);
+        wt[xl2 * WLD + xl1] = w;
       } else {
-        ea.D[x] = D;
+        Pu[off + lo] = D;
       }
     }
   block_reduce3(s0, m1, s2, red);
@@ -473,68 +528,64 @@ __global__ void __launch_bounds__(512) k_split_g2(int N1, int N2, int KL, int KL
     pp[1] = m1;
     pp[2] = s2;
   }
-  // ---- epilogue pass 3 (BA, fused): limbs of the new P tile for the next output law
-  if constexpr (EPI == EPI_BA) {
-    if (go.limbT) {
-      wmax = block_max(wmax, red + 16);
-      const int et = tile_exponent(wmax, go.bits_p);
-      if (tid == 0) go.texp[ty * TX + tx] = et;
-      constexpr int PITCH = 136;
-      _Float16(*tileT)[PITCH] = reinterpret_cast<_Float16(*)[PITCH]>(&Sl[0][0][0][0]);  // 128 x 136, 34 KB
-      const double radix = ldexp(1.0, go.bits_p);
+  // ---- BA step: limbs of the new P tile for the next output law, one
+  // exponent per 128 x 32 half.  A thread takes 8 consecutive rows of a column
+  // and writes each limb as one 16-byte store (16 threads cover a column's 128
+  // rows, 256 contiguous bytes).
+  if constexpr (EPI == EPI_BA && LP > 0) {
+    const double mw0 = block_max(wmax[0], red + 16);  // leading barrier: wt complete
+    const double mw1 = block_max(wmax[1], red + 32);
+    const int et0 = tile_exponent(mw0, go.bits_p), et1 = tile_exponent(mw1, go.bits_p);
+    if (tid < 2) go.texp[tile_index(go.map, m0, n0 + 32 * tid)] = tid ? et1 : et0;
+    const double radix = ldexp(1.0, go.bits_p);
+    const int64_t lstride = (int64_t)go.map.ncols * go.map.kdim;
+    const int xr = (tid & 15) * 8;
 #pragma unroll
-      for (int j = 0; j < 2; ++j)
+    for (int it = 0; it < 4; ++it) {
+      const int c = (tid >> 4) + 16 * it;
+      const int et = c < 32 ? et0 : et1;
+      double v[8];
 #pragma unroll
-        for (int r = 0; r < 16; ++r) Sv[j][r] = ldexp(Sv[j][r], -et);
-      for (int l = 0; l < go.limbs_p; ++l) {
+      for (int q = 0; q < 8; ++q) v[q] = ldexp(wt[c * WLD + xr + q], -et);
+      _Float16 *dst = go.limbT + (int64_t)(n0 + c) * go.map.kdim + m0 + xr;
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
+      for (int l = 0; l < LP; ++l) {
+        half8 h;
 #pragma unroll
-          for (int r = 0; r < 16; ++r) {
-            const int xl1 = wm * 32 + mfma32_row(lane, r);
-            const int xl2 = wn * 64 + j * 32 + (lane & 31);
-            const double li = rint(Sv[j][r]);
-            tileT[xl2][xl1] = (_Float16)(float)li;
-            Sv[j][r] = (Sv[j][r] - li) * radix;
-          }
-        __syncthreads();
-        _Float16 *dst = go.limbT + (int64_t)l * N1 * N2;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const int ch = tid + 512 * q, c2 = ch >> 4, g = ch & 15;
-          *reinterpret_cast<uint4 *>(dst + (int64_t)(n0 + c2) * N1 + m0 + 8 * g) =
-              *reinterpret_cast<const uint4 *>(&tileT[c2][8 * g]);
+        for (int q = 0; q < 8; ++q) {
+          const double li = rint(v[q]);
+          h[q] = (_Float16)(float)li;
+          v[q] = (v[q] - li) * radix;
         }
-        __syncthreads();
+        *reinterpret_cast<half8 *>(dst + l * lstride) = h;
       }
     }
   }
 }
 
 // ----------------------------------------------------------------- launchers
-#define BDC_SPLIT_L(L, call) \
-  switch (L) {               \
-    case 2: call(2); break;  \
-    case 3: call(3); break;  \
-    case 4: call(4); break;  \
-    case 5: call(5); break;  \
+#define BDC_SPLIT_L(L, call)     \
+  switch (L) {                   \
+    case 2: call(2); break;      \
+    case 3: call(3); break;      \
+    case 4: call(4); break;      \
     default: return BDC_ERR_ARG; \
   }
 
-int launch_split_p(const double *P, const double *scal, int N1, int N2, int limbs, int bits, _Float16 *limbT,
-                   int32_t *texp, hipStream_t s) {
-  if (N1 % 128 || N2 % 128) return BDC_ERR_ARG;
-  dim3 grid(N2 / 128, N1 / 128);
-#define LP(LL) hipLaunchKernelGGL(k_split_p<LL>, grid, dim3(1024), 0, s, P, scal, N1, N2, bits, limbT, texp)
+int launch_split_p(const double *P, int M, int N2, int limbs, int bits, _Float16 *limbT, int32_t *texp, hipStream_t s) {
+  if (M % 128 || N2 % 64) return BDC_ERR_ARG;
+  LimbMap mp{M, N2};
+  dim3 grid(N2 / 64, M / 128);
+#define LP(LL) hipLaunchKernelGGL(k_split_p<LL>, grid, dim3(256), 0, s, P, N2, bits, mp, limbT, texp)
   BDC_SPLIT_L(limbs, LP)
 #undef LP
   return hipGetLastError() == hipSuccess ? BDC_OK : BDC_ERR_HIP;
 }
 
-int launch_split_vw(const double *V, int KL, int KLp, int N2, const double *W, int N1, int KR, int KRp, int limbs,
+int launch_split_vw(const double *V, int KL, int KLp, int N2, const double *W, int M, int KR, int KRp, int limbs,
                     int bits, _Float16 *vT, _Float16 *wl, double *part, double *scal, hipStream_t s) {
   const int np = 512;
-  hipLaunchKernelGGL(k_absmax_vw, dim3(np), dim3(256), 0, s, V, (int64_t)KL * N2, W, N1, KR, KRp, part);
+  hipLaunchKernelGGL(k_absmax_vw, dim3(np), dim3(256), 0, s, V, (int64_t)KL * N2, W, M, KR, KRp, part);
   hipLaunchKernelGGL(k_vw_exponent, dim3(1), dim3(256), 0, s, part, np, bits, scal);
   if (N2 % 64 || KLp % 64) return BDC_ERR_ARG;
   dim3 gv(N2 / 64, KLp / 64);
@@ -542,66 +593,82 @@ int launch_split_vw(const double *V, int KL, int KLp, int N2, const double *W, i
   BDC_SPLIT_L(limbs, LV)
 #undef LV
   if (KR > 0) {
-    const int64_t len = (int64_t)N1 * KRp;
-#define LW(LL) hipLaunchKernelGGL(k_split_w<LL>, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, s, W, len, KR, KRp, bits, scal, wl)
+    const int64_t len = (int64_t)M * KRp;
+#define LW(LL) \
+  hipLaunchKernelGGL(k_split_w<LL>, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, s, W, len, KR, KRp, bits, scal, wl)
     BDC_SPLIT_L(limbs, LW)
 #undef LW
   }
   return hipGetLastError() == hipSuccess ? BDC_OK : BDC_ERR_HIP;
 }
 
-int launch_split_g1(int KLp, int N1, int N2, const _Float16 *aT, const _Float16 *pT, const int32_t *texp, int limbs,
-                    int bits, int ksplit, double *T, hipStream_t s) {
-  if (N1 % 128 || N2 % 128 || ksplit < 1 || (N1 / 128) % ksplit) return BDC_ERR_ARG;
-  const int krows = N1 / ksplit;
-  dim3 grid(N2 / 32, (KLp + 255) / 256, ksplit);
-#define LG(LL) hipLaunchKernelGGL(k_split_g1<LL>, grid, dim3(512), 0, s, KLp, N1, N2, aT, pT, texp, bits, krows, T)
-  switch (limbs) {
-    case 2: LG(2); break;
-    case 3: LG(3); break;
-    case 4: LG(4); break;
-    default: return BDC_ERR_ARG;
-  }
+int launch_split_g1(int KLp, int Kdim, int lda, int ncols, const _Float16 *aT, const _Float16 *pT, const int32_t *texp,
+                    int limbs, int bits, int ks, double *Tpart, hipStream_t s) {
+  if (ks < 1 || Kdim % (ks * 128) || ncols % 64 || lda < Kdim || lda % 8) return BDC_ERR_ARG;
+  const int blocks = ks * (ncols / 32) * ((KLp + 255) / 256);
+#define LG(LL)                                                                                                   \
+  hipLaunchKernelGGL(k_split_g1<LL>, dim3(blocks), dim3(512), 0, s, KLp, Kdim, lda, ncols, aT, pT, texp, bits, ks, \
+                     Tpart)
+  BDC_SPLIT_L(limbs, LG)
 #undef LG
   return hipGetLastError() == hipSuccess ? BDC_OK : BDC_ERR_HIP;
 }
 
-int launch_resid_g1(const int32_t *res, int nres, const double *P, int N2, double *T, hipStream_t s) {
+int launch_resid_g1(const int32_t *res, int nres, const double *X, int rows, int N2, double *T, hipStream_t s) {
   if (nres == 0) return BDC_OK;
-  hipLaunchKernelGGL(k_resid_g1, dim3((unsigned)((N2 + 255) / 256)), dim3(256), 0, s, res, nres, P, N2, T);
+  hipLaunchKernelGGL(k_resid_g1, dim3((unsigned)((N2 + 255) / 256)), dim3(256), 0, s, res, nres, X, rows, N2, T);
   return hipGetLastError() == hipSuccess ? BDC_OK : BDC_ERR_HIP;
 }
 
-int launch_split_g2(int N1, int N2, int KL, int KLp, int KR, int KRp, const _Float16 *A16, const _Float16 *vT,
-                    const _Float16 *wl, const _Float16 *B16, int limbs, int bits, int epi, const EpiArgs &ea,
-                    const uint8_t *blkA, const uint8_t *blkB, const int32_t *rowptrA, const int32_t *entA,
-                    const int32_t *rowptrB, const int32_t *entB, const double *V64, const double *W64,
-                    _Float16 *limbT_next, int32_t *texp_next, int limbs_p, int bits_p, hipStream_t s, int *nb) {
-  if (N1 % 128 || N2 % 128) return BDC_ERR_ARG;
-  G2Resid rs{blkA, blkB, rowptrA, entA, rowptrB, entB, V64, W64, KRp};
-  const int TX = N2 / 128, TY = N1 / 128;
-  G2Out go{limbT_next, texp_next, limbs_p, bits_p, (TX % 8 == 0 && TY % 8 == 0 && ((TX / 8) * (TY / 8)) % 8 == 0) ? 1 : 0};
-  if (limbT_next && (limbs_p < 1 || limbs_p > 4)) return BDC_ERR_ARG;
+int launch_split_g2(const SplitG2Args &a, const EpiArgs &ea, hipStream_t s, int *nb) {
+  if (a.M % 128 || a.N2 % 64 || a.KLp % 64 || a.KRp % 64) return BDC_ERR_ARG;
+  G2Resid rs{a.blkA, a.blkB, a.rowptrA, a.entA, a.rowptrB, a.entB, a.V64, a.W64, a.KRp};
+  const int TX = a.N2 / 64, TY = a.M / 128;
+  G2Out go{};
+  go.limbT = a.limbT_next;
+  go.texp = a.texp_next;
+  go.bits_p = a.bits_p;
+  go.map = LimbMap{a.M, a.N2};
+  go.patches = (TX % 8 == 0 && TY % 8 == 0 && ((TX / 8) * (TY / 8)) % 8 == 0) ? 1 : 0;
+  go.M = a.M;
+  go.exptab = a.exptab;
+  const int lp = a.epi == EPI_BA ? a.limbs_p : 0;
+  if (a.epi == EPI_BA && (!a.limbT_next || !a.texp_next)) return BDC_ERR_ARG;
   const int blocks = TX * TY;
   if (nb) *nb = blocks;
-#define LG2(LL, EE) hipLaunchKernelGGL((k_split_g2<LL, EE>), dim3(blocks), dim3(512), 0, s, N1, N2, KL, KLp, KR, KRp, A16, vT, wl, B16, bits, ea, rs, go)
-  if (epi == EPI_BA) {
-    switch (limbs) {
-      case 2: LG2(2, EPI_BA); break;
-      case 3: LG2(3, EPI_BA); break;
-      case 4: LG2(4, EPI_BA); break;
+#define LG2(LL, LPP, EE)                                                                                           \
+  do {                                                                                                             \
+    hipLaunchKernelGGL((k_split_g2<LL, LPP, EE, false>), dim3(blocks), dim3(256), 0, s, a.N2, a.KLp, a.KRp, a.A16, \
+                       a.vT, a.wl, a.B16, a.bits, ea, rs, go, nullptr);                                            \
+    if (a.nres_tiles)                                                                                              \
+      hipLaunchKernelGGL((k_split_g2<LL, LPP, EE, true>), dim3(a.nres_tiles), dim3(256), 0, s, a.N2, a.KLp, a.KRp, \
+                         a.A16, a.vT, a.wl, a.B16, a.bits, ea, rs, go, a.res_tiles);                                \
+  } while (0)
+#define LG2_LP(LL)                            \
+  switch (lp) {                               \
+    case 2: LG2(LL, 2, EPI_BA); break;        \
+    case 3: LG2(LL, 3, EPI_BA); break;        \
+    case 4: LG2(LL, 4, EPI_BA); break;        \
+    default: return BDC_ERR_ARG;              \
+  }
+  if (a.epi == EPI_BA) {
+    switch (a.limbs) {
+      case 2: LG2_LP(2); break;
+      case 3: LG2_LP(3); break;
+      case 4: LG2_LP(4); break;
       default: return BDC_ERR_ARG;
     }
-  } else if (epi == EPI_D) {
-    switch (limbs) {
-      case 2: LG2(2, EPI_D); break;
-      case 3: LG2(3, EPI_D); break;
-      case 4: LG2(4, EPI_D); break;
+  } else if (a.epi == EPI_D) {
+    switch (a.limbs) {
+      case 2: LG2(2, 0, EPI_D); break;
+      case 3: LG2(3, 0, EPI_D); break;
+      case 4: LG2(4, 0, EPI_D); break;
       default: return BDC_ERR_ARG;
     }
   } else {
     return BDC_ERR_ARG;
   }
+#undef LG2_LP
 #undef LG2
   return hipGetLastError() == hipSuccess ? BDC_OK : BDC_ERR_HIP;
 }

@@ -1,5 +1,7 @@
 // Streaming and reduction kernels of a BA step: the output-law combine with
 // ln q, the finalize of the fused update, the dual-bound weights and Adam.
+#include <algorithm>
+
 #include "device_common.hpp"
 #include "kernels.hpp"
 
@@ -12,7 +14,9 @@ namespace bdc {
 //   mx_t  = max_x log2(p_{t+1}/p_t) = (max D_t - ln Z_t)/ln 2   (BAA.cu:916-946)
 //   pscale' = 1 / sum P~',  dref' = ln Z_t.
 // Partials are combined in a fixed order, so the result is deterministic.
-__global__ void __launch_bounds__(256) k_ba_finalize(const double *__restrict__ part, int nblocks,
+// mult = 2 when the update ran on the half space x1 < N1/2: the sums over the
+// other half (complement images, p(~x) = p(x), D(~x) = D(x)) are equal.
+__global__ void __launch_bounds__(256) k_ba_finalize(const double *__restrict__ part, int nblocks, double mult,
                                                      double *__restrict__ scal, double *__restrict__ hist,
                                                      int hist_idx) {
   __shared__ double red[48];
@@ -23,6 +27,8 @@ __global__ void __launch_bounds__(256) k_ba_finalize(const double *__restrict__ 
     s2 += part[(int64_t)b * kPartStrideDev + 2];
   }
   block_reduce3(s0, m1, s2, red);
+  s0 *= mult;
+  s2 *= mult;
   if (threadIdx.x == 0) {
     const double pscale = scal[kScalPScale], dref = scal[kScalDRef];
     const double lnz = log(pscale) + dref + log(s0);
@@ -42,43 +48,77 @@ __global__ void __launch_bounds__(256) k_ba_finalize(const double *__restrict__ 
     }
   }
 }
-int launch_ba_finalize(const double *part, int nblocks, double *scal, double *hist, int hist_idx, hipStream_t s) {
-  hipLaunchKernelGGL(k_ba_finalize, dim3(1), dim3(256), 0, s, part, nblocks, scal, hist, hist_idx);
+int launch_ba_finalize(const double *part, int nblocks, double mult, double *scal, double *hist, int hist_idx,
+                       hipStream_t s) {
+  hipLaunchKernelGGL(k_ba_finalize, dim3(1), dim3(256), 0, s, part, nblocks, mult, scal, hist, hist_idx);
   return hipGetLastError() == hipSuccess ? BDC_OK : BDC_ERR_HIP;
 }
 
 // q(y) = pscale/C * (sum_{j in JL} q_j(y) + sum_{j in JR} q_j(y)).  For a
 // reversal-symmetric p the JR slices equal JL slices read at rev(y)
-// (q_j(y) = q_{k-j}(rev y)), which the mirror mode uses.  Then
+// (q_j(y) = q_{k-j}(rev y)), which the mirror mode uses.  In complement mode
+// the slices hold R_j, the contraction over the half space x1 < N1/2 only,
+// and q_j(y) = R_j(y) + R_j(~y) (the other half is {~x}, c(~x, y) = c(x, ~y)).
+// Then
 // L(y) = ln q(y) - mu with mu the mean of ln q (any constant is exact since
 // sum_y W(y|x) = 1; centring keeps |L| small for the split contraction).
-__global__ void __launch_bounds__(1024) k_q_combine(QCombine qc, double inv_c, double *__restrict__ scal,
-                                                    double *__restrict__ qout, double *__restrict__ Lout) {
+// Grid pass: q(y), ln q(y) into Lout, per-block sums of ln q.
+__global__ void __launch_bounds__(256) k_q_combine(QCombine qc, double inv_c, const double *__restrict__ scal,
+                                                   double *__restrict__ qout, double *__restrict__ Lout,
+                                                   double *__restrict__ part) {
   __shared__ double red[16];
   const uint32_t ny = 1u << qc.k;
   const double f = scal[qc.scale_slot] * inv_c;
-  double sl = 0.0;
-  for (uint32_t y = threadIdx.x; y < ny; y += blockDim.x) {
+  const uint32_t cm = ny - 1u;
+  const uint32_t y = blockIdx.x * blockDim.x + threadIdx.x;
+  double lq = 0.0;
+  if (y < ny) {
     double s = 0.0;
-    for (int i = 0; i < qc.nL; ++i) s += qc.qL[(int64_t)i * ny + y];
+    for (int i = 0; i < qc.nL; ++i) {
+      s += qc.qL[(int64_t)i * ny + y];
+      if (qc.comp) s += qc.qL[(int64_t)i * ny + (y ^ cm)];
+    }
     if (qc.mirror) {
       const uint32_t ry = rev_bits(y, qc.k);
       for (int i = 0; i < qc.nL; ++i)
-        if ((qc.mirrorSel >> i) & 1ull) s += qc.qL[(int64_t)i * ny + ry];
+        if ((qc.mirrorSel >> i) & 1ull) {
+          s += qc.qL[(int64_t)i * ny + ry];
+          if (qc.comp) s += qc.qL[(int64_t)i * ny + (ry ^ cm)];
+        }
     } else {
-      for (int i = 0; i < qc.nR; ++i) s += qc.qR[(int64_t)i * ny + y];
+      for (int i = 0; i < qc.nR; ++i) {
+        s += qc.qR[(int64_t)i * ny + y];
+        if (qc.comp) s += qc.qR[(int64_t)i * ny + (y ^ cm)];
+      }
     }
     const double q = s * f;
     qout[y] = q;
-    sl += log(q);
+    lq = log(q);
+    Lout[y] = lq;
   }
-  sl = block_sum(sl, red);
-  const double mu = sl / (double)ny;
-  for (uint32_t y = threadIdx.x; y < ny; y += blockDim.x) Lout[y] = log(qout[y]) - mu;
-  if (threadIdx.x == 0) scal[kScalMu] = mu;
+  lq = block_sum(lq, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = lq;
 }
-int launch_q_combine(const QCombine &qc, double inv_c, double *scal, double *qout, double *Lout, hipStream_t s) {
-  hipLaunchKernelGGL(k_q_combine, dim3(1), dim3(1024), 0, s, qc, inv_c, scal, qout, Lout);
+// Centre: every block sums the same partials in the same order (so all agree
+// on mu), then L = ln q - mu.
+__global__ void __launch_bounds__(256) k_q_center(int k, const double *__restrict__ part, int np,
+                                                  double *__restrict__ scal, double *__restrict__ Lout) {
+  __shared__ double red[16];
+  const uint32_t ny = 1u << k;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < np; i += blockDim.x) s += part[i];
+  s = block_sum(s, red);
+  const double mu = s / (double)ny;
+  const uint32_t y = blockIdx.x * blockDim.x + threadIdx.x;
+  if (y < ny) Lout[y] -= mu;
+  if (blockIdx.x == 0 && threadIdx.x == 0) scal[kScalMu] = mu;
+}
+int launch_q_combine(const QCombine &qc, double inv_c, double *scal, double *qout, double *Lout, double *part,
+                     hipStream_t s) {
+  const uint32_t ny = 1u << qc.k;
+  const int nb = (int)((ny + 255) / 256);
+  hipLaunchKernelGGL(k_q_combine, dim3(nb), dim3(256), 0, s, qc, inv_c, scal, qout, Lout, part);
+  hipLaunchKernelGGL(k_q_center, dim3(nb), dim3(256), 0, s, qc.k, part, nb, scal, Lout);
   return hipGetLastError() == hipSuccess ? BDC_OK : BDC_ERR_HIP;
 }
 
@@ -247,6 +287,20 @@ int launch_dot(const double *a, const double *b, uint64_t len, double *part, dou
   const int np = 1024;
   hipLaunchKernelGGL(k_dot_partial, dim3(np), dim3(256), 0, s, a, b, len, part);
   hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(256), 0, s, part, np, out);
+  return hipGetLastError() == hipSuccess ? BDC_OK : BDC_ERR_HIP;
+}
+
+// Complement image of the half space: P[~x] = P[x] for x < 2^n / 2
+// (~x = 2^n - 1 - x), restoring the rows x1 >= N1/2 after half-space steps.
+__global__ void k_reflect_half(double *__restrict__ P, uint64_t nx) {
+  const uint64_t half = nx >> 1;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < half; i += (uint64_t)gridDim.x * blockDim.x)
+    P[nx - 1 - i] = P[i];
+}
+int launch_reflect_half(double *P, uint64_t nx, hipStream_t s) {
+  const uint64_t half = nx >> 1;
+  const unsigned blocks = (unsigned)std::min<uint64_t>(8192, (half + 255) / 256);
+  hipLaunchKernelGGL(k_reflect_half, dim3(blocks), dim3(256), 0, s, P, nx);
   return hipGetLastError() == hipSuccess ? BDC_OK : BDC_ERR_HIP;
 }
 

@@ -220,3 +220,25 @@ def test_split_matches_fp64_large(bdc, n, k, iters):
     assert np.abs(s16[:, 3] - s64[:, 3]).max() / LN2 < 1e-9
     assert abs(c16["upper_cert"] - c64["upper_cert"]) / LN2 < 1e-9
     assert abs(c16["lower"] - c64["lower"]) / LN2 < 1e-9
+
+
+@pytest.mark.parametrize("n,k,iters", [(16, 8, 8), (20, 10, 5)])
+def test_half_space_matches_full_space(bdc, monkeypatch, n, k, iters):
+    """The BA step on the half space x1 < N1/2 (complement images implied)
+    against the same step over all 2^n inputs, both precisions."""
+    say(f"half vs full space n={n} k={k}")
+    out = {}
+    for mode in ("1", "0"):
+        monkeypatch.setenv("BDC_HALF_SPACE", mode)
+        with bdc.Engine(n, k) as e:
+            for prec in (bdc.PREC_FP64, bdc.PREC_SPLIT16):
+                e.set_p_uniform()
+                st = e.iterate(iters, prec)
+                out[(mode, prec)] = (st, e.certify(), e.get_p_orbits())
+    for prec in (bdc.PREC_FP64, bdc.PREC_SPLIT16):
+        (sh, ch, qh), (sf, cf, qf) = out[("1", prec)], out[("0", prec)]
+        np.testing.assert_allclose(sh, sf, rtol=1e-12, atol=1e-13)
+        np.testing.assert_allclose(qh, qf, rtol=1e-11)
+        assert abs(ch["upper_cert"] - cf["upper_cert"]) < 1e-12
+        assert abs(ch["lower"] - cf["lower"]) < 1e-12
+        assert abs(ch["sum_p"] - 1.0) < 1e-12
