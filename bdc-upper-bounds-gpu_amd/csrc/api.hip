@@ -382,6 +382,30 @@ static int stage_gemm(bdc_ctx *c, int M, int N, int K, const Opnd &A, const Opnd
   return BDC_OK;
 }
 
+// C (M x N, row stride ldc) = A (M x K, FP64, row stride lda) x B(k, n) over a
+// count matrix (transb 0: B[k * ldb + n], 1: B[n * ldb + k]) on the
+// count-streaming kernel, split over K into c->splitk when the grid is small;
+// false when the shape does not fit that kernel.
+static int count_stage(bdc_ctx *c, int M, int N, int K, const double *A, int64_t lda, const uint16_t *B, int64_t ldb,
+                       int transb, double *C, int64_t ldc, bool *done) {
+  *done = false;
+  if (M > 128 || N % 64 || K % 32) return BDC_OK;
+  int ks = 1;
+  while ((N / 64) * ks < 512 && ks < 64 && K / (ks * 2) >= 256 && (K / 32) % (ks * 2) == 0 &&
+         (size_t)(ks * 2) * M * N <= c->splitk_elems)
+    ks *= 2;
+  if (ks == 1) {
+    BDC_LAUNCH(launch_count_gemm(M, N, K, A, lda, B, ldb, transb, C, ldc, 1, 0, c->stream));
+  } else {
+    double *part = c->splitk.as<double>();
+    BDC_LAUNCH(launch_count_gemm(M, N, K, A, lda, B, ldb, transb, part, N, ks, (int64_t)M * N, c->stream));
+    if (ldc != N) return fail(BDC_ERR_STATE, "split count stage needs a contiguous output");
+    BDC_LAUNCH(launch_reduce_splits(part, ks, (int64_t)M * N, (uint64_t)M * N, C, c->stream));
+  }
+  *done = true;
+  return BDC_OK;
+}
+
 // q(y) and L(y) = ln q - mu from weights X (p = scal[scale_slot] * X).
 // mirror: use q_j(y) = q_{k-j}(rev y) for j in JR (X reversal-symmetric, n even).
 // rows = N1/2: X holds only the half space x1 < N1/2 of a complement-symmetric
@@ -422,8 +446,12 @@ static int output_law(bdc_ctx *c, int prec, const double *X, int scale_slot, boo
     } else if (d <= kMaxGatherInsertions) {
       BDC_LAUNCH(launch_ins_gather(Tj, N2, 1, dst, 1 << m, 1, 1 << j, c->n2, d, c->stream));
     } else {
-      const Opnd A{Tj, N2, 1, 0}, B{c->c2(m), 1 << m, 1, 1};
-      BDC_TRY(stage_gemm(c, 1 << j, 1 << m, N2, A, B, dst));
+      bool done = false;
+      BDC_TRY(count_stage(c, 1 << j, 1 << m, N2, Tj, N2, c->c2(m), 1 << m, 0, dst, 1 << m, &done));
+      if (!done) {
+        const Opnd A{Tj, N2, 1, 0}, B{c->c2(m), 1 << m, 1, 1};
+        BDC_TRY(stage_gemm(c, 1 << j, 1 << m, N2, A, B, dst));
+      }
     }
   }
   QCombine qc{};
@@ -477,9 +505,14 @@ static int cross_operands(bdc_ctx *c, const double *L, bool mirror = false, int 
     } else if (d <= kMaxGatherDeletions) {
       BDC_LAUNCH(launch_del_gather(L, 1 << m, 1, dst, N2, 1, 1 << j, c->n2, d, c->stream));
     } else {
-      const Opnd A{L, 1 << m, 1, 0}, B{c->c2(m), 1, 1 << m, 1};
-      const Opnd none{nullptr, 0, 0, 0};
-      BDC_LAUNCH(gemm64(1 << j, N2, A, B, 1 << m, none, none, 0, dst, N2, 1, 1, 0, EPI_STORE, nullptr, c->stream, nullptr));
+      bool done = false;
+      BDC_TRY(count_stage(c, 1 << j, N2, 1 << m, L, 1 << m, c->c2(m), 1 << m, 1, dst, N2, &done));
+      if (!done) {
+        const Opnd A{L, 1 << m, 1, 0}, B{c->c2(m), 1, 1 << m, 1};
+        const Opnd none{nullptr, 0, 0, 0};
+        BDC_LAUNCH(gemm64(1 << j, N2, A, B, 1 << m, none, none, 0, dst, N2, 1, 1, 0, EPI_STORE, nullptr, c->stream,
+                          nullptr));
+      }
     }
   }
   if (mirror && c->n1 == c->n2 && !c->JR.empty()) {

@@ -95,6 +95,11 @@ int launch_ins_gather(const double *in, int64_t irs, int64_t ics, double *out, i
                       int d, hipStream_t s);
 constexpr int kMaxGatherDeletions = 3;   // deletion form: C(n,d) terms per output
 constexpr int kMaxGatherInsertions = 2;  // insertion form: 2^d C(n,d) terms per output
+// C + z * zstride (row stride ldc) = A[M x K-range z] x B over count matrices (u16):
+// transb = 0: B(k, n) = B[k * ldb + n]; transb = 1: B(k, n) = B[n * ldb + k].
+// Needs M <= 128, N % 64 == 0, K % 32 == 0 and (K / 32) % splits == 0; else BDC_ERR_ARG.
+int launch_count_gemm(int M, int N, int K, const double *A, int64_t lda, const uint16_t *B, int64_t ldb, int transb,
+                      double *C, int64_t ldc, int splits, int64_t zstride, hipStream_t s);
 struct MirrorW {  // for each j in JR: column offset in Wbar, m = k - j, row offset of V_{k-j} in Vbar
   int count;
   int offR[18], m[18], offL[18];

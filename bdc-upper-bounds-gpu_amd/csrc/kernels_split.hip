@@ -24,6 +24,16 @@ namespace bdc {
 
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 typedef _Float16 half4 __attribute__((ext_vector_type(4)));
+typedef _Float16 half2_ __attribute__((ext_vector_type(2)));
+
+// Integer-valued doubles of magnitude <= 2^11 to FP16, exactly: their
+// conversion to float is exact, and v_cvt_pkrtz_f16_f32 packs two floats
+// (round toward zero, exact for such integers).  A plain double -> half cast
+// has no instruction and expands into ~150 integer operations.
+__device__ inline half2_ int_to_half2(double a, double b) {
+  return __builtin_bit_cast(half2_, __builtin_amdgcn_cvt_pkrtz((float)a, (float)b));
+}
+__device__ inline _Float16 int_to_half(double a) { return int_to_half2(a, 0.0)[0]; }
 typedef float float16_ __attribute__((ext_vector_type(16)));
 
 // row of the 32x32 MFMA result held by `lane` in register r (verified layout)
@@ -87,7 +97,7 @@ __global__ void __launch_bounds__(256) k_split_p(const double *__restrict__ P, i
     for (int r = 0; r < 32; ++r) {
       const int e = tid + 256 * r, row = e >> 6, col = e & 63;
       const double li = rint(v[r]);
-      tile[col][row] = (_Float16)(float)li;
+      tile[col][row] = int_to_half(li);
       v[r] = (v[r] - li) * radix;
     }
     __syncthreads();
@@ -143,7 +153,7 @@ __global__ void __launch_bounds__(256) k_split_v(const double *__restrict__ V, i
     double v = t[kk][xx];
     for (int l = 0; l < L; ++l) {
       const double li = rint(v);
-      vT[(int64_t)l * N2 * KLp + (int64_t)(x0 + xx) * KLp + k0 + kk] = (_Float16)(float)li;
+      vT[(int64_t)l * N2 * KLp + (int64_t)(x0 + xx) * KLp + k0 + kk] = int_to_half(li);
       v = (v - li) * radix;
     }
   }
@@ -160,7 +170,7 @@ __global__ void k_split_w(const double *__restrict__ W, int64_t len, int KR, int
   const double radix = ldexp(1.0, bits);
   for (int l = 0; l < L; ++l) {
     const double li = rint(v);
-    wl[(int64_t)l * len + i] = (_Float16)(float)li;
+    wl[(int64_t)l * len + i] = int_to_half(li);
     v = (v - li) * radix;
   }
 }
@@ -431,53 +441,74 @@ __global__ void __launch_bounds__(256) k_split_g2(int N2, int KLp, int KRp, cons
   double hv[2][16], pv[2][16];
 
   gload(0);
-  auto mult = [&](int s) {
-    if (s < nst1) {
+  auto mult1 = [&]() {
 #pragma unroll
-      for (int ks = 0; ks < BK / 16; ++ks) {
-        const half8 a = opnd(wave * 32 + lr, ks);
+    for (int ks = 0; ks < BK / 16; ++ks) {
+      const half8 a = opnd(wave * 32 + lr, ks);
 #pragma unroll
-        for (int l = 0; l < L; ++l)
+      for (int l = 0; l < L; ++l)
 #pragma unroll
-          for (int j = 0; j < 2; ++j)
-            acc[l][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, opnd(128 + l * 64 + j * 32 + lr, ks), acc[l][j], 0, 0,
-                                                               0);
-      }
-    } else {
+        for (int j = 0; j < 2; ++j)
+          acc[l][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, opnd(128 + l * 64 + j * 32 + lr, ks), acc[l][j], 0, 0, 0);
+    }
+  };
+  auto mult2 = [&]() {
 #pragma unroll
-      for (int ks = 0; ks < BK / 16; ++ks) {
-        half8 bb[2];
+    for (int ks = 0; ks < BK / 16; ++ks) {
+      half8 bb[2];
 #pragma unroll
-        for (int j = 0; j < 2; ++j) bb[j] = opnd(128 * L + j * 32 + lr, ks);
+      for (int j = 0; j < 2; ++j) bb[j] = opnd(128 * L + j * 32 + lr, ks);
 #pragma unroll
-        for (int l = 0; l < L; ++l) {
-          const half8 a = opnd(l * 128 + wave * 32 + lr, ks);
+      for (int l = 0; l < L; ++l) {
+        const half8 a = opnd(l * 128 + wave * 32 + lr, ks);
 #pragma unroll
-          for (int j = 0; j < 2; ++j) acc[l][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, bb[j], acc[l][j], 0, 0, 0);
-        }
+        for (int j = 0; j < 2; ++j) acc[l][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, bb[j], acc[l][j], 0, 0, 0);
       }
     }
   };
-  for (int s = 0; s + 1 < S; ++s) {
+  // the H and P loads of the epilogue overlap the last stage's multiplications
+  auto epi_loads = [&]() {
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int64_t off = (int64_t)((r >> 2) * 8 + (r & 3)) * ldx + j * 32;
+        hv[j][r] = __builtin_nontemporal_load(Hu + off + lo);
+        if constexpr (EPI == EPI_BA) pv[j][r] = __builtin_nontemporal_load(Pu + off + lo);
+      }
+  };
+  auto stage_in = [&](int s) {
     if (s > 0) __syncthreads();  // the previous stage has been read
     sstore(s);
     __syncthreads();
-    gload(s + 1);
-    mult(s);
-  }
-  if (S > 1) __syncthreads();
-  sstore(S - 1);
-  __syncthreads();
-  // the H and P loads of the epilogue overlap the last stage's multiplications
-#pragma unroll
-  for (int j = 0; j < 2; ++j)
-#pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      const int64_t off = (int64_t)((r >> 2) * 8 + (r & 3)) * ldx + j * 32;
-      hv[j][r] = __builtin_nontemporal_load(Hu + off + lo);
-      if constexpr (EPI == EPI_BA) pv[j][r] = __builtin_nontemporal_load(Pu + off + lo);
+  };
+  // one loop per phase (a phase branch inside one loop makes the compiler copy
+  // every accumulator where the two paths merge); the last stage is peeled so
+  // the epilogue loads can join it
+  if (nst2 > 0) {
+    for (int s = 0; s < nst1; ++s) {
+      stage_in(s);
+      gload(s + 1);
+      mult1();
     }
-  mult(S - 1);
+    for (int s = nst1; s + 1 < S; ++s) {
+      stage_in(s);
+      gload(s + 1);
+      mult2();
+    }
+    stage_in(S - 1);
+    epi_loads();
+    mult2();
+  } else {
+    for (int s = 0; s + 1 < S; ++s) {
+      stage_in(s);
+      gload(s + 1);
+      mult1();
+    }
+    stage_in(S - 1);
+    epi_loads();
+    mult1();
+  }
   __syncthreads();  // etab visible; the stage buffer is free from here on
 
   // ---- epilogue, one element at a time: S from the limb sums (plus the
@@ -552,10 +583,13 @@ __global__ void __launch_bounds__(256) k_split_g2(int N2, int KLp, int KRp, cons
       for (int l = 0; l < LP; ++l) {
         half8 h;
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-          const double li = rint(v[q]);
-          h[q] = (_Float16)(float)li;
-          v[q] = (v[q] - li) * radix;
+        for (int q = 0; q < 8; q += 2) {
+          const double l0 = rint(v[q]), l1 = rint(v[q + 1]);
+          const half2_ p2 = int_to_half2(l0, l1);
+          h[q] = p2[0];
+          h[q + 1] = p2[1];
+          v[q] = (v[q] - l0) * radix;
+          v[q + 1] = (v[q + 1] - l1) * radix;
         }
         *reinterpret_cast<half8 *>(dst + l * lstride) = h;
       }

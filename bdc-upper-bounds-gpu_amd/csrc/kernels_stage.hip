@@ -108,4 +108,108 @@ int launch_w_from_v(const MirrorW &mw, const double *V, int N2, int n2, double *
   return hipGetLastError() == hipSuccess ? BDC_OK : BDC_ERR_HIP;
 }
 
+// ---------------------------------------------------------------------------
+// Stage contractions with a many-deletion count factor: a skinny FP64 product
+// against a half-length count matrix (u16), which is the large operand
+// (cache_{14,11} is 64 MiB).  The counts are streamed once with 16-byte loads,
+// widened to FP64 in LDS and multiplied on the FP64 matrix cores
+// (v_mfma_f64_16x16x4); a block holds every row of the small operand (M <= 128,
+// padded to MP) against 64 columns, over one K range of the split.
+//   C[z][m][n] = sum_{k in range z} A[m][k] B(k, n)
+//   TRANSB = 0: B(k, n) = B[k * ldb + n]   (output-law stage: rows x2, columns z)
+//   TRANSB = 1: B(k, n) = B[n * ldb + k]   (cross-operand stage: B = cache rows x2 = n)
+typedef double double4_ __attribute__((ext_vector_type(4)));
+template <int TRANSB, int MP>
+__global__ void __launch_bounds__(256) k_count_gemm(int M, int K, const double *__restrict__ A, int64_t lda,
+                                                    const uint16_t *__restrict__ B, int64_t ldb, int kc,
+                                                    double *__restrict__ C, int64_t ldc, int64_t zstride) {
+  constexpr int BN = 64, BK = 32, MT = MP / 16;
+  constexpr int ACH = MP * BK / 2 / 256;  // 16-byte pieces (2 doubles) of the A tile per thread
+  static_assert(ACH >= 1, "MP >= 16");
+  __shared__ double As[BK][MP + 1];
+  __shared__ double Bs[BK][BN + 1];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int n0 = blockIdx.x * BN, kz = blockIdx.y;
+  const int kb = kz * kc, ke = min(K, kb + kc);
+  double4_ acc[MT];
+#pragma unroll
+  for (int i = 0; i < MT; ++i) acc[i] = (double4_){0.0, 0.0, 0.0, 0.0};
+  double2 ra[ACH];
+  uint4 rb;
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int c = 0; c < ACH; ++c) {
+      const int e = tid + 256 * c, m = e / (BK / 2), kp = e % (BK / 2);
+      ra[c] = m < M ? *reinterpret_cast<const double2 *>(A + (int64_t)m * lda + k0 + 2 * kp) : make_double2(0.0, 0.0);
+    }
+    if (TRANSB == 0)
+      rb = *reinterpret_cast<const uint4 *>(B + (int64_t)(k0 + (tid >> 3)) * ldb + n0 + (tid & 7) * 8);
+    else
+      rb = *reinterpret_cast<const uint4 *>(B + (int64_t)(n0 + (tid >> 2)) * ldb + k0 + (tid & 3) * 8);
+  };
+  auto sstore = [&]() {
+#pragma unroll
+    for (int c = 0; c < ACH; ++c) {
+      const int e = tid + 256 * c, m = e / (BK / 2), kp = e % (BK / 2);
+      As[2 * kp][m] = ra[c].x;
+      As[2 * kp + 1][m] = ra[c].y;
+    }
+    const uint32_t w[4] = {rb.x, rb.y, rb.z, rb.w};
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const double v = (double)((w[q >> 1] >> (16 * (q & 1))) & 0xffffu);
+      if (TRANSB == 0)
+        Bs[tid >> 3][(tid & 7) * 8 + q] = v;
+      else
+        Bs[(tid & 3) * 8 + q][tid >> 2] = v;
+    }
+  };
+  if (kb < ke) gload(kb);
+  for (int k0 = kb; k0 < ke; k0 += BK) {
+    __syncthreads();
+    sstore();
+    __syncthreads();
+    if (k0 + BK < ke) gload(k0 + BK);
+#pragma unroll
+    for (int k4 = 0; k4 < BK / 4; ++k4) {
+      const int kr = k4 * 4 + (lane >> 4);
+      const double b = Bs[kr][wave * 16 + (lane & 15)];
+#pragma unroll
+      for (int i = 0; i < MT; ++i)
+        acc[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(As[kr][i * 16 + (lane & 15)], b, acc[i], 0, 0, 0);
+    }
+  }
+  double *Cz = C + (int64_t)kz * zstride;
+#pragma unroll
+  for (int i = 0; i < MT; ++i)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int row = i * 16 + (lane >> 4) + 4 * r;
+      if (row < M) Cz[(int64_t)row * ldc + n0 + wave * 16 + (lane & 15)] = acc[i][r];
+    }
+}
+
+int launch_count_gemm(int M, int N, int K, const double *A, int64_t lda, const uint16_t *B, int64_t ldb, int transb,
+                      double *C, int64_t ldc, int splits, int64_t zstride, hipStream_t s) {
+  if (M < 1 || M > 128 || N % 64 || K % 32 || lda % 2 || ldb % 8 || splits < 1 || (K / 32) % splits)
+    return BDC_ERR_ARG;
+  const int kc = K / splits;
+  dim3 grid(N / 64, splits);
+#define CG(TB, MPP) \
+  hipLaunchKernelGGL((k_count_gemm<TB, MPP>), grid, dim3(256), 0, s, M, K, A, lda, B, ldb, kc, C, ldc, zstride)
+#define CG_M(TB)                     \
+  if (M <= 16) CG(TB, 16);           \
+  else if (M <= 32) CG(TB, 32);      \
+  else if (M <= 64) CG(TB, 64);      \
+  else CG(TB, 128);
+  if (transb) {
+    CG_M(1)
+  } else {
+    CG_M(0)
+  }
+#undef CG_M
+#undef CG
+  return hipGetLastError() == hipSuccess ? BDC_OK : BDC_ERR_HIP;
+}
+
 }  // namespace bdc

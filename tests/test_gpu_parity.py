@@ -77,7 +77,8 @@ def test_entropy_term(bdc, n, k):
         np.testing.assert_allclose(e.entropy_term(), ref, rtol=0, atol=1e-12)
 
 
-@pytest.mark.parametrize("n,k", [(8, 4), (10, 5), (11, 6), (12, 7), (12, 3)])
+# (18, 9) reaches the count-streaming stage kernel (2^6-column count factor)
+@pytest.mark.parametrize("n,k", [(8, 4), (10, 5), (11, 6), (12, 7), (12, 3), (18, 9)])
 def test_output_law_general_weights(bdc, oracle, n, k):
     say(f"output law n={n} k={k}")
     rng = np.random.default_rng(n * 100 + k)
@@ -87,7 +88,7 @@ def test_output_law_general_weights(bdc, oracle, n, k):
     np.testing.assert_allclose(q, oracle.full_output_law(n, k, w), rtol=1e-13)
 
 
-@pytest.mark.parametrize("n,k", [(10, 5), (11, 5), (12, 8)])
+@pytest.mark.parametrize("n,k", [(10, 5), (11, 5), (12, 8), (18, 10)])
 def test_kl_general_q(bdc, oracle, n, k):
     say(f"kl n={n} k={k}")
     rng = np.random.default_rng(7 + n)
