@@ -1047,8 +1047,8 @@ int split_prepare(bdc_ctx *c) {
     std::vector<int32_t> t;
     if (rows % 128 == 0)
       for (size_t ty = 0; ty < rows / 128; ++ty)
-        for (size_t tx = 0; tx < N2 / 64; ++tx)
-          if (fA[ty] || fB[tx]) t.push_back((int32_t)(ty * (N2 / 64) + tx));
+        for (size_t tx = 0; tx < N2 / 32; ++tx)
+          if (fA[ty] || fB[tx / 2]) t.push_back((int32_t)(ty * (N2 / 32) + tx));
     c->nrestiles[h] = (int)t.size();
     if (t.empty()) t.push_back(0);
     BDC_TRY(c->restiles[h].alloc(t.size() * 4));

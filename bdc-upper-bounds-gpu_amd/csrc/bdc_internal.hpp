@@ -145,7 +145,7 @@ struct bdc_ctx {
   bdc::DevBuf residA;  // (x1, kappa, r) triples of Abar counts fp16 cannot hold
   bdc::DevBuf rowptrA, entA, rowptrB, entB;  // the same residuals as CSR over x1 (Abar) and x2 (Bbar)
   bdc::DevBuf blkA, blkB;  // flags per 128 rows of Abar / 64 rows of Bbar: any residual there
-  bdc::DevBuf restiles[2];  // G2 tiles (128 x 64) touched by a residual: over N1 rows, over N1/2 rows
+  bdc::DevBuf restiles[2];  // G2 tiles (128 x 32) touched by a residual: over N1 rows, over N1/2 rows
   int nrestiles[2] = {0, 0};
   bdc::DevBuf g1part;      // split-K partial slabs of Tbar
   bool plimb_valid = false;  // plimb/ptexp hold the limbs of the current P (rows x1 < plimb_rows)

@@ -128,7 +128,7 @@ struct SplitG2Args {
   int32_t *texp_next;
   int limbs_p, bits_p;
   const double *exptab;
-  const int32_t *res_tiles;  // tiles (ty * N2/64 + tx) holding a residual, run by a second launch
+  const int32_t *res_tiles;  // 128 x 32 tiles (ty * N2/32 + tx) holding a residual, run by a second launch
   int nres_tiles;
 };
 int launch_split_g2(const SplitG2Args &a, const EpiArgs &ea, hipStream_t s, int *nb);

@@ -328,41 +328,43 @@ struct G2Out {
 };
 
 // S[x1][x2] = sum_kappa A16[x1][kappa] V[kappa][x2] + sum_kappa' W[x1][kappa'] B16[x2][kappa']
-// with V, W in L limbs of one global scale.  Block 128 x 64, 4 waves of
-// 32 x 64 (two 32 x 32 MFMA tiles per limb).
+// with V, W in L limbs of one global scale.  Block 128 x 32 (one P exponent
+// tile), 4 waves of 32 x 32: 16 L accumulators per lane leave room for two
+// blocks per CU, so one block's epilogue (FP64 VALU and HBM traffic) overlaps
+// the other's contraction.
 // Every operand is K-contiguous, so a K stage of 64 is one 128-byte line per
 // operand row: each line is fetched by 8 lanes with 16-byte loads, one L2
 // request per line.  A stage is staged through registers into one LDS buffer
 // whose rows are padded to 144 bytes (conflict-free for the 16-byte writes and
 // for the MFMA operand reads); the next stage's loads are in flight while the
 // current one is multiplied.
-//   phase 1 rows: A16 (128), then V limbs (L x 64)
-//   phase 2 rows: W limbs (L x 128), then B16 (64)
+//   phase 1 rows: A16 (128), then V limbs (L x 32)
+//   phase 2 rows: W limbs (L x 128), then B16 (32)
 // The H and P loads of the epilogue are issued before the last stage's
-// multiplications, which hide part of their HBM latency.  Tiles are dealt XCD by XCD
-// in 8 x 8 patches so the operand rows of a patch stay in that XCD's L2.
+// multiplications, which hide part of their HBM latency.  Tiles are dealt XCD
+// by XCD in 8 x 8 patches so the operand rows of a patch stay in that XCD's L2.
 // Epilogue: D, and for the BA step the update p~ exp(D - dref), the block
-// partials and the next step's P limbs (LP of them) with one exponent per
-// 128 x 32 tile.
+// partials and the next step's P limbs (LP of them) with the tile exponent.
 template <int L>
 struct G2Rows {
-  static constexpr int P1 = 128 + 64 * L;
-  static constexpr int P2 = 128 * L + 64;
+  static constexpr int P1 = 128 + 32 * L;
+  static constexpr int P2 = 128 * L + 32;
   static constexpr int MAXR = P1 > P2 ? P1 : P2;
   static constexpr int LD = 72;  // halves per LDS row (64 + 8 pad)
 };
 
 template <int L, int LP, int EPI, bool RES>
-__global__ void __launch_bounds__(256) k_split_g2(int N2, int KLp, int KRp, const _Float16 *__restrict__ A16,
-                                                 const _Float16 *__restrict__ vT, const _Float16 *__restrict__ wl,
-                                                 const _Float16 *__restrict__ B16, int bits, EpiArgs ea, G2Resid rs,
-                                                 G2Out go, const int32_t *__restrict__ tiles) {
+__global__ void __launch_bounds__(256, 2) k_split_g2(int N2, int KLp, int KRp, const _Float16 *__restrict__ A16,
+                                                    const _Float16 *__restrict__ vT, const _Float16 *__restrict__ wl,
+                                                    const _Float16 *__restrict__ B16, int bits, EpiArgs ea,
+                                                    G2Resid rs, G2Out go, const int32_t *__restrict__ tiles) {
   using G = G2Rows<L>;
-  constexpr int BM = 128, BN = 64, BK = 64, LD = G::LD;
+  constexpr int BM = 128, BN = 32, BK = 64, LD = G::LD;
   constexpr int NR1 = G::P1 / 32, NR2 = G::P2 / 32, NRM = NR1 > NR2 ? NR1 : NR2;  // rows per thread
-  constexpr int SMH = G::MAXR * LD > 64 * 130 * 4 ? G::MAXR * LD : 64 * 130 * 4;  // halves: stages, or the update tile
+  constexpr int WLD = 130;  // doubles per column of the update tile
+  constexpr int SMH = G::MAXR * LD > BN * WLD * 4 ? G::MAXR * LD : BN * WLD * 4;  // halves
   __shared__ __attribute__((aligned(16))) _Float16 sm[SMH];
-  __shared__ double red[48];
+  __shared__ double red[32];
   // the exp table lives in LDS: a global table could alias the P stores, which
   // would order every lookup behind the previous element's store
   __shared__ double etab[32];
@@ -390,30 +392,31 @@ __global__ void __launch_bounds__(256) k_split_g2(int N2, int KLp, int KRp, cons
   const int lr = lane & 31, lk = 8 * (lane >> 5);
 
   half8 rg[NRM];
+  // thread rows are rbase + 32 i: a wave-uniform base per row block (scalar
+  // registers) plus one 32-bit per-lane byte offset per operand pair (phase 1:
+  // i < 4 A16, then V limb i - 4; phase 2: i < 4 L W limb i / 4 row block i % 4,
+  // then B16)
+  const uint32_t off1 = (uint32_t)(rbase * KLp + q8) * 2u, off2 = (uint32_t)(rbase * KRp + q8) * 2u;
+  const char *cA = reinterpret_cast<const char *>(A16 + (int64_t)m0 * KLp);
+  const char *cV = reinterpret_cast<const char *>(vT + (int64_t)n0 * KLp);
+  const char *cW = reinterpret_cast<const char *>(wl + (int64_t)m0 * KRp);
+  const char *cB = reinterpret_cast<const char *>(B16 + (int64_t)n0 * KRp);
+  // (the stage offset goes into the per-lane offset: a loop-invariant base +
+  // lane offset would be hoisted as one 64-bit VGPR pointer per row block)
   auto gload = [&](int s) {
     if (s < nst1) {
-      const int k0 = s * BK + q8;
+      const uint32_t v = off1 + (uint32_t)(s * BK * 2);
 #pragma unroll
       for (int i = 0; i < NR1; ++i) {
-        const int rr = rbase + 32 * i;
-        const _Float16 *src;
-        if (rr < 128)
-          src = A16 + (int64_t)(m0 + rr) * KLp + k0;
-        else
-          src = vT + (int64_t)((rr - 128) >> 6) * N2 * KLp + (int64_t)(n0 + ((rr - 128) & 63)) * KLp + k0;
-        rg[i] = *reinterpret_cast<const half8 *>(src);
+        const char *u = i < 4 ? cA + (int64_t)(32 * i) * KLp * 2 : cV + (int64_t)(i - 4) * N2 * KLp * 2;
+        rg[i] = *reinterpret_cast<const half8 *>(u + v);
       }
     } else {
-      const int k0 = (s - nst1) * BK + q8;
+      const uint32_t v = off2 + (uint32_t)((s - nst1) * BK * 2);
 #pragma unroll
       for (int i = 0; i < NR2; ++i) {
-        const int rr = rbase + 32 * i;
-        const _Float16 *src;
-        if (rr < 128 * L)
-          src = wl + (int64_t)(rr >> 7) * go.M * KRp + (int64_t)(m0 + (rr & 127)) * KRp + k0;
-        else
-          src = B16 + (int64_t)(n0 + rr - 128 * L) * KRp + k0;
-        rg[i] = *reinterpret_cast<const half8 *>(src);
+        const char *u = i < 4 * L ? cW + ((int64_t)(i >> 2) * go.M + 32 * (i & 3)) * KRp * 2 : cB;
+        rg[i] = *reinterpret_cast<const half8 *>(u + v);
       }
     }
   };
@@ -425,20 +428,18 @@ __global__ void __launch_bounds__(256) k_split_g2(int N2, int KLp, int KRp, cons
   };
   auto opnd = [&](int row, int ks) -> half8 { return *reinterpret_cast<const half8 *>(&sm[row * LD + ks * 16 + lk]); };
 
-  float16_ acc[L][2];
+  float16_ acc[L];
 #pragma unroll
-  for (int l = 0; l < L; ++l)
-#pragma unroll
-    for (int j = 0; j < 2; ++j) acc[l][j] = (float16_){0};
+  for (int l = 0; l < L; ++l) acc[l] = (float16_){0};
 
-  // lane element (j, r) of the tile: row m0 + 32 wave + mfma32_row(lane, r), column n0 + 32 j + lane % 32,
+  // lane element r of the tile: row m0 + 32 wave + mfma32_row(lane, r), column n0 + lane % 32,
   // i.e. a wave-uniform base plus the lane offset lo plus a uniform offset per element
   const int64_t ldx = ea.ldx;
   const int64_t ubase = (int64_t)(m0 + wave * 32) * ldx + n0;
   const int lo = (lane >> 5) * 4 * (int)ldx + lr;
   const double *Hu = ea.Hc + ubase;
   double *Pu = EPI == EPI_BA ? ea.P + ubase : ea.D + ubase;
-  double hv[2][16], pv[2][16];
+  double hv[16], pv[16];
 
   gload(0);
   auto mult1 = [&]() {
@@ -447,35 +448,26 @@ __global__ void __launch_bounds__(256) k_split_g2(int N2, int KLp, int KRp, cons
       const half8 a = opnd(wave * 32 + lr, ks);
 #pragma unroll
       for (int l = 0; l < L; ++l)
-#pragma unroll
-        for (int j = 0; j < 2; ++j)
-          acc[l][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, opnd(128 + l * 64 + j * 32 + lr, ks), acc[l][j], 0, 0, 0);
+        acc[l] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, opnd(128 + l * 32 + lr, ks), acc[l], 0, 0, 0);
     }
   };
   auto mult2 = [&]() {
 #pragma unroll
     for (int ks = 0; ks < BK / 16; ++ks) {
-      half8 bb[2];
+      const half8 bb = opnd(128 * L + lr, ks);
 #pragma unroll
-      for (int j = 0; j < 2; ++j) bb[j] = opnd(128 * L + j * 32 + lr, ks);
-#pragma unroll
-      for (int l = 0; l < L; ++l) {
-        const half8 a = opnd(l * 128 + wave * 32 + lr, ks);
-#pragma unroll
-        for (int j = 0; j < 2; ++j) acc[l][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, bb[j], acc[l][j], 0, 0, 0);
-      }
+      for (int l = 0; l < L; ++l)
+        acc[l] = __builtin_amdgcn_mfma_f32_32x32x16_f16(opnd(l * 128 + wave * 32 + lr, ks), bb, acc[l], 0, 0, 0);
     }
   };
   // the H and P loads of the epilogue overlap the last stage's multiplications
   auto epi_loads = [&]() {
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
-#pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const int64_t off = (int64_t)((r >> 2) * 8 + (r & 3)) * ldx + j * 32;
-        hv[j][r] = __builtin_nontemporal_load(Hu + off + lo);
-        if constexpr (EPI == EPI_BA) pv[j][r] = __builtin_nontemporal_load(Pu + off + lo);
-      }
+    for (int r = 0; r < 16; ++r) {
+      const int64_t off = (int64_t)((r >> 2) * 8 + (r & 3)) * ldx;
+      hv[r] = __builtin_nontemporal_load(Hu + off + lo);
+      if constexpr (EPI == EPI_BA) pv[r] = __builtin_nontemporal_load(Pu + off + lo);
+    }
   };
   auto stage_in = [&](int s) {
     if (s > 0) __syncthreads();  // the previous stage has been read
@@ -486,11 +478,13 @@ __global__ void __launch_bounds__(256) k_split_g2(int N2, int KLp, int KRp, cons
   // every accumulator where the two paths merge); the last stage is peeled so
   // the epilogue loads can join it
   if (nst2 > 0) {
+#pragma unroll 1
     for (int s = 0; s < nst1; ++s) {
       stage_in(s);
       gload(s + 1);
       mult1();
     }
+#pragma unroll 1
     for (int s = nst1; s + 1 < S; ++s) {
       stage_in(s);
       gload(s + 1);
@@ -500,6 +494,7 @@ __global__ void __launch_bounds__(256) k_split_g2(int N2, int KLp, int KRp, cons
     epi_loads();
     mult2();
   } else {
+#pragma unroll 1
     for (int s = 0; s + 1 < S; ++s) {
       stage_in(s);
       gload(s + 1);
@@ -513,45 +508,42 @@ __global__ void __launch_bounds__(256) k_split_g2(int N2, int KLp, int KRp, cons
 
   // ---- epilogue, one element at a time: S from the limb sums (plus the
   // residuals of rounded counts), D, and for the BA step the update, whose
-  // result also goes to LDS (column-major, 130-double columns) for the limbs.
+  // result also goes to LDS (column-major) for the limbs.
   const int ev = (int)ea.scal[kScalVScale];
   double sc[L];
 #pragma unroll
   for (int l = 0; l < L; ++l) sc[l] = ldexp(1.0, ev - bits * l);
   const double mu = ea.scal[kScalMu], dref = ea.scal[kScalDRef];
-  constexpr int WLD = 130;
-  double *wt = reinterpret_cast<double *>(sm);  // [64 columns][WLD]
-  double s0 = 0.0, m1 = -INFINITY, s2 = 0.0, wmax[2] = {0.0, 0.0};
+  double *wt = reinterpret_cast<double *>(sm);  // [32 columns][WLD]
+  double s0 = 0.0, m1 = -INFINITY, s2 = 0.0, wmax = 0.0;
 #pragma unroll
-  for (int j = 0; j < 2; ++j)
+  for (int r = 0; r < 16; ++r) {
+    double S = 0.0;
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      double S = 0.0;
-#pragma unroll
-      for (int l = 0; l < L; ++l) S = fma((double)acc[l][j][r], sc[l], S);
-      const int xl1 = wave * 32 + mfma32_row(lane, r), xl2 = j * 32 + lr;
-      if constexpr (RES) {
-        const int x1 = m0 + xl1, x2 = n0 + xl2;
-        for (int e = rs.rowptrA[x1]; e < rs.rowptrA[x1 + 1]; ++e)
-          S += (double)rs.entA[2 * e + 1] * rs.V64[(int64_t)rs.entA[2 * e] * N2 + x2];
-        for (int e = rs.rowptrB[x2]; e < rs.rowptrB[x2 + 1]; ++e)
-          S += rs.W64[(int64_t)x1 * rs.KRp + rs.entB[2 * e]] * (double)rs.entB[2 * e + 1];
-      }
-      const int64_t off = (int64_t)((r >> 2) * 8 + (r & 3)) * ldx + j * 32;
-      const double D = fma(-S, ea.inv_c, hv[j][r] - mu);
-      m1 = fmax(m1, D);
-      if constexpr (EPI == EPI_BA) {
-        const double p = pv[j][r];
-        const double w = p * exp_tang(D - dref, etab);
-        __builtin_nontemporal_store(w, Pu + off + lo);
-        s0 += w;
-        s2 = fma(p, D, s2);
-        wmax[j] = fmax(wmax[j], w);
-        wt[xl2 * WLD + xl1] = w;
-      } else {
-        Pu[off + lo] = D;
-      }
+    for (int l = 0; l < L; ++l) S = fma((double)acc[l][r], sc[l], S);
+    const int xl1 = wave * 32 + mfma32_row(lane, r);
+    if constexpr (RES) {
+      const int x1 = m0 + xl1, x2 = n0 + lr;
+      for (int e = rs.rowptrA[x1]; e < rs.rowptrA[x1 + 1]; ++e)
+        S += (double)rs.entA[2 * e + 1] * rs.V64[(int64_t)rs.entA[2 * e] * N2 + x2];
+      for (int e = rs.rowptrB[x2]; e < rs.rowptrB[x2 + 1]; ++e)
+        S += rs.W64[(int64_t)x1 * rs.KRp + rs.entB[2 * e]] * (double)rs.entB[2 * e + 1];
     }
+    const int64_t off = (int64_t)((r >> 2) * 8 + (r & 3)) * ldx;
+    const double D = fma(-S, ea.inv_c, hv[r] - mu);
+    m1 = fmax(m1, D);
+    if constexpr (EPI == EPI_BA) {
+      const double p = pv[r];
+      const double w = p * exp_tang(D - dref, etab);
+      __builtin_nontemporal_store(w, Pu + off + lo);
+      s0 += w;
+      s2 = fma(p, D, s2);
+      wmax = fmax(wmax, w);
+      wt[lr * WLD + xl1] = w;
+    } else {
+      Pu[off + lo] = D;
+    }
+  }
   block_reduce3(s0, m1, s2, red);
   if (tid == 0) {
     double *pp = ea.part + (int64_t)(ty * TX + tx) * kPartStrideDev;
@@ -559,26 +551,22 @@ __global__ void __launch_bounds__(256) k_split_g2(int N2, int KLp, int KRp, cons
     pp[1] = m1;
     pp[2] = s2;
   }
-  // ---- BA step: limbs of the new P tile for the next output law, one
-  // exponent per 128 x 32 half.  A thread takes 8 consecutive rows of a column
-  // and writes each limb as one 16-byte store (16 threads cover a column's 128
-  // rows, 256 contiguous bytes).
+  // ---- BA step: limbs of the new P tile for the next output law.  A thread
+  // takes 16 consecutive rows of a column and writes each limb as two 16-byte
+  // stores (8 threads cover a column's 128 rows, 256 contiguous bytes).
   if constexpr (EPI == EPI_BA && LP > 0) {
-    const double mw0 = block_max(wmax[0], red + 16);  // leading barrier: wt complete
-    const double mw1 = block_max(wmax[1], red + 32);
-    const int et0 = tile_exponent(mw0, go.bits_p), et1 = tile_exponent(mw1, go.bits_p);
-    if (tid < 2) go.texp[tile_index(go.map, m0, n0 + 32 * tid)] = tid ? et1 : et0;
+    wmax = block_max(wmax, red + 16);  // leading barrier: wt complete
+    const int et = tile_exponent(wmax, go.bits_p);
+    if (tid == 0) go.texp[tile_index(go.map, m0, n0)] = et;
     const double radix = ldexp(1.0, go.bits_p);
     const int64_t lstride = (int64_t)go.map.ncols * go.map.kdim;
-    const int xr = (tid & 15) * 8;
+    const int c = tid >> 3, xr = (tid & 7) * 16;
+    _Float16 *dst = go.limbT + (int64_t)(n0 + c) * go.map.kdim + m0 + xr;
 #pragma unroll
-    for (int it = 0; it < 4; ++it) {
-      const int c = (tid >> 4) + 16 * it;
-      const int et = c < 32 ? et0 : et1;
+    for (int hh = 0; hh < 2; ++hh) {
       double v[8];
 #pragma unroll
-      for (int q = 0; q < 8; ++q) v[q] = ldexp(wt[c * WLD + xr + q], -et);
-      _Float16 *dst = go.limbT + (int64_t)(n0 + c) * go.map.kdim + m0 + xr;
+      for (int q = 0; q < 8; ++q) v[q] = ldexp(wt[c * WLD + xr + 8 * hh + q], -et);
 #pragma unroll
       for (int l = 0; l < LP; ++l) {
         half8 h;
@@ -591,7 +579,7 @@ __global__ void __launch_bounds__(256) k_split_g2(int N2, int KLp, int KRp, cons
           v[q] = (v[q] - l0) * radix;
           v[q + 1] = (v[q + 1] - l1) * radix;
         }
-        *reinterpret_cast<half8 *>(dst + l * lstride) = h;
+        *reinterpret_cast<half8 *>(dst + l * lstride + 8 * hh) = h;
       }
     }
   }
@@ -657,7 +645,7 @@ int launch_resid_g1(const int32_t *res, int nres, const double *X, int rows, int
 int launch_split_g2(const SplitG2Args &a, const EpiArgs &ea, hipStream_t s, int *nb) {
   if (a.M % 128 || a.N2 % 64 || a.KLp % 64 || a.KRp % 64) return BDC_ERR_ARG;
   G2Resid rs{a.blkA, a.blkB, a.rowptrA, a.entA, a.rowptrB, a.entB, a.V64, a.W64, a.KRp};
-  const int TX = a.N2 / 64, TY = a.M / 128;
+  const int TX = a.N2 / 32, TY = a.M / 128;
   G2Out go{};
   go.limbT = a.limbT_next;
   go.texp = a.texp_next;
