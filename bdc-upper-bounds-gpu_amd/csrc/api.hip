@@ -1088,10 +1088,10 @@ int split_prepare(bdc_ctx *c) {
   return BDC_OK;
 }
 
-// G1 split-K: one K range per XCD (8) when every range keeps >= 1024 rows
-static int g1_splits(int rows) {
+// G1 split-K: enough blocks (>= 512) for the 256 CUs, each K range >= 1024 rows
+static int g1_splits(int rows, int blocks) {
   int ks = 1;
-  while (ks < 8 && (rows / 128) % (ks * 2) == 0 && rows / (ks * 2) >= 1024) ks *= 2;
+  while (ks < 8 && blocks * ks < 512 && (rows / 128) % (ks * 2) == 0 && rows / (ks * 2) >= 1024) ks *= 2;
   return env_int("BDC_G1_KSPLIT", ks);
 }
 
@@ -1104,7 +1104,7 @@ int split_output_law_g1(bdc_ctx *c, const double *X, int rows) {
                               c->stream));
   c->plimb_valid = isP;
   c->plimb_rows = rows;
-  const int ks = g1_splits(rows);
+  const int ks = g1_splits(rows, (c->N2 / 32) * ((c->KLp + 255) / 256));
   if (ks < 1 || rows % (ks * 128)) return fail(BDC_ERR_ARG, "G1 split-K %d does not divide %d rows in 128s", ks, rows);
   const size_t slab = (size_t)c->KLp * c->N2;
   if (ks > 1 && c->g1part.bytes < ks * slab * 8) BDC_TRY(c->g1part.alloc(ks * slab * 8));

@@ -179,27 +179,55 @@ __global__ void k_split_w(const double *__restrict__ W, int64_t len, int KR, int
 // Tbar[kappa][c] = sum_k aT[kappa][k] X[k][c] with X in L limbs stored
 // transposed (pT[l][c][k]), Kdim = K extent, ncols columns, lda = row stride
 // of aT (>= Kdim: the half-space step reads the leading rows of Abar).
-// Block: 256 (kappa) x 32 (columns); 8 waves of 32 x 32 with all L limbs (64
-// FP32 + 16 FP64 accumulators per lane).  K step 32 through LDS,
-// double-buffered; FP32 sums promoted to FP64 every 128 K rows (one P tile).
-// K is split in ks (8 for large problems) and with ks = 8 the split index is
-// the XCD the block runs on (blocks are dealt round-robin to the 8 XCDs), so
-// each XCD streams one eighth of the count operand, which stays in its L2; the
-// ks partial slabs are reduced afterwards.
+// Block: 256 kappa x 32 columns (all of Tbar's rows for KLp <= 256, so every
+// P limb is read once), 8 waves of 32 x 32 with all L limbs.  Both operands are
+// K-contiguous; a K stage of 64 is one 128-byte line per operand row, fetched
+// by 8 lanes, staged through registers into an LDS buffer of 144-byte rows.
+// The FP32 sums (exact integers) are promoted to FP64 every 128 K rows, the
+// height of a P exponent tile.  ks > 1 splits K; with ks = 8 the split index is
+// the XCD the block runs on (blocks are dealt round-robin to the 8 XCDs).
 template <int L>
 __global__ void __launch_bounds__(512) k_split_g1(int KLp, int Kdim, int lda, int ncols, const _Float16 *__restrict__ aT,
                                                   const _Float16 *__restrict__ pT, const int32_t *__restrict__ texp,
                                                   int bits, int ks, double *__restrict__ Tpart) {
-  constexpr int BM = 256, BN = 32, BK = 32, PADK = BK + 8;
-  __shared__ __attribute__((aligned(16))) _Float16 As[2][BM][PADK];
-  __shared__ __attribute__((aligned(16))) _Float16 Bs[2][L][BN][PADK];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  constexpr int BM = 256, BN = 32, BK = 64, LD = 72;
+  constexpr int ROWS = BM + L * BN;     // A rows, then L limb slabs of BN rows
+  constexpr int NR = (ROWS + 63) / 64;  // rows per thread (64 rows per pass of 512 threads)
+  __shared__ __attribute__((aligned(16))) _Float16 sm[NR * 64 * LD];
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int kz = blockIdx.x % ks;
   const int tile = blockIdx.x / ks;
   const int ntn = ncols / BN;
   const int n0 = (tile % ntn) * BN, m0 = (tile / ntn) * BM;
   const int tcols = ncols / 32;
-  const int krows = Kdim / ks;
+  const int krows = Kdim / ks, kbase = kz * krows;
+  const int q8 = (tid & 7) * 8, rbase = tid >> 3, lr = lane & 31, lk = 8 * (lane >> 5);
+
+  // thread rows rbase + 64 i: i < 4 rows of aT (kappa = m0 + rbase + 64 i), then
+  // limb 2 (i - 4) + wave / 4 (rbase / 32 is wave-uniform), column n0 + rbase % 32.
+  // Wave-uniform 64-bit bases plus 32-bit per-lane byte offsets that carry the
+  // stage offset (a loop-invariant sum would be hoisted as VGPR pointers).
+  const char *cA = reinterpret_cast<const char *>(aT + (int64_t)m0 * lda + kbase);
+  const char *cP = reinterpret_cast<const char *>(pT + (int64_t)(wave >> 2) * ncols * Kdim + (int64_t)n0 * Kdim + kbase);
+  const uint32_t offA = (uint32_t)(rbase * lda + q8) * 2u;
+  const uint32_t offP = (uint32_t)((rbase & 31) * Kdim + q8) * 2u;
+  const int64_t limb2 = (int64_t)2 * ncols * Kdim * 2;  // bytes between limbs l and l + 2
+  half8 rg[NR];
+  auto gload = [&](int s) {
+    const uint32_t k2 = (uint32_t)(s * BK * 2);
+#pragma unroll
+    for (int i = 0; i < NR; ++i) {
+      if (i < 4) {
+        const bool ok = m0 + rbase + 64 * i < KLp;
+        rg[i] = ok ? *reinterpret_cast<const half8 *>(cA + (int64_t)(64 * i) * lda * 2 + (offA + k2)) : (half8){0};
+      } else if (2 * (i - 4) + (wave >> 2) < L) {
+        rg[i] = *reinterpret_cast<const half8 *>(cP + (int64_t)(i - 4) * limb2 + (offP + k2));
+      } else {
+        rg[i] = (half8){0};
+      }
+    }
+  };
+  auto opnd = [&](int row, int kk) -> half8 { return *reinterpret_cast<const half8 *>(&sm[row * LD + kk * 16 + lk]); };
 
   float16_ acc[L];
   double acc64[16];
@@ -207,63 +235,22 @@ __global__ void __launch_bounds__(512) k_split_g1(int KLp, int Kdim, int lda, in
   for (int r = 0; r < 16; ++r) acc64[r] = 0.0;
 #pragma unroll
   for (int l = 0; l < L; ++l) acc[l] = (float16_){0};
-
-  constexpr int ACH = BM * BK / 8 / 512;   // 2 chunks of 16 B per thread
-  constexpr int BTOT = L * BN * BK / 8;    // L * 128 chunks
-  constexpr int BCH = (BTOT + 511) / 512;  // 1
-  uint4 ra[ACH], rb[BCH];
-  auto gload = [&](int k0) {
+  const int nst = krows / BK;
+  gload(0);
+#pragma unroll 1
+  for (int s = 0; s < nst; ++s) {
+    if (s > 0) __syncthreads();
 #pragma unroll
-    for (int c = 0; c < ACH; ++c) {
-      const int ch = tid + 512 * c, row = ch >> 2, kq = ch & 3;
-      const int gr = m0 + row;
-      ra[c] = gr < KLp ? *reinterpret_cast<const uint4 *>(aT + (int64_t)gr * lda + k0 + kq * 8) : make_uint4(0, 0, 0, 0);
-    }
-#pragma unroll
-    for (int c = 0; c < BCH; ++c) {
-      const int ch = tid + 512 * c;
-      if (ch < BTOT) {
-        const int l = ch / (BN * BK / 8), rem = ch % (BN * BK / 8), row = rem >> 2, kq = rem & 3;
-        rb[c] = *reinterpret_cast<const uint4 *>(pT + (int64_t)l * Kdim * ncols + (int64_t)(n0 + row) * Kdim + k0 + kq * 8);
-      }
-    }
-  };
-  auto sstore = [&](int buf) {
-#pragma unroll
-    for (int c = 0; c < ACH; ++c) {
-      const int ch = tid + 512 * c, row = ch >> 2, kq = ch & 3;
-      *reinterpret_cast<uint4 *>(&As[buf][row][kq * 8]) = ra[c];
-    }
-#pragma unroll
-    for (int c = 0; c < BCH; ++c) {
-      const int ch = tid + 512 * c;
-      if (ch < BTOT) {
-        const int l = ch / (BN * BK / 8), rem = ch % (BN * BK / 8), row = rem >> 2, kq = rem & 3;
-        *reinterpret_cast<uint4 *>(&Bs[buf][l][row][kq * 8]) = rb[c];
-      }
-    }
-  };
-
-  const int kbase = kz * krows;
-  const int nsteps = krows / BK;
-  gload(kbase);
-  sstore(0);
-  __syncthreads();
-  for (int s = 0; s < nsteps; ++s) {
-    const int buf = s & 1;
-    if (s + 1 < nsteps) gload(kbase + (s + 1) * BK);
+    for (int i = 0; i < NR; ++i) *reinterpret_cast<half8 *>(&sm[(rbase + 64 * i) * LD + q8]) = rg[i];
+    __syncthreads();
+    if (s + 1 < nst) gload(s + 1);
 #pragma unroll
     for (int kk = 0; kk < BK / 16; ++kk) {
-      const int kof = kk * 16 + 8 * (lane >> 5);
-      const half8 a = *reinterpret_cast<const half8 *>(&As[buf][wave * 32 + (lane & 31)][kof]);
+      const half8 a = opnd(wave * 32 + lr, kk);
 #pragma unroll
-      for (int l = 0; l < L; ++l) {
-        const half8 b = *reinterpret_cast<const half8 *>(&Bs[buf][l][lane & 31][kof]);
-        acc[l] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, acc[l], 0, 0, 0);
-      }
+      for (int l = 0; l < L; ++l) acc[l] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, opnd(BM + l * BN + lr, kk), acc[l], 0, 0, 0);
     }
-    // promote after every 128 K rows: exact integers -> FP64
-    if (((s + 1) * BK) % 128 == 0) {
+    if (s & 1) {  // 128 K rows done: promote the exact integers to FP64
       const int et = texp[((kbase + s * BK) / 128) * tcols + n0 / 32];
       double sc[L];
 #pragma unroll
@@ -272,23 +259,18 @@ __global__ void __launch_bounds__(512) k_split_g1(int KLp, int Kdim, int lda, in
       for (int r = 0; r < 16; ++r) {
         double t = 0.0;
 #pragma unroll
-        for (int l = 0; l < L; ++l) t += (double)acc[l][r] * sc[l];
+        for (int l = 0; l < L; ++l) t = fma((double)acc[l][r], sc[l], t);
         acc64[r] += t;
       }
 #pragma unroll
       for (int l = 0; l < L; ++l) acc[l] = (float16_){0};
-    }
-    if (s + 1 < nsteps) {
-      sstore(buf ^ 1);
-      __syncthreads();
     }
   }
   double *T = Tpart + (int64_t)kz * KLp * ncols;
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
     const int row = m0 + wave * 32 + mfma32_row(lane, r);
-    const int col = n0 + (lane & 31);
-    if (row < KLp) T[(int64_t)row * ncols + col] = acc64[r];
+    if (row < KLp) T[(int64_t)row * ncols + n0 + lr] = acc64[r];
   }
 }
 
