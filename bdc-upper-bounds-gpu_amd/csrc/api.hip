@@ -297,7 +297,7 @@ int bdc_create(int n, int k, const char *cache_dir, bdc_ctx **out) {
   c->splitk_elems = (size_t)64 * ny;  // split-K partials for the stage GEMMs: up to 64 splits of 2^k outputs
   BDC_TRY_C(c->splitk.alloc(c->splitk_elems * 8));
   c->part_blocks = std::max<size_t>(4096, ((size_t)c->N1 / 64) * ((size_t)c->N2 / 64));
-  BDC_TRY_C(c->part.alloc(c->part_blocks * kPartStride * 8));
+  BDC_TRY_C(c->part.alloc((c->part_blocks + 256) * kPartStride * 8));  // + first-level finalize sums
   BDC_TRY_C(c->scal.alloc(kScalCount * 8));
   BDC_TRY_C(c->exptab.alloc(32 * 8));
   {
@@ -613,7 +613,7 @@ static int ba_step(bdc_ctx *c, int prec, int idx, int rows) {
   BDC_TRY(cross_term(c, prec, EPI_BA, nullptr, &nb, rows));
   if (prec == BDC_PREC_FP64) c->plimb_valid = false;  // P changed without new limbs
   BDC_LAUNCH(launch_ba_finalize(c->part.as<double>(), nb, (double)c->N1 / rows, c->scal.as<double>(),
-                                c->hist.as<double>(), idx, c->stream));
+                                c->hist.as<double>(), idx, c->part.as<double>() + c->part_blocks * kPartStride, c->stream));
   return BDC_OK;
 }
 

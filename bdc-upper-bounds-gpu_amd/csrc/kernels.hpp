@@ -57,8 +57,9 @@ int launch_reduce_splits(const double *part, int splits, int64_t stride, uint64_
 
 // ---- streaming (kernels_stream.hip) ----
 // finalize the fused BA epilogue: reduce block partials (times mult), update scalars, write history
+// (scratch: 256 * 4 doubles for the first reduction level)
 int launch_ba_finalize(const double *part, int nblocks, double mult, double *scal, double *hist, int hist_idx,
-                       hipStream_t s);
+                       double *scratch, hipStream_t s);
 // P[2^n - 1 - x] = P[x] for x < 2^n / 2
 int launch_reflect_half(double *P, uint64_t nx, hipStream_t s);
 // q(y) = pscale * inv_c * (sum_{j in JL} qpart[jl][y] + sum_{j in JR} ...) ; L = ln q - mu

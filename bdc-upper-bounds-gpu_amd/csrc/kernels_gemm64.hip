@@ -271,16 +271,23 @@ int gemm64(int M, int N, const Opnd &A1, const Opnd &B1, int K1, const Opnd &A2,
   return BDC_ERR_ARG;
 }
 
+// C[i] = sum_z part[z * stride + i]: four lanes per output (each a quarter of
+// the splits, combined in a fixed order), so small outputs still fill the GPU
 __global__ void k_reduce_splits(const double *__restrict__ part, int splits, int64_t stride, uint64_t len,
                                 double *__restrict__ C) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= len) return;
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t i = t >> 2;
+  const int q = (int)(t & 3);
   double s = 0.0;
-  for (int z = 0; z < splits; ++z) s += part[(int64_t)z * stride + i];
-  C[i] = s;
+  if (i < len)
+    for (int z = q; z < splits; z += 4) s += part[(int64_t)z * stride + i];
+  s += __shfl_xor(s, 1);
+  s += __shfl_xor(s, 2);
+  if (i < len && q == 0) C[i] = s;
 }
 int launch_reduce_splits(const double *part, int splits, int64_t stride, uint64_t len, double *C, hipStream_t s) {
-  hipLaunchKernelGGL(k_reduce_splits, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, s, part, splits, stride, len, C);
+  hipLaunchKernelGGL(k_reduce_splits, dim3((unsigned)((4 * len + 255) / 256)), dim3(256), 0, s, part, splits, stride,
+                     len, C);
   return hipGetLastError() == hipSuccess ? BDC_OK : BDC_ERR_HIP;
 }
 

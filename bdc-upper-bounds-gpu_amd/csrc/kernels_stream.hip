@@ -48,9 +48,31 @@ __global__ void __launch_bounds__(256) k_ba_finalize(const double *__restrict__ 
     }
   }
 }
+// First level of the finalize reduction: block g combines the partials of a
+// contiguous chunk (fixed order, so the total is deterministic).
+__global__ void __launch_bounds__(256) k_part_reduce(const double *__restrict__ part, int nblocks, int chunk,
+                                                     double *__restrict__ out) {
+  __shared__ double red[48];
+  const int b0 = blockIdx.x * chunk, b1 = min(nblocks, b0 + chunk);
+  double s0 = 0.0, m1 = -INFINITY, s2 = 0.0;
+  for (int b = b0 + threadIdx.x; b < b1; b += blockDim.x) {
+    s0 += part[(int64_t)b * kPartStrideDev + 0];
+    m1 = fmax(m1, part[(int64_t)b * kPartStrideDev + 1]);
+    s2 += part[(int64_t)b * kPartStrideDev + 2];
+  }
+  block_reduce3(s0, m1, s2, red);
+  if (threadIdx.x == 0) {
+    out[blockIdx.x * kPartStrideDev + 0] = s0;
+    out[blockIdx.x * kPartStrideDev + 1] = m1;
+    out[blockIdx.x * kPartStrideDev + 2] = s2;
+  }
+}
 int launch_ba_finalize(const double *part, int nblocks, double mult, double *scal, double *hist, int hist_idx,
-                       hipStream_t s) {
-  hipLaunchKernelGGL(k_ba_finalize, dim3(1), dim3(256), 0, s, part, nblocks, mult, scal, hist, hist_idx);
+                       double *scratch, hipStream_t s) {
+  const int groups = std::min(256, (nblocks + 255) / 256);
+  const int chunk = (nblocks + groups - 1) / groups;
+  hipLaunchKernelGGL(k_part_reduce, dim3(groups), dim3(256), 0, s, part, nblocks, chunk, scratch);
+  hipLaunchKernelGGL(k_ba_finalize, dim3(1), dim3(256), 0, s, scratch, groups, mult, scal, hist, hist_idx);
   return hipGetLastError() == hipSuccess ? BDC_OK : BDC_ERR_HIP;
 }
 
