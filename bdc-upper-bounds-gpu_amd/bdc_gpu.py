@@ -26,7 +26,7 @@ EXPORTS = [
     "bdc_supported", "bdc_num_orbits", "bdc_get_orbits", "bdc_set_p_uniform", "bdc_set_p_orbits",
     "bdc_get_p_orbits", "bdc_set_p_dirichlet", "bdc_iterate", "bdc_certify", "bdc_output_law", "bdc_kl",
     "bdc_entropy_term", "bdc_dual", "bdc_refine_dual", "bdc_certify_q", "bdc_log_output", "bdc_sync",
-    "bdc_kernel_time", "bdc_reset_kernel_timers", "bdc_work_model",
+    "bdc_kernel_time", "bdc_reset_kernel_timers", "bdc_work_model", "bdc_split_config",
 ]
 
 
@@ -87,6 +87,7 @@ def lib():
         L.bdc_kernel_time.argtypes = [vp, cp, ctypes.POINTER(dbl), ctypes.POINTER(i32)]
         L.bdc_reset_kernel_timers.argtypes = [vp, i32]
         L.bdc_work_model.argtypes = [vp, i32, ctypes.POINTER(dbl), ctypes.POINTER(dbl), ctypes.POINTER(dbl)]
+        L.bdc_split_config.argtypes = [vp] + [ctypes.POINTER(i32)] * 5
         for name in EXPORTS:
             if name != "bdc_last_error":
                 getattr(L, name).restype = i32
@@ -241,3 +242,9 @@ class Engine:
         a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
         _check(lib().bdc_work_model(self._h, precision, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
         return a.value, b.value, c.value
+
+    def split_config(self):
+        """{'g1_limbs', 'g1_bits', 'g2_limbs', 'g2_bits', 'residuals'} of the split16 path."""
+        v = [ctypes.c_int() for _ in range(5)]
+        _check(lib().bdc_split_config(self._h, *[ctypes.byref(x) for x in v]))
+        return dict(zip(("g1_limbs", "g1_bits", "g2_limbs", "g2_bits", "residuals"), (x.value for x in v)))

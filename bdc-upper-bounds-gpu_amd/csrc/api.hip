@@ -939,14 +939,26 @@ int bdc_kernel_time(bdc_ctx *c, const char *name, double *avg_ms, int *count) {
 
 int bdc_work_model(bdc_ctx *c, int prec, double *g1, double *g2, double *bytes) {
   if (!c) return fail(BDC_ERR_ARG, "null context");
+  if (prec == BDC_PREC_SPLIT16) BDC_TRY(split_prepare(c));
   // a BA step covers the rows x1 < ba_rows of P
   const double nx = (double)c->nx * ((double)ba_rows(c, prec) / c->N1);
   // G1 with the mirror rule contracts only Abar (KL columns); G2 has K = KL + KR.
   if (g1) *g1 = 2.0 * c->KL * nx;
   if (g2) *g2 = 2.0 * (c->KL + c->KR) * nx;
-  // streaming: G1 reads P once (FP64: 8 B, split: limbs); G2 epilogue reads P and
-  // H and writes P (24 B/x)
-  if (bytes) *bytes = nx * (24.0 + (prec == BDC_PREC_FP64 ? 8.0 : 2.0 * c->split.g1_limbs));
+  // G2 epilogue: p read and written, H read (24 B/x); split16 also writes the p limbs
+  if (bytes) *bytes = nx * (24.0 + (prec == BDC_PREC_SPLIT16 ? 2.0 * c->split.g1_limbs : 0.0));
+  return BDC_OK;
+}
+
+int bdc_split_config(bdc_ctx *c, int *g1_limbs, int *g1_bits, int *g2_limbs, int *g2_bits, int *residuals) {
+  if (!c) return fail(BDC_ERR_ARG, "null context");
+  BDC_TRY(split_prepare(c));
+  const Split &sp = c->split;
+  if (g1_limbs) *g1_limbs = sp.g1_limbs;
+  if (g1_bits) *g1_bits = sp.g1_bits;
+  if (g2_limbs) *g2_limbs = sp.g2_limbs;
+  if (g2_bits) *g2_bits = sp.g2_bits;
+  if (residuals) *residuals = c->n_resid_a + c->n_resid_b;
   return BDC_OK;
 }
 

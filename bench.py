@@ -21,6 +21,7 @@ sys.path.insert(0, os.path.join(REPO, "bdc-upper-bounds-gpu_amd"))
 
 PEAK_FP16 = 2.5e15   # MI355X dense FP16 MFMA, flop/s
 PEAK_FP64 = 78.6e12  # MI355X FP64 (matrix = vector), flop/s
+PEAK_HBM = 8.0e12    # MI355X HBM3E, bytes/s
 LN2 = math.log(2.0)
 
 
@@ -44,7 +45,32 @@ def run_precision(bdc, e, prec, steps, warmup):
     e.timers(False)
     cert = e.certify()
     return {"secs": t1 - t0, "iter_s": steps / (t1 - t0), "g1_ms": g1_ms, "g2_ms": g2_ms, "stats": stats,
-            "cert": cert}
+            "cert": cert, "g2_limbs": e.split_config().get("g2_limbs", 1) if prec == bdc.PREC_SPLIT16 else 1}
+
+
+def measured_traffic(n, k, key):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by what the
+    counter reads for a 1 GiB device copy), or None when it was not measured."""
+    path = os.path.join(REPO, "profiles", f"traffic_bdc{n}_{k}.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)[key]
+        return d["fetch_kib"] * 1024 * d["fetch_scale"] + d["write_kib"] * 1024 * d["write_scale"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def copy_calibration():
+    """1 GiB device-to-device copy, the plain copy whose FETCH_SIZE and
+    WRITE_SIZE calibrate the traffic counters (run under rocprofv3 --pmc)."""
+    import torch
+    a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    log("copied 1 GiB (2^28 float32) device to device")
 
 
 def cpu_baseline(n, k, seconds):
@@ -84,7 +110,12 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=None, help="FP64 steps (default: same as --steps)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 1 GiB device copy (traffic-counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs BDC_{n,k} on one GPU; --gpus above 1 is not supported "
                                    "(the sharding design is described in DESIGN.md)"}))
@@ -109,10 +140,22 @@ def main():
     head = "split" if "split" in res else "fp64"
     r = res[head]
     prec = bdc.PREC_SPLIT16 if head == "split" else bdc.PREC_FP64
-    g1f, g2f, sbytes = e.work_model(prec)
-    # dominant kernel: the cross-term contraction G2 (largest K)
-    peak = PEAK_FP16 if head == "split" else PEAK_FP64
-    achieved = g2f / (r["g2_ms"] * 1e-3) if r["g2_ms"] > 0 else 0.0
+    g1f, g2f, g2bytes = e.work_model(prec)
+    # dominant kernel: the cross-term contraction G2 with the fused BA update.
+    # split16: HBM-bound (its FP16 matrix-core work is a small fraction of peak,
+    # reported beside it); FP64: bound by the FP64 matrix cores.
+    g2s = r["g2_ms"] * 1e-3
+    if head == "split":
+        limb_flops = g2f * r["g2_limbs"]  # every FP64-equivalent multiply-add runs once per limb
+        roof = {"bound": "hbm", "achieved": g2bytes / g2s / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+                "frac": g2bytes / g2s / PEAK_HBM, "traffic": measured_traffic(n, k, "g2_split"),
+                "kernel": "k_split_g2 (cross term + BA update + next p limbs)",
+                "algorithmic_bytes_per_launch": g2bytes,
+                "mfma_tflops": limb_flops / g2s / 1e12, "mfma_frac": limb_flops / g2s / PEAK_FP16}
+    else:
+        roof = {"bound": "mfma", "achieved": g2f / g2s / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
+                "frac": g2f / g2s / PEAK_FP64, "traffic": measured_traffic(n, k, "g2_fp64"),
+                "kernel": "k_gemm64 cross term + BA update", "algorithmic_flop_per_launch": g2f}
     line = {
         "metric": "BAA iter/s at n=28,k=14: split-FP16 MFMA vs FP64 GPU, bound within 1e-9 bits",
         "value": r["iter_s"],
@@ -128,9 +171,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"BDC_({n},{k}) Blahut-Arimoto, uniform initial p, full 2^{n} input space",
                    "n": n, "k": k, "iterations": args.steps, "precision": head},
-        "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": None,
-                     "kernel": "g2 cross-term contraction", "algorithmic_flop_per_launch": g2f},
+        "roofline": roof,
         "setup_s": setup_s,
     }
     for key, v in res.items():

@@ -132,9 +132,17 @@ int bdc_sync(bdc_ctx *ctx);
  * measured with HIP events on the context stream; names: "g1", "g2" */
 int bdc_kernel_time(bdc_ctx *ctx, const char *name, double *avg_ms, int *count);
 int bdc_reset_kernel_timers(bdc_ctx *ctx, int enable);
-/* algorithmic work of one BA step: flops of the two big contractions, and HBM
- * bytes of the streaming parts, at the given precision */
-int bdc_work_model(bdc_ctx *ctx, int precision, double *g1_flops, double *g2_flops, double *stream_bytes);
+/* algorithmic work of one BA step at the given precision: flops of the two big
+ * contractions (counted as FP64-equivalent multiply-adds x 2, whatever the limb
+ * count), and the HBM bytes of one launch of the cross-term kernel (G2): p read
+ * and written, the entropy term read, and for split16 the next step's p limbs
+ * written */
+int bdc_work_model(bdc_ctx *ctx, int precision, double *g1_flops, double *g2_flops, double *g2_bytes);
+/* split16 configuration (fixed on first split16 use, prepared here if needed):
+ * limbs and bits per limb of p in the output-law contraction (g1) and of the
+ * cross-term operands (g2), and the counts fp16 cannot hold (corrected from
+ * sparse residual lists).  No counterpart in the reference (it has no split path). */
+int bdc_split_config(bdc_ctx *ctx, int *g1_limbs, int *g1_bits, int *g2_limbs, int *g2_bits, int *residuals);
 
 #ifdef __cplusplus
 }
