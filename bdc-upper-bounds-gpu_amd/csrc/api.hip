@@ -317,14 +317,26 @@ int bdc_create(int n, int k, const char *cache_dir, bdc_ctx **out) {
 
   // hoisted entropy term H(x)/C - ln C for every x (computed at representatives, expanded)
   {
-    DevBuf hrep;
+    DevBuf hrep, ltab;
     BDC_TRY_C(hrep.alloc(c->norb * 8));
+    BDC_TRY_C(ltab.alloc(130 * 8));
+    {
+      double t[130];
+      for (int j = 0; j <= 64; ++j) {
+        const long double F = 1.0L + j / 64.0L;
+        t[j] = (double)logl(F);
+        t[65 + j] = (double)(1.0L / F);
+      }
+      if (hipMemcpy(ltab.p, t, sizeof t, hipMemcpyHostToDevice) != hipSuccess)
+        return cleanup(fail(BDC_ERR_HIP, "log table"));
+    }
     BDC_TRY_C(check_launch(launch_entropy_reps(n, k, c->n1, c->n2, c->repmask.as<uint64_t>(), c->wordbase.as<uint32_t>(),
-                                               hrep.as<double>(), c->stream), "entropy"));
+                                               ltab.as<double>(), hrep.as<double>(), c->stream), "entropy"));
     BDC_TRY_C(check_launch(launch_expand_orbits(n, hrep.as<double>(), c->repmask.as<uint64_t>(), c->wordbase.as<uint32_t>(),
                                                 c->inv_c, -c->log_c, c->Hc.as<double>(), c->stream), "expand H"));
     if (hipStreamSynchronize(c->stream) != hipSuccess) return cleanup(fail(BDC_ERR_HIP, "entropy sync"));
     hrep.release();
+    ltab.release();
     c->have_h = true;
   }
   if (hipStreamSynchronize(c->stream) != hipSuccess) return cleanup(fail(BDC_ERR_HIP, "setup sync"));

@@ -2,6 +2,7 @@
 // complement/reversal orbits and ranks, initial distributions, and the hoisted
 // entropy term H(x)/C - ln C.  All run once per context.
 #include "device_common.hpp"
+#include "fast_log.hpp"
 #include "kernels.hpp"
 
 namespace bdc {
@@ -302,60 +303,114 @@ int launch_sum(const double *v, uint64_t len, double *part, int np, double *out,
 // ---------------------------------------------------------------- entropy term
 // H(x) = sum_y c(x,y) ln c(x,y) for every representative x (the per-pair
 // W ln W of backend/BAA.cu:401-411, hoisted out of the iteration).
-// One block per x1: the counts c(x1, z) for every z of length <= n1 are built
-// in LDS by a bit-serial DP over x1 (double-buffered, one barrier per bit);
-// for each x2 with (x1,x2) a representative the same is done for x2, and the
-// block sweeps all y in {0,1}^k with c(x,y) = sum_j c(x1, y_hi(j)) c(x2, y_lo(k-j)).
-// Dense layout: level d occupies [2^d - 1, 2^(d+1) - 1).  The DP runs in place:
-// for each symbol of s, levels are updated from the top down (level d reads the
-// not-yet-updated level d-1), one barrier per level.
-__device__ void subseq_counts_lds(uint32_t s, int len, uint16_t *T) {
-  const int tot = (2 << len) - 1;
-  for (int i = threadIdx.x; i < tot; i += blockDim.x) T[i] = (i == 0);
-  __syncthreads();
-  for (int i = 0; i < len; ++i) {
-    const uint32_t xi = (s >> (len - 1 - i)) & 1u;
-    for (int d = i + 1; d >= 1; --d) {
-      const int base = (1 << d) - 1, pbase = (1 << (d - 1)) - 1;
-      for (int z = threadIdx.x; z < (1 << d); z += blockDim.x)
-        if (((uint32_t)z & 1u) == xi) T[base + z] = (uint16_t)(T[base + z] + T[pbase + (z >> 1)]);
-      __syncthreads();
-    }
+// One block per x1.  The subsequence counts of a half string s (length L) for
+// every z of length d < L (level L is the identity, kept implicit) form its
+// "row set" R(s)[2^d - 1 + z]; every entry is built independently from quarter
+// tables of the two halves of s (lengths a = ceil(L/2), b = floor(L/2)):
+//   c(s, z) = sum_i c(s_hi, first i symbols of z) c(s_lo, last d - i symbols),
+// so no level-by-level pass (and no barrier per level) is needed.  The block
+// keeps R(x1) and, for each representative (x1, x2), builds R(x2) and sweeps all
+// y in {0,1}^k with c(x, y) = sum_j c(x1, y_hi(j)) c(x2, y_lo(k-j)), adding
+// c ln c with a table-driven integer log.
+struct QuarterTabs {
+  int off[8];  // byte offset in LDS of the table for strings of length l (0: absent)
+};
+// occurrences of z (length lz) as a subsequence of s (length ls), both MSB first, ls <= 7
+__device__ inline uint32_t count_small(uint32_t s, int ls, uint32_t z, int lz) {
+  uint32_t dp[8] = {1, 0, 0, 0, 0, 0, 0, 0};
+  for (int i = 0; i < ls; ++i) {
+    const uint32_t si = (s >> (ls - 1 - i)) & 1u;
+    for (int j = (i + 1 < lz ? i + 1 : lz); j >= 1; --j)
+      if (((z >> (lz - j)) & 1u) == si) dp[j] += dp[j - 1];
+  }
+  return dp[lz];
+}
+// table for length l: [s < 2^l][2^d - 1 + z, d <= l] as u8 (counts <= C(7,3) = 35)
+__device__ inline int qtab_bytes(int l) { return (1 << l) * ((2 << l) - 1); }
+__device__ inline uint32_t qget(const uint8_t *T, int l, uint32_t s, int d, uint32_t z) {
+  return T[s * ((2 << l) - 1) + (1 << d) - 1 + z];
+}
+// R(s)[2^d - 1 + z] for d < L
+__device__ void build_rowset(uint32_t s, int L, const uint8_t *lds8, const QuarterTabs &qt, uint16_t *R) {
+  const int a = (L + 1) / 2, b = L / 2;
+  const uint8_t *Ta = lds8 + qt.off[a], *Tb = lds8 + qt.off[b];
+  const uint32_t sh = s >> b, sl = s & ((1u << b) - 1u);
+  const int tot = (1 << L) - 1;
+  for (int e = threadIdx.x; e < tot; e += blockDim.x) {
+    const int d = 31 - __clz(e + 1);
+    const uint32_t z = (uint32_t)(e + 1 - (1 << d));
+    uint32_t c = 0;
+    const int ilo = d - b > 0 ? d - b : 0, ihi = d < a ? d : a;
+    for (int i = ilo; i <= ihi; ++i)
+      c += __umul24(qget(Ta, a, sh, i, z >> (d - i)), qget(Tb, b, sl, d - i, z & ((1u << (d - i)) - 1u)));
+    R[e] = (uint16_t)c;
   }
 }
 
 __global__ void __launch_bounds__(1024) k_entropy_reps(int n, int k, int n1, int n2, const uint64_t *__restrict__ mask,
-                                                      const uint32_t *__restrict__ wordbase, double *__restrict__ Hrep) {
-  extern __shared__ uint16_t lds[];
+                                                      const uint32_t *__restrict__ wordbase, QuarterTabs qt,
+                                                      int qbytes, const double *__restrict__ logtab,
+                                                      double *__restrict__ Hrep) {
+  extern __shared__ __attribute__((aligned(16))) uint8_t lds8[];
   __shared__ double red[16];
+  __shared__ double lt[130];
+  // layout: quarter tables (qbytes, rounded to 16), R(x1), R(x2)
+  uint16_t *A = reinterpret_cast<uint16_t *>(lds8 + ((qbytes + 15) & ~15));
+  uint16_t *B = A + ((1 << n1) - 1);
+  for (int i = threadIdx.x; i < 130; i += blockDim.x) lt[i] = logtab[i];
+  // quarter tables
+  for (int l = 0; l <= 7; ++l) {
+    if (qt.off[l] < 0) continue;
+    uint8_t *T = lds8 + qt.off[l];
+    const int per = (2 << l) - 1, tot = (1 << l) * per;
+    for (int e = threadIdx.x; e < tot; e += blockDim.x) {
+      const int sv = e / per, r = e % per, d = 31 - __clz(r + 1);
+      T[e] = (uint8_t)count_small((uint32_t)sv, l, (uint32_t)(r + 1 - (1 << d)), d);
+    }
+  }
+  __syncthreads();
   const uint32_t x1 = blockIdx.x;
-  uint16_t *A = lds, *B = lds + ((2 << n1) - 1);
-  subseq_counts_lds(x1, n1, A);
+  build_rowset(x1, n1, lds8, qt, A);
   const int jlo = k - n2 > 0 ? k - n2 : 0, jhi = k < n1 ? k : n1;
-  const uint32_t N2 = 1u << n2;
+  const uint32_t N2 = 1u << n2, ny = 1u << k;
+  const double *lnF = lt, *invF = lt + 65;
   for (uint32_t x2 = 0; x2 < N2; ++x2) {
     const uint32_t x = (x1 << n2) | x2;
     if (rep_of(x, n) != x) continue;  // uniform across the block
-    subseq_counts_lds(x2, n2, B);
+    __syncthreads();                  // R(x1) built / previous sweep done with R(x2)
+    build_rowset(x2, n2, lds8, qt, B);
+    __syncthreads();
     double h = 0.0;
-    for (uint32_t y = threadIdx.x; y < (1u << k); y += blockDim.x) {
+    for (uint32_t y = threadIdx.x; y < ny; y += blockDim.x) {
       uint32_t c = 0;
       for (int j = jlo; j <= jhi; ++j) {
         const uint32_t yh = y >> (k - j), yl = y & ((1u << (k - j)) - 1u);
-        c += (uint32_t)A[(1 << j) - 1 + yh] * (uint32_t)B[(1 << (k - j)) - 1 + yl];
+        const uint32_t av = j == n1 ? (yh == x1) : A[(1 << j) - 1 + yh];
+        const uint32_t bv = k - j == n2 ? (yl == x2) : B[(1 << (k - j)) - 1 + yl];
+        c += __umul24(av, bv);  // counts < 2^16: 24-bit products are exact (and full rate)
       }
-      if (c > 1u) h += (double)c * log((double)c);
+      if (c > 1u) h = fma((double)c, log_tang_int(c, lnF, invF), h);
     }
     h = block_sum(h, red);
     if (threadIdx.x == 0) Hrep[rank_of_rep(x, mask, wordbase)] = h;
-    __syncthreads();
   }
 }
-int launch_entropy_reps(int n, int k, int n1, int n2, const uint64_t *mask, const uint32_t *wordbase, double *Hrep,
-                        hipStream_t s) {
-  const size_t lds = sizeof(uint16_t) * (((2u << n1) - 1) + ((2u << n2) - 1));
-  if (lds > 160 * 1024) return BDC_ERR_ARG;
-  hipLaunchKernelGGL(k_entropy_reps, dim3(1u << n1), dim3(1024), lds, s, n, k, n1, n2, mask, wordbase, Hrep);
+int launch_entropy_reps(int n, int k, int n1, int n2, const uint64_t *mask, const uint32_t *wordbase,
+                        const double *logtab, double *Hrep, hipStream_t s) {
+  if (n1 > 14 || n2 > 14) return BDC_ERR_ARG;
+  QuarterTabs qt;
+  for (int l = 0; l < 8; ++l) qt.off[l] = -1;
+  int bytes = 0;
+  for (int L : {n1, n2})
+    for (int l : {(L + 1) / 2, L / 2})
+      if (qt.off[l] < 0) {
+        qt.off[l] = bytes;
+        bytes += (1 << l) * ((2 << l) - 1);
+      }
+  const size_t lds = ((bytes + 15) & ~15) + sizeof(uint16_t) * (((1u << n1) - 1) + ((1u << n2) - 1));
+  if (lds > 150 * 1024) return BDC_ERR_ARG;
+  hipLaunchKernelGGL(k_entropy_reps, dim3(1u << n1), dim3(1024), lds, s, n, k, n1, n2, mask, wordbase, qt, bytes,
+                     logtab, Hrep);
   return hipGetLastError() == hipSuccess ? BDC_OK : BDC_ERR_HIP;
 }
 

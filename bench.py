@@ -16,6 +16,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(REPO, "bdc-upper-bounds-gpu_amd"))
 
@@ -185,6 +187,15 @@ def main():
         line["speedup_split_vs_fp64"] = res["split"]["iter_s"] / res["fp64"]["iter_s"]
         line["ub_cert_gap_bits"] = abs(res["split"]["cert"]["upper_cert"] - res["fp64"]["cert"]["upper_cert"]) / LN2
         line["lb_gap_bits"] = abs(res["split"]["cert"]["lower"] - res["fp64"]["cert"]["lower"]) / LN2
+        # per-iteration error against the FP64 path: LB = I(p_t), UB_ref = I(p_t) + mx_t
+        s16, s64 = res["split"]["stats"], res["fp64"]["stats"]
+        t = min(len(s16), len(s64))
+        lb16, lb64 = s16[:t, 3] / LN2, s64[:t, 3] / LN2
+        line["per_iteration_max_err_bits"] = {
+            "lb": float(np.abs(lb16 - lb64).max()),
+            "ub_ref": float(np.abs((lb16 + s16[:t, 0]) - (lb64 + s64[:t, 0])).max()),
+            "mx": float(np.abs(s16[:t, 0] - s64[:t, 0]).max()),
+        }
     e.close()
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle, reference method, sampled)")
