@@ -243,3 +243,16 @@ def test_half_space_matches_full_space(bdc, monkeypatch, n, k, iters):
         assert abs(ch["upper_cert"] - cf["upper_cert"]) < 1e-12
         assert abs(ch["lower"] - cf["lower"]) < 1e-12
         assert abs(ch["sum_p"] - 1.0) < 1e-12
+
+
+@pytest.mark.parametrize("n,k,beta", [(10, 5, 64.0), (12, 6, 256.0)])
+def test_dual_gradcheck_torch(bdc, n, k, beta):
+    """torch.autograd.gradcheck of the adjoint through the autograd wrapper."""
+    import torch
+    import bdc_torch
+    say(f"dual gradcheck n={n} k={k} beta={beta}")
+    theta = torch.tensor(np.random.default_rng(2).normal(scale=0.5, size=1 << k), dtype=torch.float64,
+                         requires_grad=True)
+    with bdc.Engine(n, k) as e:
+        assert torch.autograd.gradcheck(lambda t: bdc_torch.dual_bound(t, e, beta), (theta,), eps=1e-6, atol=1e-7,
+                                        rtol=1e-5)
