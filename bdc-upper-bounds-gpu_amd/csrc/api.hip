@@ -856,7 +856,8 @@ int bdc_dual(bdc_ctx *c, int prec, const double *theta, double beta, double *u, 
   BDC_TRY(ensure_dual(c));
   const size_t ny = (size_t)1 << c->k;
   BDC_HIP(hipMemcpyAsync(c->theta.p, theta, ny * 8, hipMemcpyHostToDevice, c->stream));
-  double *devU = c->splitk.as<double>();
+  // (a scalar slot of its own: the split-K scratch is reused by the gradient's output law)
+  double *devU = c->scal.as<double>() + kScalDualU;
   BDC_TRY(dual_device(c, prec, beta, devU, grad != nullptr));
   BDC_HIP(hipMemcpyAsync(u, devU, 8, hipMemcpyDeviceToHost, c->stream));
   if (grad) BDC_HIP(hipMemcpyAsync(grad, c->grad.p, ny * 8, hipMemcpyDeviceToHost, c->stream));

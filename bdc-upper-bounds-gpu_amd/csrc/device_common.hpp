@@ -116,6 +116,7 @@ enum : int {
   kScalSumW = 7,     // scratch
   kScalDualScale = 8, // 1 / sum_x exp(beta (D - Dmax)) of the last dual pass
   kScalOne = 9,      // constant 1.0
+  kScalDualU = 10,   // U_beta of the last bdc_dual call
   kScalCount = 16,
 };
 

@@ -1,6 +1,9 @@
 // Setup kernels: subsequence-count caches, stacked count operands,
 // complement/reversal orbits and ranks, initial distributions, and the hoisted
 // entropy term H(x)/C - ln C.  All run once per context.
+#include <cstdio>
+#include <cstdlib>
+
 #include "device_common.hpp"
 #include "fast_log.hpp"
 #include "kernels.hpp"
@@ -349,7 +352,7 @@ __device__ void build_rowset(uint32_t s, int L, const uint8_t *lds8, const Quart
 
 __global__ void __launch_bounds__(1024) k_entropy_reps(int n, int k, int n1, int n2, const uint64_t *__restrict__ mask,
                                                       const uint32_t *__restrict__ wordbase, QuarterTabs qt,
-                                                      int qbytes, const double *__restrict__ logtab,
+                                                      int qbytes, const double *__restrict__ logtab, uint32_t x1base,
                                                       double *__restrict__ Hrep) {
   extern __shared__ __attribute__((aligned(16))) uint8_t lds8[];
   __shared__ double red[16];
@@ -369,7 +372,7 @@ __global__ void __launch_bounds__(1024) k_entropy_reps(int n, int k, int n1, int
     }
   }
   __syncthreads();
-  const uint32_t x1 = blockIdx.x;
+  const uint32_t x1 = x1base + blockIdx.x;
   build_rowset(x1, n1, lds8, qt, A);
   const int jlo = k - n2 > 0 ? k - n2 : 0, jhi = k < n1 ? k : n1;
   const uint32_t N2 = 1u << n2, ny = 1u << k;
@@ -380,15 +383,20 @@ __global__ void __launch_bounds__(1024) k_entropy_reps(int n, int k, int n1, int
     __syncthreads();                  // R(x1) built / previous sweep done with R(x2)
     build_rowset(x2, n2, lds8, qt, B);
     __syncthreads();
+    // splits with an identity factor (j = n1: y_hi must equal x1; k - j = n2:
+    // y_lo must equal x2) are handled outside the main loop
+    const int j0 = (k - jlo == n2) ? jlo + 1 : jlo, j1 = (jhi == n1) ? jhi - 1 : jhi;
     double h = 0.0;
     for (uint32_t y = threadIdx.x; y < ny; y += blockDim.x) {
       uint32_t c = 0;
-      for (int j = jlo; j <= jhi; ++j) {
-        const uint32_t yh = y >> (k - j), yl = y & ((1u << (k - j)) - 1u);
-        const uint32_t av = j == n1 ? (yh == x1) : A[(1 << j) - 1 + yh];
-        const uint32_t bv = k - j == n2 ? (yl == x2) : B[(1 << (k - j)) - 1 + yl];
-        c += __umul24(av, bv);  // counts < 2^16: 24-bit products are exact (and full rate)
+      for (int j = j0; j <= j1; ++j) {
+        const int kj = k - j;
+        // counts < 2^16: 24-bit products are exact (and full rate)
+        c += __umul24(A[(1 << j) - 1 + (y >> kj)], B[(1 << kj) - 1 + (y & ((1u << kj) - 1u))]);
       }
+      if (jhi == n1 && (y >> (k - n1)) == x1 && k - n1 != n2) c += B[(1 << (k - n1)) - 1 + (y & ((1u << (k - n1)) - 1u))];
+      if (k - jlo == n2 && (y & ((1u << n2) - 1u)) == x2 && jlo != n1) c += A[(1 << jlo) - 1 + (y >> n2)];
+      if (jhi == n1 && k - n1 == n2 && y == ((x1 << n2) | x2)) c += 1u;  // k = n: both factors are identities
       if (c > 1u) h = fma((double)c, log_tang_int(c, lnF, invF), h);
     }
     h = block_sum(h, red);
@@ -409,9 +417,20 @@ int launch_entropy_reps(int n, int k, int n1, int n2, const uint64_t *mask, cons
       }
   const size_t lds = ((bytes + 15) & ~15) + sizeof(uint16_t) * (((1u << n1) - 1) + ((1u << n2) - 1));
   if (lds > 150 * 1024) return BDC_ERR_ARG;
-  hipLaunchKernelGGL(k_entropy_reps, dim3(1u << n1), dim3(1024), lds, s, n, k, n1, n2, mask, wordbase, qt, bytes,
-                     logtab, Hrep);
-  return hipGetLastError() == hipSuccess ? BDC_OK : BDC_ERR_HIP;
+  // in chunks of x1 rows, so a long precompute (large k) can report progress
+  const char *v = getenv("BDC_VERBOSE");
+  const bool verbose = v && v[0] == '1';
+  const uint32_t N1 = 1u << n1, chunk = N1 >= 16 ? N1 / 16 : N1;
+  for (uint32_t b = 0; b < N1; b += chunk) {
+    hipLaunchKernelGGL(k_entropy_reps, dim3(chunk), dim3(1024), lds, s, n, k, n1, n2, mask, wordbase, qt, bytes,
+                       logtab, b, Hrep);
+    if (hipGetLastError() != hipSuccess) return BDC_ERR_HIP;
+    if (verbose) {
+      if (hipStreamSynchronize(s) != hipSuccess) return BDC_ERR_HIP;
+      fprintf(stderr, "[bdc] entropy term BDC_(%d,%d): x1 rows %u/%u done\n", n, k, b + chunk, N1);
+    }
+  }
+  return BDC_OK;
 }
 
 }  // namespace bdc

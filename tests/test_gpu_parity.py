@@ -256,3 +256,16 @@ def test_dual_gradcheck_torch(bdc, n, k, beta):
     with bdc.Engine(n, k) as e:
         assert torch.autograd.gradcheck(lambda t: bdc_torch.dual_bound(t, e, beta), (theta,), eps=1e-6, atol=1e-7,
                                         rtol=1e-5)
+
+
+def test_dual_value_with_gradient(bdc):
+    """U_beta is the same with and without the gradient pass at a size where
+    the stage contractions split K (the gradient's output law reuses the
+    split-K scratch)."""
+    say("dual value with/without gradient n=18 k=10")
+    theta = np.random.default_rng(5).normal(scale=0.3, size=1 << 10)
+    with bdc.Engine(18, 10) as e:
+        for prec in (bdc.PREC_FP64, bdc.PREC_SPLIT16):
+            u_g, _ = e.dual(theta, 256.0, prec)
+            u = e.dual(theta, 256.0, prec, want_grad=False)
+            assert abs(u_g - u) < 1e-12 * max(1.0, abs(u)), (prec, u_g, u)
