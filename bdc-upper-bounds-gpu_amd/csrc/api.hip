@@ -356,7 +356,7 @@ int bdc_destroy(bdc_ctx *c) {
   for (bdc::DevBuf *b : {&c->abar16, &c->bbar16, &c->abarh, &c->abarTh, &c->bbarh, &c->residA, &c->rowptrA,
                          &c->entA, &c->rowptrB, &c->entB, &c->blkA, &c->blkB, &c->restiles[0], &c->restiles[1], &c->g1part, &c->repmask, &c->wordbase, &c->orbits, &c->P,
                          &c->Hc, &c->T, &c->U, &c->qpart, &c->qv, &c->g2a, &c->g2b, &c->splitk, &c->part, &c->scal, &c->exptab,
-                         &c->hist, &c->plimb, &c->ptexp, &c->vlimb, &c->wlimb, &c->theta, &c->adam_m,
+                         &c->hist, &c->ptexp, &c->vlimb, &c->wlimb, &c->theta, &c->adam_m,
                          &c->adam_v, &c->grad, &c->dvec})
     b->release();
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1104,7 +1104,6 @@ int split_prepare(bdc_ctx *c) {
   BDC_HIP(hipMemcpyAsync(c->entA.p, entA.data(), entA.size() * 4, hipMemcpyHostToDevice, c->stream));
   BDC_HIP(hipMemcpyAsync(c->rowptrB.p, rowptrB.data(), rowptrB.size() * 4, hipMemcpyHostToDevice, c->stream));
   BDC_HIP(hipMemcpyAsync(c->entB.p, entB.data(), entB.size() * 4, hipMemcpyHostToDevice, c->stream));
-  BDC_TRY(c->plimb.alloc((size_t)sp.g1_limbs * c->nx * 2));
   BDC_TRY(c->ptexp.alloc((N1 / 128) * (N2 / 32) * 4));
   BDC_TRY(c->vlimb.alloc((size_t)sp.g2_limbs * N2 * KLp * 2));
   if (KRp) BDC_TRY(c->wlimb.alloc((size_t)sp.g2_limbs * N1 * KRp * 2));
@@ -1123,10 +1122,10 @@ static int g1_splits(int rows, int blocks) {
 int split_output_law_g1(bdc_ctx *c, const double *X, int rows) {
   const Split &sp = c->split;
   if (rows % 128) return fail(BDC_ERR_ARG, "split output law needs a multiple of 128 rows (%d)", rows);
+  // tile exponents of X: written by the previous BA update when X is the current P
   const bool isP = X == c->P.as<double>();
   if (!(isP && c->plimb_valid && c->plimb_rows == rows))
-    BDC_LAUNCH(launch_split_p(X, rows, c->N2, sp.g1_limbs, sp.g1_bits, c->plimb.as<_Float16>(), c->ptexp.as<int32_t>(),
-                              c->stream));
+    BDC_LAUNCH(launch_tile_exp(X, rows, c->N2, sp.g1_bits, c->ptexp.as<int32_t>(), c->stream));
   c->plimb_valid = isP;
   c->plimb_rows = rows;
   const int ks = g1_splits(rows, (c->N2 / 32) * ((c->KLp + 255) / 256));
@@ -1134,8 +1133,8 @@ int split_output_law_g1(bdc_ctx *c, const double *X, int rows) {
   const size_t slab = (size_t)c->KLp * c->N2;
   if (ks > 1 && c->g1part.bytes < ks * slab * 8) BDC_TRY(c->g1part.alloc(ks * slab * 8));
   double *dstT = ks > 1 ? c->g1part.as<double>() : c->T.as<double>();
-  BDC_LAUNCH(launch_split_g1(c->KLp, rows, c->N1, c->N2, c->abarTh.as<_Float16>(), c->plimb.as<_Float16>(),
-                             c->ptexp.as<int32_t>(), sp.g1_limbs, sp.g1_bits, ks, dstT, c->stream));
+  BDC_LAUNCH(launch_split_g1(c->KLp, rows, c->N1, c->N2, c->abarTh.as<_Float16>(), X, c->ptexp.as<int32_t>(),
+                             sp.g1_limbs, sp.g1_bits, ks, dstT, c->stream));
   if (ks > 1) BDC_LAUNCH(launch_reduce_splits(dstT, ks, (int64_t)slab, (uint64_t)slab, c->T.as<double>(), c->stream));
   BDC_LAUNCH(launch_resid_g1(c->residA.as<int32_t>(), c->n_resid_a, X, rows, c->N2, c->T.as<double>(), c->stream));
   return BDC_OK;
@@ -1177,9 +1176,7 @@ int split_cross_term(bdc_ctx *c, int epi, const EpiArgs &ea, int *nb, int rows) 
   a.entB = c->entB.as<int32_t>();
   a.V64 = c->g2b.as<double>();
   a.W64 = c->g2a.as<double>();
-  a.limbT_next = fuse ? c->plimb.as<_Float16>() : nullptr;
   a.texp_next = fuse ? c->ptexp.as<int32_t>() : nullptr;
-  a.limbs_p = sp.g1_limbs;
   a.bits_p = sp.g1_bits;
   a.exptab = c->exptab.as<double>();
   const int h = rows == c->N1 ? 0 : 1;

@@ -138,7 +138,6 @@ struct bdc_ctx {
 
   // split-FP16 buffers
   bdc::Split split;
-  bdc::DevBuf plimb;   // P limbs transposed, fp16 [L][N2][N1]
   bdc::DevBuf ptexp;   // per-tile exponents of P (int32), one per 128 x 64 tile
   bdc::DevBuf vlimb;   // Vbar limbs transposed, fp16 [L][N2][KLp]
   bdc::DevBuf wlimb;   // Wbar limbs fp16 [L][N1][KRp]
@@ -148,7 +147,7 @@ struct bdc_ctx {
   bdc::DevBuf restiles[2];  // G2 tiles (128 x 32) touched by a residual: over N1 rows, over N1/2 rows
   int nrestiles[2] = {0, 0};
   bdc::DevBuf g1part;      // split-K partial slabs of Tbar
-  bool plimb_valid = false;  // plimb/ptexp hold the limbs of the current P (rows x1 < plimb_rows)
+  bool plimb_valid = false;  // ptexp holds the tile exponents of the current P (rows x1 < plimb_rows)
   int plimb_rows = 0;
   int n_resid_a = 0, n_resid_b = 0;
   double g1_chunk_sum = 0, g2_row_sum = 0;  // exactness bounds (max count mass per FP32 accumulation)

@@ -110,12 +110,13 @@ int launch_w_from_v(const MirrorW &mw, const double *V, int N2, int n2, double *
                     hipStream_t s);
 
 // ---- split-FP16 contractions (kernels_split.hip) ----
-// P-like array (rows x1 < M, N2 columns) -> transposed limbs [l][N2][M] + 128x64 tile exponents
-int launch_split_p(const double *P, int M, int N2, int limbs, int bits, _Float16 *limbT, int32_t *texp, hipStream_t s);
+// exponents of the 128 x 32 tiles of a P-like array (rows x1 < M, N2 columns), row-major by tile
+int launch_tile_exp(const double *P, int M, int N2, int bits, int32_t *texp, hipStream_t s);
 int launch_split_vw(const double *V, int KL, int KLp, int N2, const double *W, int M, int KR, int KRp, int limbs,
                     int bits, _Float16 *vT, _Float16 *wl, double *part, double *scal, hipStream_t s);
-// ks partial slabs (ks * KLp * ncols FP64) of Tbar = aT X over the first Kdim rows (aT row stride lda)
-int launch_split_g1(int KLp, int Kdim, int lda, int ncols, const _Float16 *aT, const _Float16 *pT, const int32_t *texp,
+// ks partial slabs (ks * KLp * ncols FP64) of Tbar = aT X over the first Kdim rows of the FP64 array X
+// (row stride ncols; split into limbs on the fly with the tile exponents texp), aT row stride lda
+int launch_split_g1(int KLp, int Kdim, int lda, int ncols, const _Float16 *aT, const double *X, const int32_t *texp,
                     int limbs, int bits, int ks, double *Tpart, hipStream_t s);
 // residual corrections of G1 from the rows x1 < rows
 int launch_resid_g1(const int32_t *res, int nres, const double *X, int rows, int N2, double *T, hipStream_t s);
@@ -126,9 +127,8 @@ struct SplitG2Args {
   const uint8_t *blkA, *blkB;
   const int32_t *rowptrA, *entA, *rowptrB, *entB;
   const double *V64, *W64;
-  _Float16 *limbT_next;  // fused next-step limbs (BA) or null
-  int32_t *texp_next;
-  int limbs_p, bits_p;
+  int32_t *texp_next;  // BA: exponents of the new P tiles
+  int bits_p;
   const double *exptab;
   const int32_t *res_tiles;  // 128 x 32 tiles (ty * N2/32 + tx) holding a residual, run by a second launch
   int nres_tiles;

@@ -45,65 +45,28 @@ __device__ inline int tile_exponent(double mx, int bits) {
   return ilogb(mx) + 1 - (bits - 1);
 }
 
-// Transposed limb layout of a P-like array: limb l, column x2, row x1 at
-// l * ncols * kdim + x2 * kdim + x1; exponents per 128 x 32 tile, row-major.
-struct LimbMap {
-  int kdim;   // rows x1 held (the K extent of the output-law contraction)
-  int ncols;  // columns x2
-};
-
-// write a [64][136] LDS tile (x2 local, x1 local) of one limb to its place
-__device__ inline void store_limb_tile(const _Float16 (*t)[136], _Float16 *dst, const LimbMap &mp, int m0, int n0,
-                                       int nthreads) {
-  for (int ch = threadIdx.x; ch < 64 * 16; ch += nthreads) {
-    const int r2 = ch >> 4, g = ch & 15;
-    *reinterpret_cast<uint4 *>(dst + (int64_t)(n0 + r2) * mp.kdim + m0 + 8 * g) =
-        *reinterpret_cast<const uint4 *>(&t[r2][8 * g]);
-  }
-}
-__device__ inline int tile_index(const LimbMap &mp, int m0, int n0) { return (m0 / 128) * (mp.ncols / 32) + n0 / 32; }
-
-// ----------------------------------------------------------------- P limbs
-// One 128 x 64 block of a P-like array (rows x1 < kdim, N2 columns) per
-// workgroup: two 128 x 32 exponent tiles.  Thread tid always holds column
-// tid % 64, so lanes 0-31 of a wave cover the first tile and 32-63 the second.
-template <int L>
-__global__ void __launch_bounds__(256) k_split_p(const double *__restrict__ P, int N2, int bits, LimbMap mp,
-                                                 _Float16 *__restrict__ limbT, int32_t *__restrict__ texp) {
-  __shared__ __attribute__((aligned(16))) _Float16 tile[64][136];  // [x2][x1]
+// ----------------------------------------------------------------- P tile exponents
+// Exponent of every 128 x 32 tile of a P-like array (rows x1 < M, N2 columns):
+// the smallest e with max |v| <= 2^(e + bits - 1).  One 128 x 64 block (two
+// tiles) per workgroup; thread tid always holds column tid % 64, so lanes 0-31
+// of a wave cover the first tile and 32-63 the second.
+__global__ void __launch_bounds__(256) k_tile_exp(const double *__restrict__ P, int N2, int bits, int tcols,
+                                                  int32_t *__restrict__ texp) {
   __shared__ double red[8];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, half = lane >> 5;
   const int m0 = blockIdx.y * 128, n0 = blockIdx.x * 64;
-  double v[32];
   double mx = 0.0;
-#pragma unroll
+#pragma unroll 8
   for (int r = 0; r < 32; ++r) {
     const int e = tid + 256 * r, row = e >> 6, col = e & 63;
-    v[r] = P[(int64_t)(m0 + row) * N2 + n0 + col];
-    mx = fmax(mx, fabs(v[r]));
+    mx = fmax(mx, fabs(P[(int64_t)(m0 + row) * N2 + n0 + col]));
   }
 #pragma unroll
   for (int o = 16; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o));  // within each 32-lane half
   if ((lane & 31) == 0) red[wave * 2 + half] = mx;
   __syncthreads();
   mx = fmax(fmax(red[half], red[2 + half]), fmax(red[4 + half], red[6 + half]));
-  const int et = tile_exponent(mx, bits);
-  if (wave == 0 && (lane & 31) == 0) texp[tile_index(mp, m0, n0 + 32 * half)] = et;
-#pragma unroll
-  for (int r = 0; r < 32; ++r) v[r] = ldexp(v[r], -et);
-  const double radix = ldexp(1.0, bits);
-  for (int l = 0; l < L; ++l) {
-#pragma unroll
-    for (int r = 0; r < 32; ++r) {
-      const int e = tid + 256 * r, row = e >> 6, col = e & 63;
-      const double li = rint(v[r]);
-      tile[col][row] = int_to_half(li);
-      v[r] = (v[r] - li) * radix;
-    }
-    __syncthreads();
-    store_limb_tile(tile, limbT + (int64_t)l * mp.ncols * mp.kdim, mp, m0, n0, 256);
-    __syncthreads();
-  }
+  if (wave == 0 && (lane & 31) == 0) texp[(m0 / 128) * tcols + n0 / 32 + half] = tile_exponent(mx, bits);
 }
 
 // ----------------------------------------------------------------- V/W limbs
@@ -176,24 +139,25 @@ __global__ void k_split_w(const double *__restrict__ W, int64_t len, int KR, int
 }
 
 // ----------------------------------------------------------------- G1
-// Tbar[kappa][c] = sum_k aT[kappa][k] X[k][c] with X in L limbs stored
-// transposed (pT[l][c][k]), Kdim = K extent, ncols columns, lda = row stride
-// of aT (>= Kdim: the half-space step reads the leading rows of Abar).
-// Block: 256 kappa x 32 columns (all of Tbar's rows for KLp <= 256, so every
-// P limb is read once), 8 waves of 32 x 32 with all L limbs.  Both operands are
-// K-contiguous; a K stage of 64 is one 128-byte line per operand row, fetched
-// by 8 lanes, staged through registers into an LDS buffer of 144-byte rows.
-// The FP32 sums (exact integers) are promoted to FP64 every 128 K rows, the
-// height of a P exponent tile.  ks > 1 splits K; with ks = 8 the split index is
-// the XCD the block runs on (blocks are dealt round-robin to the 8 XCDs).
+// Tbar[kappa][c] = sum_k aT[kappa][k] X[k][c] over the first Kdim rows of the
+// FP64 array X (row stride ncols), lda = row stride of aT (>= Kdim: the
+// half-space step reads the leading rows of Abar).
+// Block: 256 kappa x 32 columns (all of Tbar's rows for KLp <= 256, so X is
+// read once), 8 waves of 32 x 32 with all L limbs.  A K stage covers 64 rows
+// of X: the counts come as one 128-byte line per row of aT (8 lanes of 16
+// bytes); the 64 x 32 block of X is read as 16-byte pairs, split on the fly
+// into L integer limbs with the exponent of its 128 x 32 tile, and written to
+// LDS transposed (column-major) next to the counts; rows of 144 bytes keep the
+// MFMA operand reads conflict-free.  The FP32 sums (exact integers) are
+// promoted to FP64 every 128 K rows, the height of an exponent tile.  ks > 1
+// splits K (partial slabs, reduced afterwards).
 template <int L>
 __global__ void __launch_bounds__(512) k_split_g1(int KLp, int Kdim, int lda, int ncols, const _Float16 *__restrict__ aT,
-                                                  const _Float16 *__restrict__ pT, const int32_t *__restrict__ texp,
+                                                  const double *__restrict__ X, const int32_t *__restrict__ texp,
                                                   int bits, int ks, double *__restrict__ Tpart) {
   constexpr int BM = 256, BN = 32, BK = 64, LD = 72;
-  constexpr int ROWS = BM + L * BN;     // A rows, then L limb slabs of BN rows
-  constexpr int NR = (ROWS + 63) / 64;  // rows per thread (64 rows per pass of 512 threads)
-  __shared__ __attribute__((aligned(16))) _Float16 sm[NR * 64 * LD];
+  constexpr int ROWS = BM + L * BN;  // A rows, then L limb slabs of BN rows (x2)
+  __shared__ __attribute__((aligned(16))) _Float16 sm[ROWS * LD];
   const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int kz = blockIdx.x % ks;
   const int tile = blockIdx.x / ks;
@@ -203,29 +167,45 @@ __global__ void __launch_bounds__(512) k_split_g1(int KLp, int Kdim, int lda, in
   const int krows = Kdim / ks, kbase = kz * krows;
   const int q8 = (tid & 7) * 8, rbase = tid >> 3, lr = lane & 31, lk = 8 * (lane >> 5);
 
-  // thread rows rbase + 64 i: i < 4 rows of aT (kappa = m0 + rbase + 64 i), then
-  // limb 2 (i - 4) + wave / 4 (rbase / 32 is wave-uniform), column n0 + rbase % 32.
-  // Wave-uniform 64-bit bases plus 32-bit per-lane byte offsets that carry the
-  // stage offset (a loop-invariant sum would be hoisted as VGPR pointers).
+  // counts: thread rows rbase + 64 i (i < 4) of aT.  X: thread takes rows
+  // 2 rp, 2 rp + 1 and columns 2 cp, 2 cp + 1 of the stage (rp = tid / 16,
+  // cp = tid % 16).  Wave-uniform bases plus 32-bit per-lane byte offsets that
+  // carry the stage offset (a loop-invariant sum would be hoisted as VGPR pointers).
+  const int rp = tid >> 4, cp = tid & 15;
   const char *cA = reinterpret_cast<const char *>(aT + (int64_t)m0 * lda + kbase);
-  const char *cP = reinterpret_cast<const char *>(pT + (int64_t)(wave >> 2) * ncols * Kdim + (int64_t)n0 * Kdim + kbase);
+  const char *cX = reinterpret_cast<const char *>(X + (int64_t)kbase * ncols + n0);
   const uint32_t offA = (uint32_t)(rbase * lda + q8) * 2u;
-  const uint32_t offP = (uint32_t)((rbase & 31) * Kdim + q8) * 2u;
-  const int64_t limb2 = (int64_t)2 * ncols * Kdim * 2;  // bytes between limbs l and l + 2
-  half8 rg[NR];
+  const uint32_t offX = (uint32_t)(2 * rp * ncols + 2 * cp) * 8u;
+  half8 ra[4];
+  double2 rx[2];
+  int etile = 0;
   auto gload = [&](int s) {
     const uint32_t k2 = (uint32_t)(s * BK * 2);
 #pragma unroll
-    for (int i = 0; i < NR; ++i) {
-      if (i < 4) {
-        const bool ok = m0 + rbase + 64 * i < KLp;
-        rg[i] = ok ? *reinterpret_cast<const half8 *>(cA + (int64_t)(64 * i) * lda * 2 + (offA + k2)) : (half8){0};
-      } else if (2 * (i - 4) + (wave >> 2) < L) {
-        rg[i] = *reinterpret_cast<const half8 *>(cP + (int64_t)(i - 4) * limb2 + (offP + k2));
-      } else {
-        rg[i] = (half8){0};
-      }
+    for (int i = 0; i < 4; ++i) {
+      const bool ok = m0 + rbase + 64 * i < KLp;
+      ra[i] = ok ? *reinterpret_cast<const half8 *>(cA + (int64_t)(64 * i) * lda * 2 + (offA + k2)) : (half8){0};
     }
+    const char *xs = cX + (int64_t)s * BK * ncols * 8;  // stage base (64-bit, uniform)
+#pragma unroll
+    for (int i = 0; i < 2; ++i) rx[i] = *reinterpret_cast<const double2 *>(xs + (int64_t)i * ncols * 8 + offX);
+    etile = texp[((kbase + s * BK) / 128) * tcols + n0 / 32];
+  };
+  const double radix = ldexp(1.0, bits);
+  auto sstore = [&]() {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) *reinterpret_cast<half8 *>(&sm[(rbase + 64 * i) * LD + q8]) = ra[i];
+    // limbs of rows (2 rp, 2 rp + 1) of column 2 cp + q: one packed half2 per limb
+    double v[2][2] = {{ldexp(rx[0].x, -etile), ldexp(rx[1].x, -etile)}, {ldexp(rx[0].y, -etile), ldexp(rx[1].y, -etile)}};
+#pragma unroll
+    for (int l = 0; l < L; ++l)
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const double l0 = rint(v[q][0]), l1 = rint(v[q][1]);
+        *reinterpret_cast<half2_ *>(&sm[(BM + l * BN + 2 * cp + q) * LD + 2 * rp]) = int_to_half2(l0, l1);
+        v[q][0] = (v[q][0] - l0) * radix;
+        v[q][1] = (v[q][1] - l1) * radix;
+      }
   };
   auto opnd = [&](int row, int kk) -> half8 { return *reinterpret_cast<const half8 *>(&sm[row * LD + kk * 16 + lk]); };
 
@@ -240,8 +220,8 @@ __global__ void __launch_bounds__(512) k_split_g1(int KLp, int Kdim, int lda, in
 #pragma unroll 1
   for (int s = 0; s < nst; ++s) {
     if (s > 0) __syncthreads();
-#pragma unroll
-    for (int i = 0; i < NR; ++i) *reinterpret_cast<half8 *>(&sm[(rbase + 64 * i) * LD + q8]) = rg[i];
+    const int et = etile;
+    sstore();
     __syncthreads();
     if (s + 1 < nst) gload(s + 1);
 #pragma unroll
@@ -251,7 +231,6 @@ __global__ void __launch_bounds__(512) k_split_g1(int KLp, int Kdim, int lda, in
       for (int l = 0; l < L; ++l) acc[l] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, opnd(BM + l * BN + lr, kk), acc[l], 0, 0, 0);
     }
     if (s & 1) {  // 128 K rows done: promote the exact integers to FP64
-      const int et = texp[((kbase + s * BK) / 128) * tcols + n0 / 32];
       double sc[L];
 #pragma unroll
       for (int l = 0; l < L; ++l) sc[l] = ldexp(1.0, et - bits * l);
@@ -300,10 +279,9 @@ struct G2Resid {
   int KRp;
 };
 struct G2Out {
-  _Float16 *limbT;  // next P limbs (BA epilogue)
-  int32_t *texp;    // their tile exponents
+  int32_t *texp;  // exponents of the new P tiles (BA epilogue)
   int bits_p;
-  LimbMap map;
+  int tcols;      // exponent tiles per row block (N2 / 32)
   int patches;  // 1: XCD patch mapping of tiles
   int M;        // rows x1 computed (N1/2 in the half-space BA step, else N1)
   const double *exptab;
@@ -326,7 +304,7 @@ struct G2Out {
 // multiplications, which hide part of their HBM latency.  Tiles are dealt XCD
 // by XCD in 8 x 8 patches so the operand rows of a patch stay in that XCD's L2.
 // Epilogue: D, and for the BA step the update p~ exp(D - dref), the block
-// partials and the next step's P limbs (LP of them) with the tile exponent.
+// partials and the exponent of the new P tile.
 template <int L>
 struct G2Rows {
   static constexpr int P1 = 128 + 32 * L;
@@ -335,7 +313,7 @@ struct G2Rows {
   static constexpr int LD = 72;  // halves per LDS row (64 + 8 pad)
 };
 
-template <int L, int LP, int EPI, bool RES>
+template <int L, int EPI, bool RES>
 __global__ void __launch_bounds__(256, 2) k_split_g2(int N2, int KLp, int KRp, const _Float16 *__restrict__ A16,
                                                     const _Float16 *__restrict__ vT, const _Float16 *__restrict__ wl,
                                                     const _Float16 *__restrict__ B16, int bits, EpiArgs ea,
@@ -343,9 +321,7 @@ __global__ void __launch_bounds__(256, 2) k_split_g2(int N2, int KLp, int KRp, c
   using G = G2Rows<L>;
   constexpr int BM = 128, BN = 32, BK = 64, LD = G::LD;
   constexpr int NR1 = G::P1 / 32, NR2 = G::P2 / 32, NRM = NR1 > NR2 ? NR1 : NR2;  // rows per thread
-  constexpr int WLD = 130;  // doubles per column of the update tile
-  constexpr int SMH = G::MAXR * LD > BN * WLD * 4 ? G::MAXR * LD : BN * WLD * 4;  // halves
-  __shared__ __attribute__((aligned(16))) _Float16 sm[SMH];
+  __shared__ __attribute__((aligned(16))) _Float16 sm[G::MAXR * LD];
   __shared__ double red[32];
   // the exp table lives in LDS: a global table could alias the P stores, which
   // would order every lookup behind the previous element's store
@@ -486,17 +462,15 @@ __global__ void __launch_bounds__(256, 2) k_split_g2(int N2, int KLp, int KRp, c
     epi_loads();
     mult1();
   }
-  __syncthreads();  // etab visible; the stage buffer is free from here on
+  __syncthreads();  // etab visible
 
   // ---- epilogue, one element at a time: S from the limb sums (plus the
-  // residuals of rounded counts), D, and for the BA step the update, whose
-  // result also goes to LDS (column-major) for the limbs.
+  // residuals of rounded counts), D, and for the BA step the update.
   const int ev = (int)ea.scal[kScalVScale];
   double sc[L];
 #pragma unroll
   for (int l = 0; l < L; ++l) sc[l] = ldexp(1.0, ev - bits * l);
   const double mu = ea.scal[kScalMu], dref = ea.scal[kScalDRef];
-  double *wt = reinterpret_cast<double *>(sm);  // [32 columns][WLD]
   double s0 = 0.0, m1 = -INFINITY, s2 = 0.0, wmax = 0.0;
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
@@ -521,7 +495,6 @@ __global__ void __launch_bounds__(256, 2) k_split_g2(int N2, int KLp, int KRp, c
       s0 += w;
       s2 = fma(p, D, s2);
       wmax = fmax(wmax, w);
-      wt[lr * WLD + xl1] = w;
     } else {
       Pu[off + lo] = D;
     }
@@ -533,37 +506,11 @@ __global__ void __launch_bounds__(256, 2) k_split_g2(int N2, int KLp, int KRp, c
     pp[1] = m1;
     pp[2] = s2;
   }
-  // ---- BA step: limbs of the new P tile for the next output law.  A thread
-  // takes 16 consecutive rows of a column and writes each limb as two 16-byte
-  // stores (8 threads cover a column's 128 rows, 256 contiguous bytes).
-  if constexpr (EPI == EPI_BA && LP > 0) {
-    wmax = block_max(wmax, red + 16);  // leading barrier: wt complete
-    const int et = tile_exponent(wmax, go.bits_p);
-    if (tid == 0) go.texp[tile_index(go.map, m0, n0)] = et;
-    const double radix = ldexp(1.0, go.bits_p);
-    const int64_t lstride = (int64_t)go.map.ncols * go.map.kdim;
-    const int c = tid >> 3, xr = (tid & 7) * 16;
-    _Float16 *dst = go.limbT + (int64_t)(n0 + c) * go.map.kdim + m0 + xr;
-#pragma unroll
-    for (int hh = 0; hh < 2; ++hh) {
-      double v[8];
-#pragma unroll
-      for (int q = 0; q < 8; ++q) v[q] = ldexp(wt[c * WLD + xr + 8 * hh + q], -et);
-#pragma unroll
-      for (int l = 0; l < LP; ++l) {
-        half8 h;
-#pragma unroll
-        for (int q = 0; q < 8; q += 2) {
-          const double l0 = rint(v[q]), l1 = rint(v[q + 1]);
-          const half2_ p2 = int_to_half2(l0, l1);
-          h[q] = p2[0];
-          h[q + 1] = p2[1];
-          v[q] = (v[q] - l0) * radix;
-          v[q + 1] = (v[q + 1] - l1) * radix;
-        }
-        *reinterpret_cast<half8 *>(dst + l * lstride + 8 * hh) = h;
-      }
-    }
+  // ---- BA step: exponent of the new P tile for the next output law (whose
+  // contraction splits P into limbs as it reads it)
+  if constexpr (EPI == EPI_BA) {
+    wmax = block_max(wmax, red + 16);
+    if (tid == 0) go.texp[(m0 / 128) * go.tcols + n0 / 32] = tile_exponent(wmax, go.bits_p);
   }
 }
 
@@ -576,13 +523,9 @@ __global__ void __launch_bounds__(256, 2) k_split_g2(int N2, int KLp, int KRp, c
     default: return BDC_ERR_ARG; \
   }
 
-int launch_split_p(const double *P, int M, int N2, int limbs, int bits, _Float16 *limbT, int32_t *texp, hipStream_t s) {
+int launch_tile_exp(const double *P, int M, int N2, int bits, int32_t *texp, hipStream_t s) {
   if (M % 128 || N2 % 64) return BDC_ERR_ARG;
-  LimbMap mp{M, N2};
-  dim3 grid(N2 / 64, M / 128);
-#define LP(LL) hipLaunchKernelGGL(k_split_p<LL>, grid, dim3(256), 0, s, P, N2, bits, mp, limbT, texp)
-  BDC_SPLIT_L(limbs, LP)
-#undef LP
+  hipLaunchKernelGGL(k_tile_exp, dim3(N2 / 64, M / 128), dim3(256), 0, s, P, N2, bits, N2 / 32, texp);
   return hipGetLastError() == hipSuccess ? BDC_OK : BDC_ERR_HIP;
 }
 
@@ -606,12 +549,12 @@ int launch_split_vw(const double *V, int KL, int KLp, int N2, const double *W, i
   return hipGetLastError() == hipSuccess ? BDC_OK : BDC_ERR_HIP;
 }
 
-int launch_split_g1(int KLp, int Kdim, int lda, int ncols, const _Float16 *aT, const _Float16 *pT, const int32_t *texp,
+int launch_split_g1(int KLp, int Kdim, int lda, int ncols, const _Float16 *aT, const double *X, const int32_t *texp,
                     int limbs, int bits, int ks, double *Tpart, hipStream_t s) {
   if (ks < 1 || Kdim % (ks * 128) || ncols % 64 || lda < Kdim || lda % 8) return BDC_ERR_ARG;
   const int blocks = ks * (ncols / 32) * ((KLp + 255) / 256);
-#define LG(LL)                                                                                                   \
-  hipLaunchKernelGGL(k_split_g1<LL>, dim3(blocks), dim3(512), 0, s, KLp, Kdim, lda, ncols, aT, pT, texp, bits, ks, \
+#define LG(LL)                                                                                                  \
+  hipLaunchKernelGGL(k_split_g1<LL>, dim3(blocks), dim3(512), 0, s, KLp, Kdim, lda, ncols, aT, X, texp, bits, ks, \
                      Tpart)
   BDC_SPLIT_L(limbs, LG)
 #undef LG
@@ -629,50 +572,38 @@ int launch_split_g2(const SplitG2Args &a, const EpiArgs &ea, hipStream_t s, int 
   G2Resid rs{a.blkA, a.blkB, a.rowptrA, a.entA, a.rowptrB, a.entB, a.V64, a.W64, a.KRp};
   const int TX = a.N2 / 32, TY = a.M / 128;
   G2Out go{};
-  go.limbT = a.limbT_next;
   go.texp = a.texp_next;
   go.bits_p = a.bits_p;
-  go.map = LimbMap{a.M, a.N2};
+  go.tcols = a.N2 / 32;
   go.patches = (TX % 8 == 0 && TY % 8 == 0 && ((TX / 8) * (TY / 8)) % 8 == 0) ? 1 : 0;
   go.M = a.M;
   go.exptab = a.exptab;
-  const int lp = a.epi == EPI_BA ? a.limbs_p : 0;
-  if (a.epi == EPI_BA && (!a.limbT_next || !a.texp_next)) return BDC_ERR_ARG;
+  if (a.epi == EPI_BA && !a.texp_next) return BDC_ERR_ARG;
   const int blocks = TX * TY;
   if (nb) *nb = blocks;
-#define LG2(LL, LPP, EE)                                                                                           \
-  do {                                                                                                             \
-    hipLaunchKernelGGL((k_split_g2<LL, LPP, EE, false>), dim3(blocks), dim3(256), 0, s, a.N2, a.KLp, a.KRp, a.A16, \
-                       a.vT, a.wl, a.B16, a.bits, ea, rs, go, nullptr);                                            \
-    if (a.nres_tiles)                                                                                              \
-      hipLaunchKernelGGL((k_split_g2<LL, LPP, EE, true>), dim3(a.nres_tiles), dim3(256), 0, s, a.N2, a.KLp, a.KRp, \
-                         a.A16, a.vT, a.wl, a.B16, a.bits, ea, rs, go, a.res_tiles);                                \
+#define LG2(LL, EE)                                                                                               \
+  do {                                                                                                            \
+    hipLaunchKernelGGL((k_split_g2<LL, EE, false>), dim3(blocks), dim3(256), 0, s, a.N2, a.KLp, a.KRp, a.A16, a.vT, \
+                       a.wl, a.B16, a.bits, ea, rs, go, nullptr);                                                 \
+    if (a.nres_tiles)                                                                                             \
+      hipLaunchKernelGGL((k_split_g2<LL, EE, true>), dim3(a.nres_tiles), dim3(256), 0, s, a.N2, a.KLp, a.KRp,     \
+                         a.A16, a.vT, a.wl, a.B16, a.bits, ea, rs, go, a.res_tiles);                               \
   } while (0)
-#define LG2_LP(LL)                            \
-  switch (lp) {                               \
-    case 2: LG2(LL, 2, EPI_BA); break;        \
-    case 3: LG2(LL, 3, EPI_BA); break;        \
-    case 4: LG2(LL, 4, EPI_BA); break;        \
-    default: return BDC_ERR_ARG;              \
+#define LG2_E(EE)                \
+  switch (a.limbs) {             \
+    case 2: LG2(2, EE); break;   \
+    case 3: LG2(3, EE); break;   \
+    case 4: LG2(4, EE); break;   \
+    default: return BDC_ERR_ARG; \
   }
   if (a.epi == EPI_BA) {
-    switch (a.limbs) {
-      case 2: LG2_LP(2); break;
-      case 3: LG2_LP(3); break;
-      case 4: LG2_LP(4); break;
-      default: return BDC_ERR_ARG;
-    }
+    LG2_E(EPI_BA)
   } else if (a.epi == EPI_D) {
-    switch (a.limbs) {
-      case 2: LG2(2, 0, EPI_D); break;
-      case 3: LG2(3, 0, EPI_D); break;
-      case 4: LG2(4, 0, EPI_D); break;
-      default: return BDC_ERR_ARG;
-    }
+    LG2_E(EPI_D)
   } else {
     return BDC_ERR_ARG;
   }
-#undef LG2_LP
+#undef LG2_E
 #undef LG2
   return hipGetLastError() == hipSuccess ? BDC_OK : BDC_ERR_HIP;
 }
