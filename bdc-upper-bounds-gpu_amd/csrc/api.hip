@@ -402,8 +402,10 @@ static int count_stage(bdc_ctx *c, int M, int N, int K, const double *A, int64_t
                        int transb, double *C, int64_t ldc, bool *done) {
   *done = false;
   if (M > 128 || N % 64 || K % 32) return BDC_OK;
+  // split K until ~2048 blocks stream the count matrix (each holds one 4 KiB
+  // stage in flight, so memory parallelism comes from the block count)
   int ks = 1;
-  while ((N / 64) * ks < 512 && ks < 64 && K / (ks * 2) >= 256 && (K / 32) % (ks * 2) == 0 &&
+  while ((N / 64) * ks < 2048 && ks < 64 && K / (ks * 2) >= 128 && (K / 32) % (ks * 2) == 0 &&
          (size_t)(ks * 2) * M * N <= c->splitk_elems)
     ks *= 2;
   if (ks == 1) {

@@ -95,7 +95,7 @@ int launch_del_gather(const double *in, int64_t irs, int64_t ics, double *out, i
                       int d, hipStream_t s);
 int launch_ins_gather(const double *in, int64_t irs, int64_t ics, double *out, int64_t ors, int64_t ocs, int R, int nb,
                       int d, hipStream_t s);
-constexpr int kMaxGatherDeletions = 3;   // deletion form: C(n,d) terms per output
+constexpr int kMaxGatherDeletions = 2;   // deletion form: C(n,d) terms per output (d = 3 streams the counts faster)
 constexpr int kMaxGatherInsertions = 2;  // insertion form: 2^d C(n,d) terms per output
 // C + z * zstride (row stride ldc) = A[M x K-range z] x B over count matrices (u16):
 // transb = 0: B(k, n) = B[k * ldb + n]; transb = 1: B(k, n) = B[n * ldb + k].

@@ -43,46 +43,116 @@ __device__ inline uint32_t next_subset(uint32_t v, int nb) {
   return w < (1u << nb) ? w : 0u;
 }
 
+// The d-subsets of nb positions (C(16,3) = 560 at most here) are listed in LDS
+// once per block.  A block of 8 waves covers 64 consecutive outputs: wave w
+// sums the subsets with index = w mod 8 (lanes = consecutive outputs, so the
+// loads stay coalesced), and the 8 partial sums are combined in a fixed order
+// through LDS.  This gives the latency-bound gathers 8x the waves in flight.
+constexpr int kMaxSubsets = 1024;
+constexpr int kGatherWaves = 8;
+__device__ inline int list_subsets(int nb, int d, uint32_t *masks) {
+  __shared__ int count;
+  if (threadIdx.x == 0) {
+    int c = 0;
+    for (uint32_t m = (1u << d) - 1u; m && c < kMaxSubsets; m = next_subset(m, nb)) masks[c++] = m;
+    count = c;
+  }
+  __syncthreads();
+  return count;
+}
+__device__ inline double combine_waves(double v, double *part) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  part[w * 64 + lane] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (w == 0)
+#pragma unroll
+    for (int q = 0; q < kGatherWaves; ++q) t += part[q * 64 + lane];
+  return t;  // valid in wave 0
+}
+
 // out[r*ors + x*ocs] = sum_{|S| = d} in[r*irs + del(x, S)*ics]   for r < R, x < 2^nb
-__global__ void k_del_gather(const double *__restrict__ in, int64_t irs, int64_t ics, double *__restrict__ out,
-                             int64_t ors, int64_t ocs, int R, int nb, int d) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(512) k_del_gather(const double *__restrict__ in, int64_t irs, int64_t ics,
+                                                    double *__restrict__ out, int64_t ors, int64_t ocs, int R, int nb,
+                                                    int d) {
+  __shared__ uint32_t masks[kMaxSubsets];
+  __shared__ double part[kGatherWaves * 64];
+  const int ns = list_subsets(nb, d, masks);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int64_t i = (int64_t)blockIdx.x * 64 + lane;
   const int64_t nx = 1ll << nb;
-  if (i >= (int64_t)R * nx) return;
+  const bool live = i < (int64_t)R * nx;
   const int r = (int)(i >> nb);
   const uint32_t x = (uint32_t)(i & (nx - 1));
   double s = 0.0;
-  for (uint32_t m = (1u << d) - 1u; m; m = next_subset(m, nb)) s += in[r * irs + (int64_t)del_bits(x, m) * ics];
-  out[r * ors + (int64_t)x * ocs] = s;
+  if (live) {
+    const double *row = in + r * irs;
+    double s2 = 0.0;
+    int t = w;
+    for (; t + kGatherWaves < ns; t += 2 * kGatherWaves) {
+      s += row[(int64_t)del_bits(x, masks[t]) * ics];
+      s2 += row[(int64_t)del_bits(x, masks[t + kGatherWaves]) * ics];
+    }
+    if (t < ns) s += row[(int64_t)del_bits(x, masks[t]) * ics];
+    s += s2;
+  }
+  s = combine_waves(s, part);
+  if (live && w == 0) out[r * ors + (int64_t)x * ocs] = s;
 }
 
 // out[r*ors + z*ocs] = sum_{|S| = d} sum_b in[r*irs + ins(z, S, b)*ics]   for r < R, z < 2^(nb-d)
-__global__ void k_ins_gather(const double *__restrict__ in, int64_t irs, int64_t ics, double *__restrict__ out,
-                             int64_t ors, int64_t ocs, int R, int nb, int d) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(512) k_ins_gather(const double *__restrict__ in, int64_t irs, int64_t ics,
+                                                    double *__restrict__ out, int64_t ors, int64_t ocs, int R, int nb,
+                                                    int d) {
+  __shared__ uint32_t masks[kMaxSubsets];
+  __shared__ double part[kGatherWaves * 64];
+  const int ns = list_subsets(nb, d, masks);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int64_t i = (int64_t)blockIdx.x * 64 + lane;
   const int nz = nb - d;
   const int64_t cnt = 1ll << nz;
-  if (i >= (int64_t)R * cnt) return;
+  const bool live = i < (int64_t)R * cnt;
   const int r = (int)(i >> nz);
   const uint32_t z = (uint32_t)(i & (cnt - 1));
   double s = 0.0;
-  for (uint32_t m = (1u << d) - 1u; m; m = next_subset(m, nb))
-    for (uint32_t b = 0; b < (1u << d); ++b) s += in[r * irs + (int64_t)ins_bits(z, m, b) * ics];
-  out[r * ors + (int64_t)z * ocs] = s;
+  if (live) {
+    const double *row = in + r * irs;
+    const uint32_t nbv = 1u << d;
+    double s2 = 0.0;
+    for (int t = w; t < ns; t += kGatherWaves) {
+      const uint32_t m = masks[t];
+      uint32_t b = 0;
+      for (; b + 2 <= nbv; b += 2) {
+        s += row[(int64_t)ins_bits(z, m, b) * ics];
+        s2 += row[(int64_t)ins_bits(z, m, b + 1) * ics];
+      }
+      if (b < nbv) s += row[(int64_t)ins_bits(z, m, b) * ics];
+    }
+    s += s2;
+  }
+  s = combine_waves(s, part);
+  if (live && w == 0) out[r * ors + (int64_t)z * ocs] = s;
 }
 
+static uint64_t n_subsets(int nb, int d) {
+  uint64_t c = 1;
+  for (int i = 0; i < d; ++i) c = c * (uint64_t)(nb - i) / (uint64_t)(i + 1);
+  return c;
+}
 int launch_del_gather(const double *in, int64_t irs, int64_t ics, double *out, int64_t ors, int64_t ocs, int R, int nb,
                       int d, hipStream_t s) {
+  if (d < 1 || n_subsets(nb, d) > (uint64_t)kMaxSubsets) return BDC_ERR_ARG;
   const int64_t tot = (int64_t)R << nb;
-  hipLaunchKernelGGL(k_del_gather, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, in, irs, ics, out, ors, ocs, R,
-                     nb, d);
+  hipLaunchKernelGGL(k_del_gather, dim3((unsigned)((tot + 63) / 64)), dim3(64 * kGatherWaves), 0, s, in, irs, ics, out,
+                     ors, ocs, R, nb, d);
   return hipGetLastError() == hipSuccess ? BDC_OK : BDC_ERR_HIP;
 }
 int launch_ins_gather(const double *in, int64_t irs, int64_t ics, double *out, int64_t ors, int64_t ocs, int R, int nb,
                       int d, hipStream_t s) {
+  if (d < 1 || n_subsets(nb, d) > (uint64_t)kMaxSubsets) return BDC_ERR_ARG;
   const int64_t tot = (int64_t)R << (nb - d);
-  hipLaunchKernelGGL(k_ins_gather, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, in, irs, ics, out, ors, ocs, R,
-                     nb, d);
+  hipLaunchKernelGGL(k_ins_gather, dim3((unsigned)((tot + 63) / 64)), dim3(64 * kGatherWaves), 0, s, in, irs, ics, out,
+                     ors, ocs, R, nb, d);
   return hipGetLastError() == hipSuccess ? BDC_OK : BDC_ERR_HIP;
 }
 
