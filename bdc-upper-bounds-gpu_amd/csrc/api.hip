@@ -960,8 +960,9 @@ int bdc_work_model(bdc_ctx *c, int prec, double *g1, double *g2, double *bytes) 
   // G1 with the mirror rule contracts only Abar (KL columns); G2 has K = KL + KR.
   if (g1) *g1 = 2.0 * c->KL * nx;
   if (g2) *g2 = 2.0 * (c->KL + c->KR) * nx;
-  // G2 epilogue: p read and written, H read (24 B/x); split16 also writes the p limbs
-  if (bytes) *bytes = nx * (24.0 + (prec == BDC_PREC_SPLIT16 ? 2.0 * c->split.g1_limbs : 0.0));
+  // G2 epilogue: p read and written, H read (24 B/x) at both precisions (the
+  // split16 output law forms p's limbs itself from the FP64 p)
+  if (bytes) *bytes = nx * 24.0;
   return BDC_OK;
 }
 

@@ -16,6 +16,8 @@
 // contraction reads P limbs transposed ([l][x2][x1]) so that its K index x1 is
 // contiguous.  The BA step keeps only the rows x1 < N1/2 of P (the other half
 // is its complement image), so K runs over those rows.
+#include <type_traits>
+
 #include "device_common.hpp"
 #include "fast_exp.hpp"
 #include "kernels.hpp"
@@ -307,14 +309,15 @@ struct G2Out {
 // partials and the exponent of the new P tile.
 template <int L>
 struct G2Rows {
+  static constexpr int NP2 = (L + 1) / 2;  // phase-2 sub-stages per K stage: W limbs in pairs
   static constexpr int P1 = 128 + 32 * L;
-  static constexpr int P2 = 128 * L + 32;
+  static constexpr int P2 = 2 * 128 + 32;
   static constexpr int MAXR = P1 > P2 ? P1 : P2;
   static constexpr int LD = 72;  // halves per LDS row (64 + 8 pad)
 };
 
 template <int L, int EPI, bool RES>
-__global__ void __launch_bounds__(256, 2) k_split_g2(int N2, int KLp, int KRp, const _Float16 *__restrict__ A16,
+__global__ void __launch_bounds__(256, 3) k_split_g2(int N2, int KLp, int KRp, const _Float16 *__restrict__ A16,
                                                     const _Float16 *__restrict__ vT, const _Float16 *__restrict__ wl,
                                                     const _Float16 *__restrict__ B16, int bits, EpiArgs ea,
                                                     G2Resid rs, G2Out go, const int32_t *__restrict__ tiles) {
@@ -345,15 +348,16 @@ __global__ void __launch_bounds__(256, 2) k_split_g2(int N2, int KLp, int KRp, c
   // tiles touched by a residual of the rounded counts run in the second launch
   if (!RES && (rs.blkA[m0 / 128] || rs.blkB[n0 / 64])) return;
   if (EPI == EPI_BA && tid < 32) etab[tid] = go.exptab[tid];
-  const int nst1 = KLp / BK, nst2 = KRp / BK, S = nst1 + nst2;
+  constexpr int NP2 = G::NP2;
+  const int nst1 = KLp / BK, nst2 = KRp / BK, S = nst1 + nst2 * NP2;  // sub-stages
   const int q8 = (tid & 7) * 8, rbase = tid >> 3;  // 16-byte piece of a row, first row of this thread
   const int lr = lane & 31, lk = 8 * (lane >> 5);
 
   half8 rg[NRM];
   // thread rows are rbase + 32 i: a wave-uniform base per row block (scalar
   // registers) plus one 32-bit per-lane byte offset per operand pair (phase 1:
-  // i < 4 A16, then V limb i - 4; phase 2: i < 4 L W limb i / 4 row block i % 4,
-  // then B16)
+  // i < 4 A16, then V limb i - 4; phase-2 sub-stage p: i < 8 W limb 2p + i / 4
+  // row block i % 4, then B16)
   const uint32_t off1 = (uint32_t)(rbase * KLp + q8) * 2u, off2 = (uint32_t)(rbase * KRp + q8) * 2u;
   const char *cA = reinterpret_cast<const char *>(A16 + (int64_t)m0 * KLp);
   const char *cV = reinterpret_cast<const char *>(vT + (int64_t)n0 * KLp);
@@ -361,21 +365,23 @@ __global__ void __launch_bounds__(256, 2) k_split_g2(int N2, int KLp, int KRp, c
   const char *cB = reinterpret_cast<const char *>(B16 + (int64_t)n0 * KRp);
   // (the stage offset goes into the per-lane offset: a loop-invariant base +
   // lane offset would be hoisted as one 64-bit VGPR pointer per row block)
-  auto gload = [&](int s) {
-    if (s < nst1) {
-      const uint32_t v = off1 + (uint32_t)(s * BK * 2);
+  auto gload1 = [&](int s) {
+    const uint32_t v = off1 + (uint32_t)(s * BK * 2);
 #pragma unroll
-      for (int i = 0; i < NR1; ++i) {
-        const char *u = i < 4 ? cA + (int64_t)(32 * i) * KLp * 2 : cV + (int64_t)(i - 4) * N2 * KLp * 2;
-        rg[i] = *reinterpret_cast<const half8 *>(u + v);
-      }
-    } else {
-      const uint32_t v = off2 + (uint32_t)((s - nst1) * BK * 2);
+    for (int i = 0; i < NR1; ++i) {
+      const char *u = i < 4 ? cA + (int64_t)(32 * i) * KLp * 2 : cV + (int64_t)(i - 4) * N2 * KLp * 2;
+      rg[i] = *reinterpret_cast<const half8 *>(u + v);
+    }
+  };
+  auto gload2 = [&](int kst, auto pc) {
+    constexpr int p = decltype(pc)::value;
+    const uint32_t v = off2 + (uint32_t)(kst * BK * 2);
 #pragma unroll
-      for (int i = 0; i < NR2; ++i) {
-        const char *u = i < 4 * L ? cW + ((int64_t)(i >> 2) * go.M + 32 * (i & 3)) * KRp * 2 : cB;
-        rg[i] = *reinterpret_cast<const half8 *>(u + v);
-      }
+    for (int i = 0; i < NR2; ++i) {
+      const int l = 2 * p + (i >> 2);
+      if (i < 8 && l >= L) continue;  // odd limb count: the last pair has one limb
+      const char *u = i < 8 ? cW + ((int64_t)l * go.M + 32 * (i & 3)) * KRp * 2 : cB;
+      rg[i] = *reinterpret_cast<const half8 *>(u + v);
     }
   };
   auto sstore = [&](int s) {
@@ -399,7 +405,7 @@ __global__ void __launch_bounds__(256, 2) k_split_g2(int N2, int KLp, int KRp, c
   double *Pu = EPI == EPI_BA ? ea.P + ubase : ea.D + ubase;
   double hv[16], pv[16];
 
-  gload(0);
+  gload1(0);
   auto mult1 = [&]() {
 #pragma unroll
     for (int ks = 0; ks < BK / 16; ++ks) {
@@ -409,13 +415,17 @@ __global__ void __launch_bounds__(256, 2) k_split_g2(int N2, int KLp, int KRp, c
         acc[l] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, opnd(128 + l * 32 + lr, ks), acc[l], 0, 0, 0);
     }
   };
-  auto mult2 = [&]() {
+  // phase-2 sub-stage p holds W limbs 2p, 2p + 1 (rows 0-255) and B16 (rows 256-287)
+  auto mult2 = [&](auto pc) {
+    constexpr int p = decltype(pc)::value;
 #pragma unroll
     for (int ks = 0; ks < BK / 16; ++ks) {
-      const half8 bb = opnd(128 * L + lr, ks);
+      const half8 bb = opnd(256 + lr, ks);
 #pragma unroll
-      for (int l = 0; l < L; ++l)
-        acc[l] = __builtin_amdgcn_mfma_f32_32x32x16_f16(opnd(l * 128 + wave * 32 + lr, ks), bb, acc[l], 0, 0, 0);
+      for (int li = 0; li < 2; ++li)
+        if (2 * p + li < L)
+          acc[2 * p + li] =
+              __builtin_amdgcn_mfma_f32_32x32x16_f16(opnd(li * 128 + wave * 32 + lr, ks), bb, acc[2 * p + li], 0, 0, 0);
     }
   };
   // the H and P loads of the epilogue overlap the last stage's multiplications
@@ -424,7 +434,17 @@ __global__ void __launch_bounds__(256, 2) k_split_g2(int N2, int KLp, int KRp, c
     for (int r = 0; r < 16; ++r) {
       const int64_t off = (int64_t)((r >> 2) * 8 + (r & 3)) * ldx;
       hv[r] = __builtin_nontemporal_load(Hu + off + lo);
-      if constexpr (EPI == EPI_BA) pv[r] = __builtin_nontemporal_load(Pu + off + lo);
+    }
+  };
+  // P is loaded once the accumulators are being consumed (holding both H and P
+  // through the last stage would exceed the register budget of 3 blocks per CU)
+  auto p_loads = [&]() {
+    if constexpr (EPI == EPI_BA) {
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int64_t off = (int64_t)((r >> 2) * 8 + (r & 3)) * ldx;
+        pv[r] = __builtin_nontemporal_load(Pu + off + lo);
+      }
     }
   };
   auto stage_in = [&](int s) {
@@ -433,29 +453,47 @@ __global__ void __launch_bounds__(256, 2) k_split_g2(int N2, int KLp, int KRp, c
     __syncthreads();
   };
   // one loop per phase (a phase branch inside one loop makes the compiler copy
-  // every accumulator where the two paths merge); the last stage is peeled so
-  // the epilogue loads can join it
+  // every accumulator where the two paths merge); the last sub-stage is peeled
+  // so the epilogue loads can join it
+  using P0 = std::integral_constant<int, 0>;
+  using P1c = std::integral_constant<int, 1>;
+  using PL = std::integral_constant<int, NP2 - 1>;
   if (nst2 > 0) {
 #pragma unroll 1
-    for (int s = 0; s < nst1; ++s) {
+    for (int s = 0; s + 1 < nst1; ++s) {
       stage_in(s);
-      gload(s + 1);
+      gload1(s + 1);
       mult1();
     }
+    stage_in(nst1 - 1);
+    gload2(0, P0{});
+    mult1();
 #pragma unroll 1
-    for (int s = nst1; s + 1 < S; ++s) {
-      stage_in(s);
-      gload(s + 1);
-      mult2();
+    for (int kst = 0; kst + 1 < nst2; ++kst) {
+      const int u = nst1 + kst * NP2;
+      stage_in(u);
+      if constexpr (NP2 == 2) {
+        gload2(kst, P1c{});
+        mult2(P0{});
+        stage_in(u + 1);
+      }
+      gload2(kst + 1, P0{});
+      mult2(PL{});
     }
-    stage_in(S - 1);
+    const int u = nst1 + (nst2 - 1) * NP2;
+    stage_in(u);
+    if constexpr (NP2 == 2) {
+      gload2(nst2 - 1, P1c{});
+      mult2(P0{});
+      stage_in(u + 1);
+    }
     epi_loads();
-    mult2();
+    mult2(PL{});
   } else {
 #pragma unroll 1
     for (int s = 0; s + 1 < S; ++s) {
       stage_in(s);
-      gload(s + 1);
+      gload1(s + 1);
       mult1();
     }
     stage_in(S - 1);
@@ -472,6 +510,7 @@ __global__ void __launch_bounds__(256, 2) k_split_g2(int N2, int KLp, int KRp, c
   for (int l = 0; l < L; ++l) sc[l] = ldexp(1.0, ev - bits * l);
   const double mu = ea.scal[kScalMu], dref = ea.scal[kScalDRef];
   double s0 = 0.0, m1 = -INFINITY, s2 = 0.0, wmax = 0.0;
+  double dv[16];
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
     double S = 0.0;
@@ -485,8 +524,14 @@ __global__ void __launch_bounds__(256, 2) k_split_g2(int N2, int KLp, int KRp, c
       for (int e = rs.rowptrB[x2]; e < rs.rowptrB[x2 + 1]; ++e)
         S += rs.W64[(int64_t)x1 * rs.KRp + rs.entB[2 * e]] * (double)rs.entB[2 * e + 1];
     }
+    dv[r] = fma(-S, ea.inv_c, hv[r] - mu);
+  }
+  // the accumulators and H are dead now: room for the P tile
+  p_loads();
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
     const int64_t off = (int64_t)((r >> 2) * 8 + (r & 3)) * ldx;
-    const double D = fma(-S, ea.inv_c, hv[r] - mu);
+    const double D = dv[r];
     m1 = fmax(m1, D);
     if constexpr (EPI == EPI_BA) {
       const double p = pv[r];

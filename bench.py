@@ -229,7 +229,7 @@ def main():
         limb_flops = g2f * r["g2_limbs"]  # every FP64-equivalent multiply-add runs once per limb
         roof = {"bound": "hbm", "achieved": g2bytes / g2s / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
                 "frac": g2bytes / g2s / PEAK_HBM, "traffic": measured_traffic(n, k, "g2_split"),
-                "kernel": "k_split_g2 (cross term + BA update + next p limbs)",
+                "kernel": "k_split_g2 (cross term + BA update + p tile exponents)",
                 "algorithmic_bytes_per_launch": g2bytes,
                 "mfma_tflops": limb_flops / g2s / 1e12, "mfma_frac": limb_flops / g2s / PEAK_FP16}
     else:
