@@ -42,7 +42,8 @@ class IterStats(ctypes.Structure):
 
 
 class Certificate(ctypes.Structure):
-    _fields_ = [("lower", ctypes.c_double), ("upper_cert", ctypes.c_double), ("sum_p", ctypes.c_double)]
+    _fields_ = [("lower", ctypes.c_double), ("upper_cert", ctypes.c_double), ("sum_p", ctypes.c_double),
+                ("upper_ref", ctypes.c_double)]
 
 
 _lib = None
@@ -180,7 +181,7 @@ class Engine:
     def certify(self):
         c = Certificate()
         _check(lib().bdc_certify(self._h, ctypes.byref(c)))
-        return {"lower": c.lower, "upper_cert": c.upper_cert, "sum_p": c.sum_p}
+        return {"lower": c.lower, "upper_cert": c.upper_cert, "sum_p": c.sum_p, "upper_ref": c.upper_ref}
 
     def output_law(self, w, precision=PREC_FP64):
         w = np.ascontiguousarray(w, dtype=np.float64)

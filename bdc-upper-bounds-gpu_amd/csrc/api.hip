@@ -662,6 +662,7 @@ int bdc_set_p_uniform(bdc_ctx *c) {
   BDC_LAUNCH(launch_fill(c->P.as<double>(), c->nx, 1.0 / (double)c->nx, c->stream));  // main_BAA.cu:64
   BDC_TRY(reset_scalars(c, 1.0));
   c->have_p = true;
+  c->last_mx_bits = NAN;
   c->plimb_valid = false;
   return BDC_OK;
 }
@@ -685,6 +686,7 @@ int bdc_set_p_orbits(bdc_ctx *c, const double *q, uint64_t len) {
   d.release();
   BDC_TRY(reset_scalars(c, 1.0));
   c->have_p = true;
+  c->last_mx_bits = NAN;
   c->plimb_valid = false;
   return BDC_OK;
 }
@@ -721,6 +723,7 @@ int bdc_set_p_dirichlet(bdc_ctx *c, double alpha, uint64_t seed) {
   BDC_HIP(hipStreamSynchronize(c->stream));
   BDC_TRY(reset_scalars(c, 1.0));
   c->have_p = true;
+  c->last_mx_bits = NAN;
   c->plimb_valid = false;
   return BDC_OK;
 }
@@ -746,7 +749,10 @@ int bdc_iterate(bdc_ctx *c, int prec, int iters, double tol, bdc_iter_stats *sta
   }
   if (ran && rows < c->N1) BDC_LAUNCH(launch_reflect_half(c->P.as<double>(), c->nx, c->stream));
   if (stats && ran) BDC_HIP(hipMemcpyAsync(stats, c->hist.p, (size_t)ran * sizeof(bdc_iter_stats), hipMemcpyDeviceToHost, c->stream));
+  double last_mx = c->last_mx_bits;
+  if (ran) BDC_HIP(hipMemcpyAsync(&last_mx, c->hist.as<double>() + 4 * (ran - 1), 8, hipMemcpyDeviceToHost, c->stream));
   BDC_HIP(hipStreamSynchronize(c->stream));
+  c->last_mx_bits = last_mx;
   if (done) *done = ran;
   return BDC_OK;
 }
@@ -769,6 +775,8 @@ int bdc_certify(bdc_ctx *c, bdc_certificate *out) {
   out->lower = ps * r[0];
   out->upper_cert = r[1];
   out->sum_p = ps * r[2];
+  // the reference's reported bound (main_BAA.cu:123-130): I(p_T) + mx_T
+  out->upper_ref = out->lower + c->last_mx_bits * M_LN2;
   return BDC_OK;
 }
 

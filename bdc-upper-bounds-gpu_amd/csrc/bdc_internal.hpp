@@ -118,6 +118,7 @@ struct bdc_ctx {
   bdc::DevBuf P;   // p-tilde (unnormalised), FP64; p = pscale * P
   bdc::DevBuf Hc;  // H(x)/C - ln C, FP64
   bool have_p = false;
+  double last_mx_bits = NAN;  // mx of the step that produced the current p (NaN if p was set)
   bool have_h = false;
 
   // contraction buffers

@@ -144,6 +144,46 @@ def run_dual(bdc, n, k, steps, lr, init_iters, betas):
     return out
 
 
+def run_sweep(bdc, n, ks, max_iters, tol, alpha, seed, cpu_seconds):
+    """k-sweep at BDC_(n,k): p from Dirichlet(alpha) over the orbits with
+    `seed`, Blahut-Arimoto until mx < tol (the reference rule) or `max_iters`
+    steps, at FP64 and again at split-FP16 from the same p; both are certified
+    in FP64.  A sampled CPU rate of the reference method is timed per k when
+    cpu_seconds > 0."""
+    out = {"metric": f"BAA k-sweep at BDC_({n},k), Dirichlet({alpha}) seed {seed}, tol {tol}",
+           "n": n, "tol": tol, "max_iters": max_iters, "alpha": alpha, "seed": seed, "ks": {}}
+    for k in ks:
+        t = time.perf_counter()
+        e = bdc.Engine(n, k)
+        rec = {"setup_s": time.perf_counter() - t}
+        e.set_p_dirichlet(alpha, seed)
+        p0 = e.get_p_orbits()
+        for name, prec in (("fp64", bdc.PREC_FP64), ("split", bdc.PREC_SPLIT16)):
+            e.set_p_orbits(p0)
+            e.sync()
+            t = time.perf_counter()
+            st = e.iterate(max_iters, prec, tol=tol)
+            e.sync()
+            secs = time.perf_counter() - t
+            c = e.certify()
+            rec[name] = {"iters": len(st), "converged": bool(st[-1, 0] < tol), "secs": secs,
+                         "iter_s": len(st) / secs, "mx_bits_last": float(st[-1, 0]),
+                         "lb_bits": c["lower"] / LN2, "ub_ref_bits": c["upper_ref"] / LN2,
+                         "ub_cert_bits": c["upper_cert"] / LN2}
+        a, b = rec["fp64"], rec["split"]
+        if a["iters"] == b["iters"]:
+            rec["split_vs_fp64_bits"] = {key: abs(a[key] - b[key]) for key in ("lb_bits", "ub_ref_bits", "ub_cert_bits")}
+        rec["speedup_split_vs_fp64"] = b["iter_s"] / a["iter_s"]
+        e.close()
+        if cpu_seconds > 0:
+            rec["cpu_baseline"] = cpu_baseline(n, k, cpu_seconds)
+        log(f"k={k}: fp64 {a['iters']} steps {a['iter_s']:.1f} it/s mx {a['mx_bits_last']:.2e} "
+            f"LB {a['lb_bits']:.10f} UB_ref {a['ub_ref_bits']:.10f} UB_cert {a['ub_cert_bits']:.10f}; "
+            f"split {b['iter_s']:.1f} it/s, UB_cert gap {abs(a['ub_cert_bits'] - b['ub_cert_bits']):.1e}")
+        out["ks"][str(k)] = rec
+    return out
+
+
 def cpu_baseline(n, k, seconds):
     """The oracle's reference-method passes on a sample of orbits, host cores."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
@@ -187,9 +227,21 @@ def main():
                     help="dual-bound refinement config instead (default BDC_(28,18), 200 BA steps, 500 Adam steps)")
     ap.add_argument("--dual-k", type=int, default=18)
     ap.add_argument("--adam-steps", type=int, default=500)
+    ap.add_argument("--sweep", action="store_true",
+                    help="k-sweep config instead: BDC_(24,k), k = 8..20 step 2, Dirichlet(1) seed 0, tol 1e-8")
+    ap.add_argument("--sweep-n", type=int, default=24)
+    ap.add_argument("--sweep-ks", default="8,10,12,14,16,18,20")
+    ap.add_argument("--sweep-max-iters", type=int, default=2000)
+    ap.add_argument("--tol", type=float, default=1e-8)
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.sweep:
+        import bdc_gpu as bdc
+        ks = [int(v) for v in args.sweep_ks.split(",")]
+        cs = 0.0 if args.no_cpu_baseline else min(args.cpu_seconds, 5.0)
+        print(json.dumps(run_sweep(bdc, args.sweep_n, ks, args.sweep_max_iters, args.tol, 1.0, 0, cs)), flush=True)
         return
     if args.dual:
         import bdc_gpu as bdc

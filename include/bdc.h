@@ -58,6 +58,8 @@ typedef struct {
   double lower;      /* I(p) — the reference's reported rate (main_BAA.cu:123-125) */
   double upper_cert; /* max_x D(x||q(p)) — certified upper bound on C_{n,k} */
   double sum_p;      /* sum_x p(x) (should be 1) */
+  double upper_ref;  /* I(p) + mx of the step that produced p (the reference's printed UB,
+                        main_BAA.cu:123-130); NaN when p was set rather than iterated */
 } bdc_certificate;
 
 const char *bdc_last_error(void);

@@ -120,6 +120,50 @@ def test_ba_fp64_matches_oracle(bdc, oracle, n, k, iters):
         assert (np.diff(stats[:, 3]) >= -1e-12).all()
 
 
+@pytest.mark.parametrize("k", [4, 6, 8, 10, 12, 14])
+def test_ba_dirichlet_k_sweep(bdc, oracle, k):
+    """The k-sweep workload (Dirichlet(1) init with seed 0 over orbits) at
+    n = 16: both precisions against the oracle started from the same bytes."""
+    n, iters = 16, 12
+    say(f"dirichlet sweep n={n} k={k}")
+    with bdc.Engine(n, k) as e:
+        e.set_p_dirichlet(1.0, 0)
+        p0 = e.get_p_orbits()
+        ref = oracle.BA(n, k)
+        ref.set_q(p0)
+        mx_ref = np.array([ref.iterate() for _ in range(iters)])
+        lb_ref, ub_ref = ref.rate() / LN2, ref.max_kl() / LN2
+        for prec, tol in ((bdc.PREC_FP64, 1e-10), (bdc.PREC_SPLIT16, 1e-9)):
+            e.set_p_orbits(p0)
+            stats = e.iterate(iters, prec)
+            np.testing.assert_allclose(stats[:, 0], mx_ref, rtol=0, atol=tol)
+            cert = e.certify()
+            assert abs(cert["lower"] / LN2 - lb_ref) < tol
+            assert abs(cert["upper_cert"] / LN2 - ub_ref) < tol
+            assert abs(cert["upper_ref"] / LN2 - (lb_ref + mx_ref[-1])) < tol
+
+
+def test_ba_dirichlet_n24_matches_oracle(bdc, oracle):
+    """One FP64 step of BDC_(24,8) from Dirichlet(1), seed 0, against the
+    oracle's reference-method passes (about 4 M orbits on the host cores)."""
+    n, k = 24, 8
+    say(f"dirichlet n={n} k={k}")
+    with bdc.Engine(n, k) as e:
+        e.set_p_dirichlet(1.0, 0)
+        p0 = e.get_p_orbits()
+        stats = e.iterate(1, bdc.PREC_FP64)
+        cert = e.certify()
+        p1 = e.get_p_orbits()
+    ref = oracle.BA(n, k)
+    ref.set_q(p0)
+    mx = ref.iterate()
+    assert abs(stats[0, 0] - mx) < 1e-10
+    np.testing.assert_allclose(p1, ref.get_q(), rtol=1e-10)
+    assert abs(cert["lower"] - ref.rate()) / LN2 < 1e-10
+    assert abs(cert["upper_cert"] - ref.max_kl()) / LN2 < 1e-10
+    ref.close()
+
+
 def test_ba_tolerance_stop(bdc, oracle):
     n, k = 12, 6
     say("ba tol stop")
