@@ -677,13 +677,23 @@ int bdc_set_p_orbits(bdc_ctx *c, const double *q, uint64_t len) {
   for (uint64_t i = 0; i < len; ++i)
     if (!(q[i] >= 0.0) || !std::isfinite(q[i]))
       return fail(BDC_ERR_ARG, "distribution value %llu is negative or not finite", (unsigned long long)i);
-  DevBuf d;
+  DevBuf d, part;
   BDC_TRY(d.alloc(len * 8));
+  BDC_TRY(part.alloc(1025 * 8));
   BDC_HIP(hipMemcpyAsync(d.p, q, len * 8, hipMemcpyHostToDevice, c->stream));
   BDC_LAUNCH(launch_expand_orbits(c->n, d.as<double>(), c->repmask.as<uint64_t>(), c->wordbase.as<uint32_t>(), 1.0, 0.0,
                                   c->P.as<double>(), c->stream));
+  // sum over all strings (orbit value x orbit size) must be 1
+  BDC_LAUNCH(launch_sum(c->P.as<double>(), c->nx, part.as<double>(), 1024, part.as<double>() + 1024, c->stream));
+  double total = 0.0;
+  BDC_HIP(hipMemcpyAsync(&total, part.as<double>() + 1024, 8, hipMemcpyDeviceToHost, c->stream));
   BDC_HIP(hipStreamSynchronize(c->stream));
   d.release();
+  part.release();
+  if (!(std::fabs(total - 1.0) <= 1e-6)) {
+    c->have_p = false;
+    return fail(BDC_ERR_ARG, "distribution sums to %.17g over the 2^%d strings, not 1", total, c->n);
+  }
   BDC_TRY(reset_scalars(c, 1.0));
   c->have_p = true;
   c->last_mx_bits = NAN;

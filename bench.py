@@ -321,11 +321,14 @@ def main():
         s16, s64 = res["split"]["stats"], res["fp64"]["stats"]
         t = min(len(s16), len(s64))
         lb16, lb64 = s16[:t, 3] / LN2, s64[:t, 3] / LN2
-        line["per_iteration_max_err_bits"] = {
-            "lb": float(np.abs(lb16 - lb64).max()),
-            "ub_ref": float(np.abs((lb16 + s16[:t, 0]) - (lb64 + s64[:t, 0])).max()),
-            "mx": float(np.abs(s16[:t, 0] - s64[:t, 0]).max()),
-        }
+        err = {"lb": np.abs(lb16 - lb64), "ub_ref": np.abs((lb16 + s16[:t, 0]) - (lb64 + s64[:t, 0])),
+               "mx": np.abs(s16[:t, 0] - s64[:t, 0])}
+        line["per_iteration_max_err_bits"] = {key: float(v.max()) for key, v in err.items()}
+        line["per_iteration_argmax"] = {key: int(v.argmax()) for key, v in err.items()}
+        # by window of iterations: the early steps move p by orders of magnitude
+        wins = [(a, min(b, t)) for a, b in ((0, 10), (10, 100), (100, t)) if min(b, t) > a]
+        line["per_iteration_max_err_bits_by_window"] = {
+            f"{a}-{b}": {key: float(v[a:b].max()) for key, v in err.items()} for a, b in wins}
     e.close()
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle, reference method, sampled)")
