@@ -175,7 +175,10 @@ int bdc_write_caches(int n_max, const char *dir) {
 }
 
 int bdc_supported(int n, int k) {
-  if (n < 2 || n > 28) return fail(0, "n must be in [2, 28] in this build"), 0;
+  if (n < 2 || n > 28)
+    return fail(0, "n must be in [2, 28] in this build: the entropy-term kernel keeps the half-string row sets in LDS "
+                   "(n1 <= 14), and half-length counts above 2048 (n' >= 15) would need two exact fp16 limbs"),
+           0;
   if (k < 1 || k >= n) return fail(0, "k must be in [1, n-1] (C_{n,n} = n bits needs no iteration)"), 0;
   if (k > 20) return fail(0, "k <= 20 in this build"), 0;
   return 1;

@@ -250,7 +250,7 @@ def main():
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs BDC_{n,k} on one GPU; --gpus above 1 is not supported "
-                                   "(the sharding design is described in DESIGN.md)"}))
+                                   "(sharding p by x1 row blocks with an all-reduce of q is designed, not built)"}))
         sys.exit(2)
 
     import bdc_gpu as bdc
