@@ -205,6 +205,10 @@ int bdc_create(int n, int k, const char *cache_dir, bdc_ctx **out) {
   c->log_c = std::log(c->nck);
   if (hipGetDevice(&c->device) != hipSuccess) return cleanup(fail(BDC_ERR_HIP, "no HIP device"));
   if (hipStreamCreate(&c->stream) != hipSuccess) return cleanup(fail(BDC_ERR_HIP, "hipStreamCreate failed"));
+  {
+    const char *e = getenv("BDC_STAGE_FACTORED");
+    c->stage_factored = !(e && e[0] == '0');
+  }
 
   // split sets
   const int jlo = std::max(0, k - c->n2), jhi = std::min(k, c->n1);
@@ -452,8 +456,31 @@ static int output_law(bdc_ctx *c, int prec, const double *X, int scale_slot, boo
   }
   double *qL = c->qpart.as<double>();
   double *qR = qL + c->JL.size() * ny;
-  // stage 2, j in JL: q_j = T_j B_{k-j}
-  for (size_t i = 0; i < c->JL.size(); ++i) {
+  // stage 2, j in JL: q_j = T_j B_{k-j}, all j in one launch of factored
+  // single-deletion steps (row y1 of T_j, 2^n2 entries, to 2^(k-j) entries)
+  bool staged = false;
+  if (c->stage_factored) {
+    ChainJobs jb{};
+    jb.count = (int)c->JL.size();
+    int rows = 0;
+    for (size_t i = 0; i < c->JL.size() && jb.count <= kMaxChainJobs; ++i) {
+      const int j = c->JL[i], m = k - j;
+      jb.row0[i] = rows;
+      rows += 1 << j;
+      jb.d[i] = c->n2 - m;
+      jb.src_off[i] = (int64_t)c->offL[i] * N2;
+      jb.src_ld[i] = N2;
+      jb.dst_off[i] = (int64_t)i * ny;
+      jb.dst_ld[i] = (int64_t)1 << m;
+    }
+    if (jb.count <= kMaxChainJobs) {
+      jb.row0[jb.count] = rows;
+      const int rc = launch_chain_ins(c->T.as<double>(), qL, c->n2, jb, c->stream);
+      if (rc == BDC_ERR_HIP) return fail(rc, "chain_ins launch failed");
+      staged = rc == BDC_OK;
+    }
+  }
+  for (size_t i = 0; i < c->JL.size() && !staged; ++i) {
     const int j = c->JL[i], m = k - j;
     const double *Tj = c->T.as<double>() + (size_t)c->offL[i] * N2;
     double *dst = qL + i * ny;
@@ -513,7 +540,31 @@ static int output_law(bdc_ctx *c, int prec, const double *X, int scale_slot, boo
 // only the first `rows` rows of Wbar are formed then.
 static int cross_operands(bdc_ctx *c, const double *L, bool mirror = false, int rows = 0) {
   const int k = c->k, N1 = c->N1, N2 = c->N2;
-  for (size_t i = 0; i < c->JL.size(); ++i) {
+  // V_j = L_j B_{k-j}^T for all j in JL in one launch of factored
+  // single-deletion steps (row y1 of L_j, 2^(k-j) entries, to 2^n2 entries)
+  bool staged = false;
+  if (c->stage_factored) {
+    ChainJobs jb{};
+    jb.count = (int)c->JL.size();
+    int nrows = 0;
+    for (size_t i = 0; i < c->JL.size() && jb.count <= kMaxChainJobs; ++i) {
+      const int j = c->JL[i], m = k - j;
+      jb.row0[i] = nrows;
+      nrows += 1 << j;
+      jb.d[i] = c->n2 - m;
+      jb.src_off[i] = 0;
+      jb.src_ld[i] = (int64_t)1 << m;
+      jb.dst_off[i] = (int64_t)c->offL[i] * N2;
+      jb.dst_ld[i] = N2;
+    }
+    if (jb.count <= kMaxChainJobs) {
+      jb.row0[jb.count] = nrows;
+      const int rc = launch_chain_del(L, c->g2b.as<double>(), c->n2, jb, c->stream);
+      if (rc == BDC_ERR_HIP) return fail(rc, "chain_del launch failed");
+      staged = rc == BDC_OK;
+    }
+  }
+  for (size_t i = 0; i < c->JL.size() && !staged; ++i) {
     const int j = c->JL[i], m = k - j;
     double *dst = c->g2b.as<double>() + (size_t)c->offL[i] * N2;
     const int d = c->n2 - m;

@@ -102,6 +102,23 @@ constexpr int kMaxGatherInsertions = 2;  // insertion form: 2^d C(n,d) terms per
 // Needs M <= 128, N % 64 == 0, K % 32 == 0 and (K / 32) % splits == 0; else BDC_ERR_ARG.
 int launch_count_gemm(int M, int N, int K, const double *A, int64_t lda, const uint16_t *B, int64_t ldb, int transb,
                       double *C, int64_t ldc, int splits, int64_t zstride, hipStream_t s);
+// ---- factored stage contractions (kernels_stage.hip) ----
+// One block per row of every job, one launch per stage.  Row y of job i reads
+// src + src_off[i] + y * src_ld[i] and writes dst + dst_off[i] + y * dst_ld[i].
+//   chain_ins: long rows (2^L entries) to short rows (2^m, m = L - d):
+//     out[z] = sum_x in[x] c(x, z) over the half-length counts c = cache_{L,m}
+//   chain_del: short rows (2^m) to long rows (2^L):
+//     out[x] = sum_z c(x, z) in[z]
+// Needs 2^L doubles of LDS (L <= 14) and at most kMaxChainJobs jobs; else BDC_ERR_ARG.
+constexpr int kMaxChainJobs = 24;
+struct ChainJobs {
+  int count;
+  int row0[kMaxChainJobs + 1];  // first block of each job (prefix sum of the row counts)
+  int d[kMaxChainJobs];         // deletions L - m
+  int64_t src_off[kMaxChainJobs], src_ld[kMaxChainJobs], dst_off[kMaxChainJobs], dst_ld[kMaxChainJobs];
+};
+int launch_chain_ins(const double *src, double *dst, int L, const ChainJobs &jobs, hipStream_t s);
+int launch_chain_del(const double *src, double *dst, int L, const ChainJobs &jobs, hipStream_t s);
 struct MirrorW {  // for each j in JR: column offset in Wbar, m = k - j, row offset of V_{k-j} in Vbar
   int count;
   int offR[18], m[18], offL[18];

@@ -179,6 +179,145 @@ int launch_w_from_v(const MirrorW &mw, const double *V, int N2, int n2, double *
 }
 
 // ---------------------------------------------------------------------------
+// Factored stage contractions.  A count matrix between strings of lengths L and
+// m = L - d is a power of the single-deletion operator:
+//   c(x, z) = (D^d)(x, z) / d!,   D(u, v) = #{ i : u with bit i removed = v },
+// because each set of d deleted positions is reached by exactly d! ordered
+// deletion sequences (one per order), and a sequence through intermediate
+// strings u_1 .. u_{d-1} is counted by the product of the D factors.  So a
+// stage contraction is d sparse steps (2l or l + 1 terms per output at length
+// l) on one row held in LDS, instead of streaming the 2^(L+m)-entry count
+// matrix: the counts never leave the index arithmetic.
+//   insertion form (long -> short): out[v] = sum_{p < l, b} in[ins(v, p, b)]
+//     (inserting bit b at position p of v; each u with D(u, v) = r is hit r times)
+//   deletion form (short -> long):  out[u] = sum_{i <= l} in[del(u, i)]
+// All sums run in a fixed order; the 1/d! scale is applied once at the end.
+__device__ inline uint32_t ins_bit(uint32_t v, int p, uint32_t b) {
+  return ((v >> p) << (p + 1)) | (b << p) | (v & ((1u << p) - 1u));
+}
+__device__ inline uint32_t del_bit(uint32_t u, int i) { return ((u >> (i + 1)) << i) | (u & ((1u << i) - 1u)); }
+__device__ inline double factorial(int d) {
+  double f = 1.0;
+  for (int i = 2; i <= d; ++i) f *= (double)i;
+  return f;
+}
+__device__ inline int chain_job(const ChainJobs &jb, int b) {
+  int i = 0;
+  while (i + 1 < jb.count && b >= jb.row0[i + 1]) ++i;
+  return i;
+}
+
+__global__ void __launch_bounds__(1024) k_chain_ins(const double *__restrict__ src, double *__restrict__ dst, int L,
+                                                    ChainJobs jb) {
+  extern __shared__ double lds[];
+  const int i = chain_job(jb, blockIdx.x), y = blockIdx.x - jb.row0[i], d = jb.d[i];
+  const double *in = src + jb.src_off[i] + (int64_t)y * jb.src_ld[i];
+  double *out = dst + jb.dst_off[i] + (int64_t)y * jb.dst_ld[i];
+  const uint32_t tid = threadIdx.x, bd = blockDim.x;
+  if (d == 0) {
+    for (uint32_t t = tid; t < (1u << L); t += bd) out[t] = in[t];
+    return;
+  }
+  const double f = factorial(d);
+  // step 1 streams the long row through LDS in two halves (the whole row and
+  // the step's output would not fit): an output v takes the top-position
+  // insertion from each half, and its other insertions from the half of its
+  // own top bit
+  const uint32_t H = 1u << (L - 1);
+  double *acc = lds, *half = lds + H;
+  for (uint32_t h = 0; h < 2; ++h) {
+    __syncthreads();  // the previous half has been read
+    for (uint32_t t = tid; t < H; t += bd) half[t] = in[h * H + t];
+    __syncthreads();
+    for (uint32_t v = tid; v < H; v += bd) {
+      double a = h ? acc[v] : 0.0;
+      a += half[v];
+      if (L >= 2 && (v >> (L - 2)) == h)
+        for (int p = 0; p < L - 1; ++p) {
+          a += half[ins_bit(v, p, 0u) - h * H];
+          a += half[ins_bit(v, p, 1u) - h * H];
+        }
+      acc[v] = a;
+    }
+  }
+  __syncthreads();
+  if (d == 1) {
+    for (uint32_t v = tid; v < H; v += bd) out[v] = acc[v];
+    return;
+  }
+  double *cur = acc, *nxt = half;
+  int l = L - 1;  // bits of cur
+  for (int s = 2; s <= d; ++s, --l) {
+    const uint32_t nout = 1u << (l - 1);
+    for (uint32_t v = tid; v < nout; v += bd) {
+      double a = 0.0;
+      for (int p = 0; p < l; ++p) {
+        a += cur[ins_bit(v, p, 0u)];
+        a += cur[ins_bit(v, p, 1u)];
+      }
+      if (s == d)
+        out[v] = a / f;
+      else
+        nxt[v] = a;
+    }
+    double *t = cur;
+    cur = nxt;
+    nxt = t;
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(1024) k_chain_del(const double *__restrict__ src, double *__restrict__ dst, int L,
+                                                    ChainJobs jb) {
+  extern __shared__ double lds[];
+  const int i = chain_job(jb, blockIdx.x), y = blockIdx.x - jb.row0[i], d = jb.d[i], m = L - d;
+  const double *in = src + jb.src_off[i] + (int64_t)y * jb.src_ld[i];
+  double *out = dst + jb.dst_off[i] + (int64_t)y * jb.dst_ld[i];
+  const uint32_t tid = threadIdx.x, bd = blockDim.x;
+  if (d == 0) {
+    for (uint32_t t = tid; t < (1u << L); t += bd) out[t] = in[t];
+    return;
+  }
+  const double f = factorial(d);
+  double *cur = lds, *nxt = lds + (1u << (L - 1));  // each region holds up to 2^(L-1) entries
+  for (uint32_t t = tid; t < (1u << m); t += bd) cur[t] = in[t];
+  __syncthreads();
+  int l = m;  // bits of cur
+  for (int s = 1; s <= d; ++s, ++l) {
+    const uint32_t nout = 1u << (l + 1);
+    for (uint32_t u = tid; u < nout; u += bd) {
+      double a = 0.0;
+      for (int q = 0; q <= l; ++q) a += cur[del_bit(u, q)];
+      if (s == d)
+        out[u] = a / f;
+      else
+        nxt[u] = a;
+    }
+    double *t = cur;
+    cur = nxt;
+    nxt = t;
+    __syncthreads();
+  }
+}
+
+static int chain_check(int L, const ChainJobs &jb) {
+  if (L < 1 || L > 14 || jb.count < 1 || jb.count > kMaxChainJobs || jb.row0[0] != 0) return BDC_ERR_ARG;
+  for (int i = 0; i < jb.count; ++i)
+    if (jb.d[i] < 0 || jb.d[i] > L || jb.row0[i + 1] <= jb.row0[i]) return BDC_ERR_ARG;
+  return BDC_OK;
+}
+int launch_chain_ins(const double *src, double *dst, int L, const ChainJobs &jobs, hipStream_t s) {
+  if (chain_check(L, jobs) != BDC_OK) return BDC_ERR_ARG;
+  hipLaunchKernelGGL(k_chain_ins, dim3(jobs.row0[jobs.count]), dim3(1024), sizeof(double) << L, s, src, dst, L, jobs);
+  return hipGetLastError() == hipSuccess ? BDC_OK : BDC_ERR_HIP;
+}
+int launch_chain_del(const double *src, double *dst, int L, const ChainJobs &jobs, hipStream_t s) {
+  if (chain_check(L, jobs) != BDC_OK) return BDC_ERR_ARG;
+  hipLaunchKernelGGL(k_chain_del, dim3(jobs.row0[jobs.count]), dim3(1024), sizeof(double) << L, s, src, dst, L, jobs);
+  return hipGetLastError() == hipSuccess ? BDC_OK : BDC_ERR_HIP;
+}
+
+// ---------------------------------------------------------------------------
 // Stage contractions with a many-deletion count factor: a skinny FP64 product
 // against a half-length count matrix (u16), which is the large operand
 // (cache_{14,11} is 64 MiB).  The counts are streamed once with 16-byte loads,

@@ -164,6 +164,31 @@ def test_ba_dirichlet_n24_matches_oracle(bdc, oracle):
     ref.close()
 
 
+@pytest.mark.parametrize("n,k", [(16, 8), (15, 9), (18, 8), (20, 6), (14, 13)])
+def test_factored_stage_matches_count_matrices(bdc, oracle, n, k):
+    """Stage contractions as powers of the single-deletion operator (the
+    default) against the count-matrix kernels (BDC_STAGE_FACTORED=0), through
+    the output law and the cross term, and both against the oracle."""
+    say(f"factored stage n={n} k={k}")
+    rng = np.random.default_rng(5)
+    w = rng.random(1 << n)
+    w /= w.sum()
+    q = rng.random(1 << k) + 0.01
+    q /= q.sum()
+    res = {}
+    for flag in ("1", "0"):
+        os.environ["BDC_STAGE_FACTORED"] = flag
+        try:
+            with bdc.Engine(n, k) as e:
+                res[flag] = (e.output_law(w), e.kl(np.log(q)))
+        finally:
+            os.environ.pop("BDC_STAGE_FACTORED", None)
+    np.testing.assert_allclose(res["1"][0], res["0"][0], rtol=1e-13, atol=0)
+    np.testing.assert_allclose(res["1"][1], res["0"][1], rtol=0, atol=1e-12)
+    np.testing.assert_allclose(res["1"][0], oracle.full_output_law(n, k, w), rtol=1e-12, atol=1e-300)
+    np.testing.assert_allclose(res["1"][1], oracle.full_kl(n, k, np.log(q)), rtol=0, atol=1e-12)
+
+
 def test_ba_tolerance_stop(bdc, oracle):
     n, k = 12, 6
     say("ba tol stop")
