@@ -26,8 +26,9 @@ EXPORTS = [
     "bdc_supported", "bdc_num_orbits", "bdc_get_orbits", "bdc_set_p_uniform", "bdc_set_p_orbits",
     "bdc_get_p_orbits", "bdc_set_p_dirichlet", "bdc_iterate", "bdc_certify", "bdc_output_law", "bdc_kl",
     "bdc_entropy_term", "bdc_dual", "bdc_refine_dual", "bdc_certify_q", "bdc_log_output", "bdc_sync",
-    "bdc_kernel_time", "bdc_reset_kernel_timers", "bdc_work_model", "bdc_split_config",
+    "bdc_kernel_time", "bdc_reset_kernel_timers", "bdc_work_model", "bdc_split_config", "bdc_sample_inputs",
 ]
+SAMPLE_ENTROPY, SAMPLE_P, SAMPLE_KL = 0, 1, 2
 
 
 class BdcError(RuntimeError):
@@ -80,6 +81,7 @@ def lib():
         L.bdc_output_law.argtypes = [vp, i32, vp, vp]
         L.bdc_kl.argtypes = [vp, i32, vp, vp]
         L.bdc_entropy_term.argtypes = [vp, vp]
+        L.bdc_sample_inputs.argtypes = [vp, i32, vp, u64, vp]
         L.bdc_dual.argtypes = [vp, i32, vp, dbl, ctypes.POINTER(dbl), vp]
         L.bdc_refine_dual.argtypes = [vp, i32, vp, dbl, i32, dbl, vp]
         L.bdc_certify_q.argtypes = [vp, vp, ctypes.POINTER(dbl)]
@@ -201,6 +203,13 @@ class Engine:
         h = np.empty(1 << self.n)
         _check(lib().bdc_entropy_term(self._h, _ptr(h)))
         return h
+
+    def sample(self, what, xs):
+        """values at selected inputs (SAMPLE_ENTROPY, SAMPLE_P or SAMPLE_KL)"""
+        xs = np.ascontiguousarray(xs, dtype=np.uint32)
+        out = np.empty(xs.size)
+        _check(lib().bdc_sample_inputs(self._h, what, _ptr(xs), xs.size, _ptr(out)))
+        return out
 
     def dual(self, theta, beta, precision=PREC_FP64, want_grad=True):
         theta = np.ascontiguousarray(theta, dtype=np.float64)

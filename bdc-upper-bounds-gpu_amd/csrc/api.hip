@@ -175,12 +175,11 @@ int bdc_write_caches(int n_max, const char *dir) {
 }
 
 int bdc_supported(int n, int k) {
-  if (n < 2 || n > 28)
-    return fail(0, "n must be in [2, 28] in this build: the entropy-term kernel keeps the half-string row sets in LDS "
-                   "(n1 <= 14), and half-length counts above 2048 (n' >= 15) would need two exact fp16 limbs"),
-           0;
+  // backend/config.hpp:24 caps n at 32; the reference itself overflows `1 << n`
+  // at n >= 31 (main_BAA.cu:47,64), this engine indexes inputs in 64 bits
+  if (n < 2 || n > 32) return fail(0, "n must be in [2, 32]"), 0;
   if (k < 1 || k >= n) return fail(0, "k must be in [1, n-1] (C_{n,n} = n bits needs no iteration)"), 0;
-  if (k > 20) return fail(0, "k <= 20 in this build"), 0;
+  if (k > 20) return fail(0, "k <= 20 in this build (2^k outputs held per stage buffer)"), 0;
   return 1;
 }
 
@@ -651,6 +650,7 @@ static int ensure_hist(bdc_ctx *c, int n) {
 }
 
 static int ensure_dvec(bdc_ctx *c) {
+  c->have_d = false;  // every user but bdc_certify overwrites D(x||q)
   if (c->dvec.p) return BDC_OK;
   return c->dvec.alloc(c->nx * 8);
 }
@@ -831,6 +831,7 @@ int bdc_certify(bdc_ctx *c, bdc_certificate *out) {
   BDC_TRY(cross_operands(c, L));
   int nb = 0;
   BDC_TRY(cross_term(c, BDC_PREC_FP64, EPI_D, c->dvec.as<double>(), &nb));
+  c->have_d = true;
   double r[3], ps = 0.0;
   BDC_LAUNCH(launch_dot(c->P.as<double>(), c->dvec.as<double>(), c->nx, c->part.as<double>(), c->splitk.as<double>(), c->stream));
   BDC_HIP(hipMemcpyAsync(r, c->splitk.p, 24, hipMemcpyDeviceToHost, c->stream));
@@ -884,6 +885,36 @@ int bdc_kl(bdc_ctx *c, int prec, const double *log_q, double *d) {
 int bdc_entropy_term(bdc_ctx *c, double *h) {
   if (!c || !h) return fail(BDC_ERR_ARG, "null argument");
   BDC_HIP(hipMemcpyAsync(h, c->Hc.p, c->nx * 8, hipMemcpyDeviceToHost, c->stream));
+  BDC_HIP(hipStreamSynchronize(c->stream));
+  return BDC_OK;
+}
+
+int bdc_sample_inputs(bdc_ctx *c, int what, const uint32_t *x, uint64_t len, double *out) {
+  if (!c || !x || !out) return fail(BDC_ERR_ARG, "null argument");
+  if (len == 0) return BDC_OK;
+  if (len > (1ull << 24)) return fail(BDC_ERR_ARG, "at most 2^24 inputs per call");
+  for (uint64_t i = 0; i < len; ++i)
+    if ((uint64_t)x[i] >= c->nx) return fail(BDC_ERR_ARG, "input %u out of range for n = %d", x[i], c->n);
+  const double *src = nullptr;
+  int slot = -1;
+  if (what == BDC_SAMPLE_ENTROPY) {
+    src = c->Hc.as<double>();
+  } else if (what == BDC_SAMPLE_P) {
+    if (!c->have_p) return fail(BDC_ERR_STATE, "p is not set");
+    src = c->P.as<double>();
+    slot = kScalPScale;
+  } else if (what == BDC_SAMPLE_KL) {
+    if (!c->have_d) return fail(BDC_ERR_STATE, "no D(x||q) pass yet (call bdc_certify)");
+    src = c->dvec.as<double>();
+  } else {
+    return fail(BDC_ERR_ARG, "unknown sample kind %d", what);
+  }
+  DevBuf dx, dout;
+  BDC_TRY(dx.alloc(len * 4));
+  BDC_TRY(dout.alloc(len * 8));
+  BDC_HIP(hipMemcpyAsync(dx.p, x, len * 4, hipMemcpyHostToDevice, c->stream));
+  BDC_LAUNCH(launch_gather_at(src, dx.as<uint32_t>(), len, c->scal.as<double>(), slot, dout.as<double>(), c->stream));
+  BDC_HIP(hipMemcpyAsync(out, dout.p, len * 8, hipMemcpyDeviceToHost, c->stream));
   BDC_HIP(hipStreamSynchronize(c->stream));
   return BDC_OK;
 }

@@ -120,6 +120,7 @@ struct bdc_ctx {
   bool have_p = false;
   double last_mx_bits = NAN;  // mx of the step that produced the current p (NaN if p was set)
   bool have_h = false;
+  bool have_d = false;  // dvec holds D(x||q) of the last bdc_certify
   bool stage_factored = true;  // stage contractions as factored single-deletion steps (BDC_STAGE_FACTORED=0: count matrices)
 
   // contraction buffers

@@ -25,6 +25,9 @@ int launch_gather_orbits(uint64_t norb, const uint32_t *orbits, const double *P,
 int launch_dirichlet_orbits(int n, uint64_t norb, const uint32_t *orbits, double alpha, uint64_t seed, double *Q,
                             double *mass, hipStream_t s);
 int launch_sum(const double *v, uint64_t len, double *part, int np, double *out, hipStream_t s);
+// out[i] = scal[slot] * src[idx[i]] (scale 1 when slot < 0), len <= 2^28
+int launch_gather_at(const double *src, const uint32_t *idx, uint64_t len, const double *scal, int slot, double *out,
+                     hipStream_t s);
 // logtab: ln(1 + j/64) then 1/(1 + j/64), j = 0..64 (130 doubles)
 int launch_entropy_reps(int n, int k, int n1, int n2, const uint64_t *mask, const uint32_t *wordbase,
                         const double *logtab, double *Hrep, hipStream_t s);

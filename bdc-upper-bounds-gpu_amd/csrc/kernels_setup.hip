@@ -1,6 +1,7 @@
 // Setup kernels: subsequence-count caches, stacked count operands,
 // complement/reversal orbits and ranks, initial distributions, and the hoisted
 // entropy term H(x)/C - ln C.  All run once per context.
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 
@@ -190,13 +191,17 @@ int launch_orbit_list(uint64_t nwords, const uint64_t *mask, const uint32_t *wor
 }
 
 // ---------------------------------------------------------------- distributions
+// 1-D grids: at most 2^20 blocks of 256 (a dispatch holds fewer than 2^32
+// work-items per dimension, and 2^n = 2^32 inputs at n = 32); kernels loop.
+static unsigned grid_for(uint64_t len) { return (unsigned)std::min<uint64_t>(1u << 20, (len + 255) / 256); }
+
 // P[x] = val (uniform, backend/main_BAA.cu:64)
 __global__ void k_fill(double *__restrict__ P, uint64_t len, double val) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < len) P[i] = val;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < len; i += (uint64_t)gridDim.x * blockDim.x)
+    P[i] = val;
 }
 int launch_fill(double *P, uint64_t len, double val, hipStream_t s) {
-  hipLaunchKernelGGL(k_fill, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, s, P, len, val);
+  hipLaunchKernelGGL(k_fill, dim3(grid_for(len)), dim3(256), 0, s, P, len, val);
   return hipGetLastError() == hipSuccess ? BDC_OK : BDC_ERR_HIP;
 }
 
@@ -204,16 +209,15 @@ int launch_fill(double *P, uint64_t len, double val, hipStream_t s) {
 __global__ void k_expand_orbits(int n, const double *__restrict__ Q, const uint64_t *__restrict__ mask,
                                 const uint32_t *__restrict__ wordbase, double scale, double add,
                                 double *__restrict__ P) {
-  const uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (x >= (1ull << n)) return;
-  const uint32_t r = rep_of((uint32_t)x, n);
-  P[x] = scale * Q[rank_of_rep(r, mask, wordbase)] + add;
+  const uint64_t nx = 1ull << n;
+  for (uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; x < nx; x += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t r = rep_of((uint32_t)x, n);
+    P[x] = scale * Q[rank_of_rep(r, mask, wordbase)] + add;
+  }
 }
 int launch_expand_orbits(int n, const double *Q, const uint64_t *mask, const uint32_t *wordbase, double scale,
                          double add, double *P, hipStream_t s) {
-  const uint64_t tot = 1ull << n;
-  hipLaunchKernelGGL(k_expand_orbits, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, n, Q, mask, wordbase,
-                     scale, add, P);
+  hipLaunchKernelGGL(k_expand_orbits, dim3(grid_for(1ull << n)), dim3(256), 0, s, n, Q, mask, wordbase, scale, add, P);
   return hipGetLastError() == hipSuccess ? BDC_OK : BDC_ERR_HIP;
 }
 
@@ -227,6 +231,18 @@ int launch_gather_orbits(uint64_t norb, const uint32_t *orbits, const double *P,
                          hipStream_t s) {
   hipLaunchKernelGGL(k_gather_orbits, dim3((unsigned)((norb + 255) / 256)), dim3(256), 0, s, norb, orbits, P, scal,
                      Q);
+  return hipGetLastError() == hipSuccess ? BDC_OK : BDC_ERR_HIP;
+}
+
+// out[i] = scale * src[idx[i]], scale = scal[slot] (1 when slot < 0)
+__global__ void k_gather_at(const double *__restrict__ src, const uint32_t *__restrict__ idx, uint64_t len,
+                            const double *__restrict__ scal, int slot, double *__restrict__ out) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < len) out[i] = (slot >= 0 ? scal[slot] : 1.0) * src[idx[i]];
+}
+int launch_gather_at(const double *src, const uint32_t *idx, uint64_t len, const double *scal, int slot, double *out,
+                     hipStream_t s) {
+  hipLaunchKernelGGL(k_gather_at, dim3(grid_for(len)), dim3(256), 0, s, src, idx, len, scal, slot, out);
   return hipGetLastError() == hipSuccess ? BDC_OK : BDC_ERR_HIP;
 }
 
@@ -305,22 +321,36 @@ int launch_sum(const double *v, uint64_t len, double *part, int np, double *out,
 
 // ---------------------------------------------------------------- entropy term
 // H(x) = sum_y c(x,y) ln c(x,y) for every representative x (the per-pair
-// W ln W of backend/BAA.cu:401-411, hoisted out of the iteration).
-// One block per x1.  The subsequence counts of a half string s (length L) for
-// every z of length d < L (level L is the identity, kept implicit) form its
-// "row set" R(s)[2^d - 1 + z]; every entry is built independently from quarter
-// tables of the two halves of s (lengths a = ceil(L/2), b = floor(L/2)):
-//   c(s, z) = sum_i c(s_hi, first i symbols of z) c(s_lo, last d - i symbols),
-// so no level-by-level pass (and no barrier per level) is needed.  The block
-// keeps R(x1) and, for each representative (x1, x2), builds R(x2) and sweeps all
-// y in {0,1}^k with c(x, y) = sum_j c(x1, y_hi(j)) c(x2, y_lo(k-j)), adding
-// c ln c with a table-driven integer log.
-struct QuarterTabs {
-  int off[8];  // byte offset in LDS of the table for strings of length l (0: absent)
+// W ln W of backend/BAA.cu:401-411, hoisted out of the iteration), with
+//   c(x, y) = sum_j c(x1, y_hi(j)) c(x2, y_lo(k-j))      (backend/BAA.cu:355-392).
+// The block for x1 holds the "row set" of x1, A[j][z] = c(x1, z) for |z| = j,
+// and, for each representative x = (x1, x2) in turn, the row set B[m][z] =
+// c(x2, z) of x2; then it sweeps the outputs y.  Row-set entries are built
+// independently (no level-by-level pass) from two per-string half tables:
+//   c(s, z) = sum_i c(s_hi, first i symbols of z) c(s_lo, last |z| - i symbols).
+// When both row sets do not fit LDS (half length 16 with k >= 16, or 16/15
+// with large k), each launch covers one quadrant of the outputs: the top bit
+// of y fixed to t (A keeps only the entries z whose top bit is t) and the bottom
+// bit fixed to b (B keeps only the entries whose bottom bit is b); the four
+// launches add their sums in a fixed order.  A thread sweeps 4 outputs that
+// differ in two adjacent bits, so for all but the last splits the A entry is
+// shared and the 4 B entries are consecutive (one 8-byte LDS load).
+struct EntropyArgs {
+  int n, k, n1, n2;
+  int rt, rb;        // restrict the A row set to y's top bit t / the B row set to y's bottom bit b
+  int t, b;          // quadrant of this launch
+  int jlo, jhi;      // split range (backend/BAA.cu:355-357)
+  int aLo, aHi;      // stored levels of A (non-identity) and of B
+  int bLo, bHi;
+  int aWords, bWords;  // u16 entries of the A and B regions (multiples of 4)
+  int groups;        // 4-output groups per quadrant
+  uint32_t x1base;
+  int accumulate;    // 0: write H, 1: add to H
+  int htoff[9];      // byte offset of the half tables of strings of length l (l <= 8) in htab, -1: absent
 };
-// occurrences of z (length lz) as a subsequence of s (length ls), both MSB first, ls <= 7
+// occurrences of z (length lz) as a subsequence of s (length ls), both MSB first, ls <= 8
 __device__ inline uint32_t count_small(uint32_t s, int ls, uint32_t z, int lz) {
-  uint32_t dp[8] = {1, 0, 0, 0, 0, 0, 0, 0};
+  uint32_t dp[9] = {1, 0, 0, 0, 0, 0, 0, 0, 0};
   for (int i = 0; i < ls; ++i) {
     const uint32_t si = (s >> (ls - 1 - i)) & 1u;
     for (int j = (i + 1 < lz ? i + 1 : lz); j >= 1; --j)
@@ -328,109 +358,225 @@ __device__ inline uint32_t count_small(uint32_t s, int ls, uint32_t z, int lz) {
   }
   return dp[lz];
 }
-// table for length l: [s < 2^l][2^d - 1 + z, d <= l] as u8 (counts <= C(7,3) = 35)
-__device__ inline int qtab_bytes(int l) { return (1 << l) * ((2 << l) - 1); }
-__device__ inline uint32_t qget(const uint8_t *T, int l, uint32_t s, int d, uint32_t z) {
-  return T[s * ((2 << l) - 1) + (1 << d) - 1 + z];
+// Level l of a row set: offset and entry count.  Full layout: [2^l, 2^(l+1));
+// restricted (half of every level l >= 1): [2^(l-1), 2^l), level 0 at 0.
+__device__ inline int lev_off(int l, int r) { return r ? (l ? 1 << (l - 1) : 0) : (1 << l); }
+__device__ inline uint32_t lev_mask_top(int l, int r) { return r ? (l ? (1u << (l - 1)) - 1u : 0u) : (1u << l) - 1u; }
+// Half tables of every string h of length l <= 8: row h of (2 << l) - 1 bytes,
+// T[2^d - 1 + w] = c(h, w) for |w| = d <= l (counts <= C(8,4) = 70).  Built
+// once per entropy pass; every row set of the pass reads them.
+__global__ void k_half_tables(int l, uint8_t *__restrict__ tab) {
+  const int per = (2 << l) - 1;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (1 << l) * per) return;
+  const int h = i / per, e = i % per, d = 31 - __clz(e + 1);
+  tab[i] = (uint8_t)count_small((uint32_t)h, l, (uint32_t)(e + 1 - (1 << d)), d);
 }
-// R(s)[2^d - 1 + z] for d < L
-__device__ void build_rowset(uint32_t s, int L, const uint8_t *lds8, const QuarterTabs &qt, uint16_t *R) {
-  const int a = (L + 1) / 2, b = L / 2;
-  const uint8_t *Ta = lds8 + qt.off[a], *Tb = lds8 + qt.off[b];
-  const uint32_t sh = s >> b, sl = s & ((1u << b) - 1u);
-  const int tot = (1 << L) - 1;
-  for (int e = threadIdx.x; e < tot; e += blockDim.x) {
-    const int d = 31 - __clz(e + 1);
-    const uint32_t z = (uint32_t)(e + 1 - (1 << d));
-    uint32_t c = 0;
-    const int ilo = d - b > 0 ? d - b : 0, ihi = d < a ? d : a;
-    for (int i = ilo; i <= ihi; ++i)
-      c += __umul24(qget(Ta, a, sh, i, z >> (d - i)), qget(Tb, b, sl, d - i, z & ((1u << (d - i)) - 1u)));
-    R[e] = (uint16_t)c;
+__device__ inline const uint8_t *half_row(const uint8_t *htab, const EntropyArgs &ea, int l, uint32_t h) {
+  return htab + ea.htoff[l] + (size_t)h * ((2 << l) - 1);
+}
+// c(s, z), |z| = d < |s|, s = (s_hi of length a, s_lo of length bl).  The split
+// points that use a half whole are identity factors (a string's only
+// subsequence of its own length is itself), so those two terms are compares.
+__device__ inline uint32_t count_halves(const uint8_t *Th, int a, uint32_t sh, const uint8_t *Tl, int bl, uint32_t sl,
+                                        uint32_t z, int d) {
+  uint32_t c = 0;
+  int ilo = d - bl > 0 ? d - bl : 0, ihi = d < a ? d : a;
+  if (ihi == a) {  // i = a: z's first a symbols must be s_hi
+    const int r = d - a;
+    if ((z >> r) == sh) c += Tl[(1 << r) - 1 + (z & ((1u << r) - 1u))];
+    --ihi;
+  }
+  if (ilo == d - bl && ilo <= ihi) {  // d - i = bl: z's last bl symbols must be s_lo
+    if ((z & ((1u << bl) - 1u)) == sl) c += Th[(1 << ilo) - 1 + (z >> bl)];
+    ++ilo;
+  }
+  for (int i = ilo; i <= ihi; ++i) {
+    const int r = d - i;
+    c += __umul24(Th[(1 << i) - 1 + (z >> r)], Tl[(1 << r) - 1 + (z & ((1u << r) - 1u))]);
+  }
+  return c;
+}
+// Row-set levels [lo, hi] of s into R.  top = 1: restricted to entries whose
+// top bit is `bit` (A side); top = 0: restricted to entries whose bottom bit is
+// `bit` (B side); r = 0: unrestricted.
+__device__ inline void build_levels(const uint8_t *Th, const uint8_t *Tl, uint32_t s, int L, int lo, int hi, int r,
+                                    int top, uint32_t bit, uint16_t *R) {
+  if (lo > hi) return;
+  const int a = (L + 1) / 2, bl = L / 2;
+  const uint32_t sh = s >> bl, sl = s & ((1u << bl) - 1u);
+  const int f0 = lev_off(lo, r), f1 = lev_off(hi, r) + (r ? (hi ? 1 << (hi - 1) : 1) : (1 << hi));
+  for (int f = f0 + threadIdx.x; f < f1; f += blockDim.x) {
+    const int l = r ? (f ? 32 - __clz(f) : 0) : 31 - __clz(f);
+    const uint32_t e = (uint32_t)(f - lev_off(l, r));
+    uint32_t z = e;
+    if (r && l) z = top ? ((bit << (l - 1)) | e) : ((e << 1) | bit);
+    R[f] = (uint16_t)count_halves(Th, a, sh, Tl, bl, sl, z, l);
   }
 }
 
-__global__ void __launch_bounds__(1024) k_entropy_reps(int n, int k, int n1, int n2, const uint64_t *__restrict__ mask,
-                                                      const uint32_t *__restrict__ wordbase, QuarterTabs qt,
-                                                      int qbytes, const double *__restrict__ logtab, uint32_t x1base,
-                                                      double *__restrict__ Hrep) {
-  extern __shared__ __attribute__((aligned(16))) uint8_t lds8[];
-  __shared__ double red[16];
+__global__ void __launch_bounds__(1024) k_entropy_reps(EntropyArgs ea, const uint64_t *__restrict__ mask,
+                                                      const uint32_t *__restrict__ wordbase,
+                                                      const double *__restrict__ logtab,
+                                                      const uint8_t *__restrict__ htab, double *__restrict__ Hrep) {
+  extern __shared__ __attribute__((aligned(16))) uint16_t lds16[];
+  __shared__ double hw[2][16];  // per-wave sums of the last two representatives
   __shared__ double lt[130];
-  // layout: quarter tables (qbytes, rounded to 16), R(x1), R(x2)
-  uint16_t *A = reinterpret_cast<uint16_t *>(lds8 + ((qbytes + 15) & ~15));
-  uint16_t *B = A + ((1 << n1) - 1);
+  uint16_t *A = lds16, *B = lds16 + ea.aWords;
+  const int n = ea.n, k = ea.k, n1 = ea.n1, n2 = ea.n2, rt = ea.rt, rb = ea.rb;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
   for (int i = threadIdx.x; i < 130; i += blockDim.x) lt[i] = logtab[i];
-  // quarter tables
-  for (int l = 0; l <= 7; ++l) {
-    if (qt.off[l] < 0) continue;
-    uint8_t *T = lds8 + qt.off[l];
-    const int per = (2 << l) - 1, tot = (1 << l) * per;
-    for (int e = threadIdx.x; e < tot; e += blockDim.x) {
-      const int sv = e / per, r = e % per, d = 31 - __clz(r + 1);
-      T[e] = (uint8_t)count_small((uint32_t)sv, l, (uint32_t)(r + 1 - (1 << d)), d);
-    }
-  }
-  __syncthreads();
-  const uint32_t x1 = x1base + blockIdx.x;
-  build_rowset(x1, n1, lds8, qt, A);
-  const int jlo = k - n2 > 0 ? k - n2 : 0, jhi = k < n1 ? k : n1;
-  const uint32_t N2 = 1u << n2, ny = 1u << k;
+  const uint32_t x1 = ea.x1base + blockIdx.x;
+  const int a1 = (n1 + 1) / 2, b1 = n1 / 2, a2 = (n2 + 1) / 2, b2 = n2 / 2;
+  build_levels(half_row(htab, ea, a1, x1 >> b1), half_row(htab, ea, b1, x1 & ((1u << b1) - 1u)), x1, n1, ea.aLo,
+               ea.aHi, rt, 1, (uint32_t)ea.t, A);
   const double *lnF = lt, *invF = lt + 65;
+  const bool idA = k >= n1, idB = k >= n2;  // identity factors: j = n1 (y_hi = x1), k - j = n2 (y_lo = x2)
+  const int j0 = ea.jlo + (idB ? 1 : 0), j1 = ea.jhi - (idA ? 1 : 0);
+  const int jb1 = min(j1, k - rb - 2);  // bulk splits: A entry shared by the 4 outputs, B entries consecutive
+  const int je0 = max(j0, k - rb - 1);  // edge splits: m = k - j < rb + 2
+  const uint32_t N2 = 1u << n2, ky = 1u << k, m2 = N2 - 1u;
+  const uint32_t ybase = (rt ? ((uint32_t)ea.t << (k - 1)) : 0u) | (rb ? (uint32_t)ea.b : 0u);
+  // The sum of a representative is reduced per wave into hw[parity] and
+  // combined by thread 0 after the next barrier (one representative later),
+  // so a representative costs two barriers.
+  uint32_t prev = 0xffffffffu;
+  int par = 0;
+  auto flush = [&](int p) {
+    if (threadIdx.x == 0 && prev != 0xffffffffu) {
+      double h = 0.0;
+      for (int w = 0; w < nw; ++w) h += hw[p][w];
+      const uint32_t r = rank_of_rep(prev, mask, wordbase);
+      Hrep[r] = ea.accumulate ? Hrep[r] + h : h;
+    }
+  };
   for (uint32_t x2 = 0; x2 < N2; ++x2) {
     const uint32_t x = (x1 << n2) | x2;
     if (rep_of(x, n) != x) continue;  // uniform across the block
-    __syncthreads();                  // R(x1) built / previous sweep done with R(x2)
-    build_rowset(x2, n2, lds8, qt, B);
+    __syncthreads();                  // A built / previous sweep done with B; hw[par ^ 1] complete
+    flush(par ^ 1);
+    build_levels(half_row(htab, ea, a2, x2 >> b2), half_row(htab, ea, b2, x2 & ((1u << b2) - 1u)), x2, n2, ea.bLo,
+                 ea.bHi, rb, 0, (uint32_t)ea.b, B);
     __syncthreads();
-    // splits with an identity factor (j = n1: y_hi must equal x1; k - j = n2:
-    // y_lo must equal x2) are handled outside the main loop
-    const int j0 = (k - jlo == n2) ? jlo + 1 : jlo, j1 = (jhi == n1) ? jhi - 1 : jhi;
     double h = 0.0;
-    for (uint32_t y = threadIdx.x; y < ny; y += blockDim.x) {
-      uint32_t c = 0;
-      for (int j = j0; j <= j1; ++j) {
-        const int kj = k - j;
-        // counts < 2^16: 24-bit products are exact (and full rate)
-        c += __umul24(A[(1 << j) - 1 + (y >> kj)], B[(1 << kj) - 1 + (y & ((1u << kj) - 1u))]);
+    for (int g = threadIdx.x; g < ea.groups; g += blockDim.x) {
+      const uint32_t y0 = ybase | ((uint32_t)g << (2 + rb));
+      uint32_t c0 = 0, c1 = 0, c2 = 0, c3 = 0;
+      for (int j = j0; j <= jb1; ++j) {
+        const int m = k - j;
+        const uint32_t a = A[lev_off(j, rt) + ((y0 >> m) & lev_mask_top(j, rt))];
+        const uint2 bv = *reinterpret_cast<const uint2 *>(B + lev_off(m, rb) + ((y0 & ((1u << m) - 1u)) >> rb));
+        c0 += __umul24(a, bv.x & 0xffffu);
+        c1 += __umul24(a, bv.x >> 16);
+        c2 += __umul24(a, bv.y & 0xffffu);
+        c3 += __umul24(a, bv.y >> 16);
       }
-      if (jhi == n1 && (y >> (k - n1)) == x1 && k - n1 != n2) c += B[(1 << (k - n1)) - 1 + (y & ((1u << (k - n1)) - 1u))];
-      if (k - jlo == n2 && (y & ((1u << n2) - 1u)) == x2 && jlo != n1) c += A[(1 << jlo) - 1 + (y >> n2)];
-      if (jhi == n1 && k - n1 == n2 && y == ((x1 << n2) | x2)) c += 1u;  // k = n: both factors are identities
-      if (c > 1u) h = fma((double)c, log_tang_int(c, lnF, invF), h);
+      uint32_t cc[4] = {c0, c1, c2, c3};
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const uint32_t y = y0 | ((uint32_t)i << rb);
+        if (y >= ky) continue;
+        uint32_t c = cc[i];
+        for (int j = je0; j <= j1; ++j) {
+          const int m = k - j;
+          c += __umul24(A[lev_off(j, rt) + ((y >> m) & lev_mask_top(j, rt))],
+                        B[lev_off(m, rb) + ((y & ((1u << m) - 1u)) >> rb)]);
+        }
+        if (idA && (y >> (k - n1)) == x1) {
+          const int m = k - n1;
+          c += B[lev_off(m, rb) + ((y & ((1u << m) - 1u)) >> rb)];
+        }
+        if (idB && (y & m2) == x2) c += A[lev_off(ea.jlo, rt) + ((y >> n2) & lev_mask_top(ea.jlo, rt))];
+        if (c > 1u) h = fma((double)c, log_tang_int(c, lnF, invF), h);
+      }
     }
-    h = block_sum(h, red);
-    if (threadIdx.x == 0) Hrep[rank_of_rep(x, mask, wordbase)] = h;
+    h = wave_sum(h);
+    if (lane == 0) hw[par][wave] = h;
+    prev = x;
+    par ^= 1;
   }
+  __syncthreads();
+  flush(par ^ 1);
 }
 int launch_entropy_reps(int n, int k, int n1, int n2, const uint64_t *mask, const uint32_t *wordbase,
                         const double *logtab, double *Hrep, hipStream_t s) {
-  if (n1 > 14 || n2 > 14) return BDC_ERR_ARG;
-  QuarterTabs qt;
-  for (int l = 0; l < 8; ++l) qt.off[l] = -1;
-  int bytes = 0;
+  if (n1 > 16 || n2 > 16 || k < 1 || k >= n) return BDC_ERR_ARG;
+  EntropyArgs ea{};
+  ea.n = n;
+  ea.k = k;
+  ea.n1 = n1;
+  ea.n2 = n2;
+  ea.jlo = std::max(0, k - n2);
+  ea.jhi = std::min(k, n1);
+  ea.aLo = ea.jlo;
+  ea.aHi = std::min(ea.jhi, n1 - 1);
+  ea.bLo = k - ea.jhi;
+  ea.bHi = std::min(k - ea.jlo, n2 - 1);
+  auto words = [](int hi, int r) {
+    const int w = r ? (hi ? 1 << hi : 1) : (2 << hi);
+    return (w + 3) & ~3;
+  };
+  // restrict to output quadrants only when the full row sets do not fit
+  const size_t kLdsCap = 136 * 1024;
+  int r = 0;
+  if (2 * (size_t)(words(ea.aHi, 0) + words(ea.bHi, 0)) > kLdsCap) r = 1;
+  const char *fq = getenv("BDC_ENTROPY_QUADRANTS");  // tests: force the quadrant layout (k >= 4)
+  if (fq && fq[0] == '1' && k >= 4) r = 1;
+  if (r && k < 4) return BDC_ERR_ARG;
+  ea.rt = ea.rb = r;
+  ea.aWords = words(ea.aHi, r);
+  ea.bWords = words(ea.bHi, r);
+  const size_t lds = 2 * (size_t)(ea.aWords + ea.bWords);
+  if (lds > kLdsCap) return BDC_ERR_ARG;
+  // half tables of the lengths the row sets need (built once, read by every block)
+  for (int l = 0; l < 9; ++l) ea.htoff[l] = -1;
+  size_t tbytes = 0;
   for (int L : {n1, n2})
     for (int l : {(L + 1) / 2, L / 2})
-      if (qt.off[l] < 0) {
-        qt.off[l] = bytes;
-        bytes += (1 << l) * ((2 << l) - 1);
+      if (ea.htoff[l] < 0) {
+        ea.htoff[l] = (int)tbytes;
+        tbytes += ((size_t)(1u << l) * ((2u << l) - 1u) + 15) & ~(size_t)15;
       }
-  const size_t lds = ((bytes + 15) & ~15) + sizeof(uint16_t) * (((1u << n1) - 1) + ((1u << n2) - 1));
-  if (lds > 150 * 1024) return BDC_ERR_ARG;
-  // in chunks of x1 rows, so a long precompute (large k) can report progress
+  uint8_t *htab = nullptr;
+  if (hipMalloc(&htab, tbytes) != hipSuccess) return BDC_ERR_MEMORY;
+  for (int l = 0; l < 9; ++l)
+    if (ea.htoff[l] >= 0) {
+      const int tot = (1 << l) * ((2 << l) - 1);
+      hipLaunchKernelGGL(k_half_tables, dim3((tot + 255) / 256), dim3(256), 0, s, l, htab + ea.htoff[l]);
+    }
+  const int free_bits = k - 2 * r - 2;
+  ea.groups = free_bits > 0 ? 1 << free_bits : 1;
+  int threads = 64;
+  while (threads < 1024 && threads < ea.groups) threads *= 2;
+  // representatives have top bit 0 (x <= ~x), so only x1 < N1/2 holds any
   const char *v = getenv("BDC_VERBOSE");
   const bool verbose = v && v[0] == '1';
-  const uint32_t N1 = 1u << n1, chunk = N1 >= 16 ? N1 / 16 : N1;
-  for (uint32_t b = 0; b < N1; b += chunk) {
-    hipLaunchKernelGGL(k_entropy_reps, dim3(chunk), dim3(1024), lds, s, n, k, n1, n2, mask, wordbase, qt, bytes,
-                       logtab, b, Hrep);
-    if (hipGetLastError() != hipSuccess) return BDC_ERR_HIP;
-    if (verbose) {
-      if (hipStreamSynchronize(s) != hipSuccess) return BDC_ERR_HIP;
-      fprintf(stderr, "[bdc] entropy term BDC_(%d,%d): x1 rows %u/%u done\n", n, k, b + chunk, N1);
+  const uint32_t X1 = 1u << (n1 - 1), chunk = X1 >= 16 ? X1 / 16 : X1;
+  const int quads = r ? 4 : 1;
+  for (int q = 0; q < quads; ++q) {
+    ea.t = q >> 1;
+    ea.b = q & 1;
+    ea.accumulate = q > 0;
+    for (uint32_t b = 0; b < X1; b += chunk) {
+      ea.x1base = b;
+      hipLaunchKernelGGL(k_entropy_reps, dim3(chunk), dim3(threads), lds, s, ea, mask, wordbase, logtab, htab, Hrep);
+      if (hipGetLastError() != hipSuccess) {
+        (void)hipFree(htab);
+        return BDC_ERR_HIP;
+      }
+      if (verbose) {
+        if (hipStreamSynchronize(s) != hipSuccess) {
+          (void)hipFree(htab);
+          return BDC_ERR_HIP;
+        }
+        fprintf(stderr, "[bdc] entropy term BDC_(%d,%d): quadrant %d/%d, x1 rows %u/%u done\n", n, k, q + 1, quads,
+                b + chunk, X1);
+      }
     }
   }
-  return BDC_OK;
+  const bool ok = hipStreamSynchronize(s) == hipSuccess;  // the tables are in use until the pass ends
+  (void)hipFree(htab);
+  return ok ? BDC_OK : BDC_ERR_HIP;
 }
 
 }  // namespace bdc

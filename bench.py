@@ -8,6 +8,7 @@ are certified in FP64 afterwards; the line reports the split-FP16 rate as
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 28 --k 14]
        [--precision both|fp64|split] [--no-cpu-baseline]
+       python bench.py --sweep | --dual | --scale | --published   (the other configurations)
 """
 import argparse
 import json
@@ -184,6 +185,120 @@ def run_sweep(bdc, n, ks, max_iters, tol, alpha, seed, cpu_seconds):
     return out
 
 
+def run_scale(bdc, n, k, alpha, seed, tol, fp64_steps, split_steps):
+    """Scale config: BDC_(n,k) (default (32,16): 2^32 inputs, p and the entropy
+    term 32 GiB each in HBM), p from Dirichlet(alpha) over the orbits with
+    `seed`.  A fixed number of steps at each precision from the same p (the
+    reference's tolerance rule is reported, not waited for), then the FP64
+    certificate of both.  Per kernel: the output-law (g1) and cross-term (g2)
+    contractions as FP64-equivalent TFLOP/s against the matrix peak of their
+    precision, and the cross-term kernel's p/H stream (24 B per input of the
+    step) as GB/s against HBM."""
+    os.environ.setdefault("BDC_VERBOSE", "1")  # progress of the entropy precompute (minutes at n = 32)
+    t = time.perf_counter()
+    e = bdc.Engine(n, k)
+    setup_s = time.perf_counter() - t
+    log(f"engine BDC_({n},{k}) ready in {setup_s:.1f} s")
+    out = {"metric": f"BAA iter/s at BDC_({n},{k}), Dirichlet({alpha}) seed {seed}", "n": n, "k": k, "alpha": alpha,
+           "seed": seed, "tol": tol, "setup_s": setup_s, "num_orbits": e.num_orbits}
+    for name, prec, steps in (("fp64", bdc.PREC_FP64, fp64_steps), ("split", bdc.PREC_SPLIT16, split_steps)):
+        e.set_p_dirichlet(alpha, seed)
+        e.iterate(1, prec)  # warm-up step (first split16 use prepares the limb operands)
+        e.set_p_dirichlet(alpha, seed)
+        e.sync()
+        e.timers(True)
+        t = time.perf_counter()
+        st = e.iterate(steps, prec, tol=tol)
+        e.sync()
+        secs = time.perf_counter() - t
+        g1_ms, _ = e.kernel_time("g1")
+        g2_ms, _ = e.kernel_time("g2")
+        e.timers(False)
+        g1f, g2f, g2bytes = e.work_model(prec)
+        peak = PEAK_FP16 if prec == bdc.PREC_SPLIT16 else PEAK_FP64
+        c = e.certify()
+        rec = {"steps": len(st), "secs": secs, "iter_s": len(st) / secs, "mx_bits": [float(v) for v in st[:, 0]],
+               "converged": bool(st[-1, 0] < tol), "lb_bits": c["lower"] / LN2, "ub_ref_bits": c["upper_ref"] / LN2,
+               "ub_cert_bits": c["upper_cert"] / LN2, "sum_p": c["sum_p"],
+               "g1": {"ms": g1_ms, "tflops_fp64_equiv": g1f / (g1_ms * 1e-3) / 1e12},
+               "g2": {"ms": g2_ms, "tflops_fp64_equiv": g2f / (g2_ms * 1e-3) / 1e12,
+                      "p_stream_gb_s": g2bytes / (g2_ms * 1e-3) / 1e9,
+                      "p_stream_frac_hbm": g2bytes / (g2_ms * 1e-3) / PEAK_HBM}}
+        if prec == bdc.PREC_SPLIT16:
+            cfg = e.split_config()
+            rec["split_config"] = cfg
+            rec["g1"]["mfma_frac"] = g1f * cfg["g1_limbs"] / (g1_ms * 1e-3) / peak
+            rec["g2"]["mfma_frac"] = g2f * cfg["g2_limbs"] / (g2_ms * 1e-3) / peak
+        else:
+            rec["g1"]["mfma_frac"] = g1f / (g1_ms * 1e-3) / peak
+            rec["g2"]["mfma_frac"] = g2f / (g2_ms * 1e-3) / peak
+        log(f"{name}: {len(st)} steps, {rec['iter_s']:.3f} it/s, g1 {g1_ms:.2f} ms, g2 {g2_ms:.2f} ms "
+            f"({rec['g2']['p_stream_gb_s']:.0f} GB/s p stream), UB_cert {rec['ub_cert_bits']:.10f} bits")
+        out[name] = {**rec, "stats": st.tolist()}
+    a, b = out["fp64"], out["split"]
+    t = min(a["steps"], b["steps"])
+    s64, s16 = np.array(a["stats"])[:t], np.array(b["stats"])[:t]
+    out["per_iteration_max_err_bits"] = {"mx": float(np.abs(s16[:, 0] - s64[:, 0]).max()),
+                                         "lb": float(np.abs(s16[:, 3] - s64[:, 3]).max() / LN2)}
+    if a["steps"] == b["steps"]:
+        out["ub_cert_gap_bits"] = abs(a["ub_cert_bits"] - b["ub_cert_bits"])
+        out["lb_gap_bits"] = abs(a["lb_bits"] - b["lb_bits"])
+    out["speedup_split_vs_fp64"] = b["iter_s"] / a["iter_s"]
+    for r in (a, b):
+        del r["stats"]
+    e.close()
+    return out
+
+
+# Upper bounds on C_{n,k} (bits, 4 d.p.) published with the reference
+# (Upper_Bounds.txt:1-72), computed at convergence threshold a = 0.005 except
+# C_{31,k} for k = 14..18 at a = 0.05 (README.md:220).
+PUBLISHED_UB = {
+    29: {2: 1.1898, 3: 1.5165, 4: 1.8137, 5: 2.0916, 6: 2.3761, 7: 2.6647, 8: 2.9598, 9: 3.2663, 10: 3.5867,
+         11: 3.9247, 12: 4.2841, 13: 4.6727, 14: 5.0930, 15: 5.5573, 16: 6.0713, 17: 6.6464, 18: 7.2964,
+         19: 8.0364, 20: 8.8850},
+    31: {2: 1.1888, 3: 1.5136, 4: 1.8086, 5: 2.0831, 6: 2.3632, 7: 2.6458, 8: 2.9332, 9: 3.2298, 10: 3.5384,
+         11: 3.8605, 12: 4.2005, 13: 4.5623, 14: 4.9511, 15: 5.3895, 16: 5.8826, 17: 6.3947, 18: 6.9592},
+}
+
+
+def published_tol(n, k):
+    return 0.05 if (n == 31 and 14 <= k <= 18) else 0.005
+
+
+def run_published(bdc, cases, max_iters, precision):
+    """The reference's published cases: BDC_(n,k) from its uniform p, BA until
+    mx < a (its stopping rule, main_BAA.cu:73) at the published threshold a,
+    then the FP64 certificate.  UB_ref = I(p) + mx is the bound the reference
+    prints (main_BAA.cu:123-130); UB_cert = max_x D(x||q) is certified."""
+    out = {"metric": "published upper bounds C_{n,k} (Upper_Bounds.txt) reproduced from the uniform p",
+           "precision": "split16" if precision == bdc.PREC_SPLIT16 else "fp64", "cases": {}}
+    for n, k in cases:
+        tol = published_tol(n, k)
+        t = time.perf_counter()
+        e = bdc.Engine(n, k)
+        setup = time.perf_counter() - t
+        e.set_p_uniform()
+        e.sync()
+        t = time.perf_counter()
+        st = e.iterate(max_iters, precision, tol=tol)
+        e.sync()
+        secs = time.perf_counter() - t
+        c = e.certify()
+        e.close()
+        pub = PUBLISHED_UB.get(n, {}).get(k)
+        rec = {"a": tol, "iters": len(st), "converged": bool(st[-1, 0] < tol), "setup_s": setup, "secs": secs,
+               "iter_s": len(st) / secs, "mx_bits": float(st[-1, 0]), "lb_bits": c["lower"] / LN2,
+               "ub_ref_bits": c["upper_ref"] / LN2, "ub_cert_bits": c["upper_cert"] / LN2, "published_ub": pub}
+        if pub is not None:
+            rec["ub_ref_minus_published"] = rec["ub_ref_bits"] - pub
+            rec["lb_below_published"] = bool(rec["lb_bits"] <= pub + 5e-5)
+        log(f"BDC_({n},{k}) a={tol}: {len(st)} steps ({secs:.2f} s, setup {setup:.1f} s) LB {rec['lb_bits']:.6f} "
+            f"UB_ref {rec['ub_ref_bits']:.6f} UB_cert {rec['ub_cert_bits']:.6f} published {pub}")
+        out["cases"][f"{n}_{k}"] = rec
+    return out
+
+
 def cpu_baseline(n, k, seconds):
     """The oracle's reference-method passes on a sample of orbits, host cores."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
@@ -233,6 +348,13 @@ def main():
     ap.add_argument("--sweep-ks", default="8,10,12,14,16,18,20")
     ap.add_argument("--sweep-max-iters", type=int, default=2000)
     ap.add_argument("--tol", type=float, default=1e-8)
+    ap.add_argument("--scale", action="store_true",
+                    help="scale config instead: BDC_(32,16), Dirichlet(0.5) seed 1, tol 1e-7, fixed step counts")
+    ap.add_argument("--published", action="store_true",
+                    help="the reference's published odd-n cases (Upper_Bounds.txt): n=29 k=2..16, n=31 k=2..14")
+    ap.add_argument("--published-cases", default="29:2-16,31:2-14")
+    ap.add_argument("--scale-fp64-steps", type=int, default=6)
+    ap.add_argument("--scale-split-steps", type=int, default=6)
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
@@ -242,6 +364,22 @@ def main():
         ks = [int(v) for v in args.sweep_ks.split(",")]
         cs = 0.0 if args.no_cpu_baseline else min(args.cpu_seconds, 5.0)
         print(json.dumps(run_sweep(bdc, args.sweep_n, ks, args.sweep_max_iters, args.tol, 1.0, 0, cs)), flush=True)
+        return
+    if args.published:
+        import bdc_gpu as bdc
+        cases = []
+        for part in args.published_cases.split(","):
+            nn, ks = part.split(":")
+            lo, hi = (int(v) for v in ks.split("-"))
+            cases += [(int(nn), kk) for kk in range(lo, hi + 1)]
+        prec = bdc.PREC_FP64 if args.precision == "fp64" else bdc.PREC_SPLIT16
+        print(json.dumps(run_published(bdc, cases, 20000, prec)), flush=True)
+        return
+    if args.scale:
+        import bdc_gpu as bdc
+        n, k = (args.n, args.k) if (args.n, args.k) != (28, 14) else (32, 16)
+        print(json.dumps(run_scale(bdc, n, k, 0.5, 1, 1e-7, args.scale_fp64_steps, args.scale_split_steps)),
+              flush=True)
         return
     if args.dual:
         import bdc_gpu as bdc

@@ -112,6 +112,14 @@ int bdc_kl(bdc_ctx *ctx, int precision, const double *log_q, double *d);
 /* H(x)/C(n,k) - ln C(n,k) = sum_y W ln W for all x (2^n) — the hoisted entropy term
  * (the per-pair W ln W of log_W_term, BAA.cu:401-411) */
 int bdc_entropy_term(bdc_ctx *ctx, double *h);
+/* Values at selected inputs x[0..len) (len <= 2^24), for checks where the
+ * full 2^n arrays are too large to copy (n up to 32): what = BDC_SAMPLE_ENTROPY
+ * gives H(x)/C(n,k) - ln C(n,k) (as bdc_entropy_term), BDC_SAMPLE_P gives p(x)
+ * (per string), BDC_SAMPLE_KL gives D(x||q) from the last bdc_certify.  No
+ * reference counterpart (the reference only ever copies whole arrays,
+ * BAA.cu:642-724). */
+enum { BDC_SAMPLE_ENTROPY = 0, BDC_SAMPLE_P = 1, BDC_SAMPLE_KL = 2 };
+int bdc_sample_inputs(bdc_ctx *ctx, int what, const uint32_t *x, uint64_t len, double *out);
 
 /* ---- smoothed dual bound and its adjoint (no reference counterpart) ---- */
 /* U_beta(theta) = beta^-1 log sum_x exp(beta D(x||softmax theta)) and, when grad

@@ -474,3 +474,31 @@ double or_full_dual(int n, int k, const double *theta, double beta, double *grad
 }
 
 int or_num_threads(void) { return omp_get_max_threads(); }
+
+/* ------------------------------------------------------- per-input sums */
+
+/* sum_y c(x,y) ln c(x,y) in nats over all 2^k outputs: the W ln W part of
+ * log_W_term (backend/BAA.cu:401-411) times C(n,k), for a single input x
+ * (n <= 32).  Terms with c <= 1 contribute 0. */
+double or_entropy_x(int n, int k, uint32_t x) {
+  const uint64_t ny = 1ull << k;
+  double h = 0.0;
+  for (uint64_t y = 0; y < ny; ++y) {
+    const uint32_t c = or_count(x, n, (uint32_t)y, k);
+    if (c > 1) h += (double)c * log((double)c);
+  }
+  return h;
+}
+
+/* D(x||q) = sum_y W(y|x) (ln W(y|x) - ln q(y)) in nats for a single input x,
+ * W = c / C(n,k) (compute_KL_kernel / KL_term, backend/BAA.cu:419-425, 533-571) */
+double or_kl_x(int n, int k, uint32_t x, const double *logq) {
+  const uint64_t ny = 1ull << k;
+  const double C = (double)or_nck((uint32_t)n, (uint32_t)k), lnC = log(C);
+  double d = 0.0;
+  for (uint64_t y = 0; y < ny; ++y) {
+    const uint32_t c = or_count(x, n, (uint32_t)y, k);
+    if (c) d += (c / C) * (log((double)c) - lnC - logq[y]);
+  }
+  return d;
+}

@@ -65,6 +65,10 @@ void or_full_kl(int n, int k, const double *logq, double *D);
  * writes dU/dtheta (2^k) when grad != NULL.  Returns U. */
 double or_full_dual(int n, int k, const double *theta, double beta, double *grad);
 
+/* single-input sums (any n <= 32): sum_y c ln c, and D(x||q) given ln q (2^k) */
+double or_entropy_x(int n, int k, uint32_t x);
+double or_kl_x(int n, int k, uint32_t x, const double *logq);
+
 int or_num_threads(void);
 
 #ifdef __cplusplus

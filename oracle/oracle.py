@@ -49,6 +49,8 @@ def lib():
         L.or_full_kl.argtypes, L.or_full_kl.restype = [i32, i32, vp, vp], None
         L.or_full_dual.argtypes, L.or_full_dual.restype = [i32, i32, vp, dbl, vp], dbl
         L.or_num_threads.argtypes, L.or_num_threads.restype = [], i32
+        L.or_entropy_x.argtypes, L.or_entropy_x.restype = [i32, i32, u32], dbl
+        L.or_kl_x.argtypes, L.or_kl_x.restype = [i32, i32, u32, vp], dbl
         _lib = L
     return _lib
 
@@ -105,6 +107,17 @@ def full_dual(n, k, theta, beta, want_grad=True):
     g = np.empty(1 << k) if want_grad else None
     U = lib().or_full_dual(n, k, _ptr(theta), float(beta), _ptr(g) if want_grad else None)
     return (U, g) if want_grad else U
+
+
+def entropy_x(n, k, x):
+    """sum_y c(x,y) ln c(x,y) (nats) for one input x, any n <= 32"""
+    return lib().or_entropy_x(n, k, int(x))
+
+
+def kl_x(n, k, x, logq):
+    """D(x||q) (nats) for one input x given ln q over all 2^k outputs"""
+    logq = np.ascontiguousarray(logq, dtype=np.float64)
+    return lib().or_kl_x(n, k, int(x), _ptr(logq))
 
 
 class BA:

@@ -35,7 +35,10 @@ def test_supported_validation_without_gpu():
     assert L.bdc_supported(16, 8) == 1
     assert L.bdc_supported(16, 16) == 0
     assert L.bdc_supported(16, 0) == 0
-    # the scale config BDC_(32,16) is refused with its reason
-    assert L.bdc_supported(32, 16) == 0
-    assert b"n1 <= 14" in L.bdc_last_error()
+    # inputs up to 2^32 (the scale config BDC_(32,16)), outputs up to 2^20
+    assert L.bdc_supported(32, 16) == 1
+    assert L.bdc_supported(29, 14) == 1
+    assert L.bdc_supported(33, 16) == 0
+    assert b"[2, 32]" in L.bdc_last_error()
+    assert L.bdc_supported(32, 21) == 0
     assert L.bdc_version() >= 1

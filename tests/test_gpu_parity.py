@@ -77,6 +77,20 @@ def test_entropy_term(bdc, n, k):
         np.testing.assert_allclose(e.entropy_term(), ref, rtol=0, atol=1e-12)
 
 
+@pytest.mark.parametrize("n,k", [(12, 7), (11, 6), (13, 4)])
+def test_entropy_term_quadrant_layout(bdc, monkeypatch, n, k):
+    """The entropy kernel's quadrant layout (row sets restricted to one
+    output quadrant, used when they do not fit LDS at half length 16) forced
+    at small n, against the dense channel."""
+    say(f"entropy quadrants n={n} k={k}")
+    monkeypatch.setenv("BDC_ENTROPY_QUADRANTS", "1")
+    W = dense_ba.channel(n, k)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        ref = np.where(W > 0, W * np.log(np.where(W > 0, W, 1.0)), 0.0).sum(axis=1)
+    with bdc.Engine(n, k) as e:
+        np.testing.assert_allclose(e.entropy_term(), ref, rtol=0, atol=1e-12)
+
+
 # (18, 9) reaches the count-streaming stage kernel (2^6-column count factor)
 @pytest.mark.parametrize("n,k", [(8, 4), (10, 5), (11, 6), (12, 7), (12, 3), (18, 9)])
 def test_output_law_general_weights(bdc, oracle, n, k):

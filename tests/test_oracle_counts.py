@@ -78,3 +78,44 @@ def test_cache_invariants(oracle, n, k):
 def test_cache_k0_all_ones(oracle):
     for n in range(1, 9):
         assert (oracle.cache(n, 0) == 1).all()
+
+
+def test_entropy_x_matches_dense_channel(oracle):
+    # sum_y W ln W over the dense channel equals H(x)/C - ln C
+    for n, k in [(10, 5), (9, 3), (8, 7)]:
+        W = dense_ba.channel(n, k)
+        C = math.comb(n, k)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            ref = np.where(W > 0, W * np.log(np.where(W > 0, W, 1.0)), 0.0).sum(axis=1)
+        got = np.array([oracle.entropy_x(n, k, x) / C - math.log(C) for x in range(1 << n)])
+        np.testing.assert_allclose(got, ref, rtol=0, atol=1e-13)
+
+
+def test_entropy_x_closed_form_runs(oracle):
+    # x = 0^a 1^b: the length-k subsequences are 0^i 1^(k-i), each C(a,i) C(b,k-i)
+    # times, so H = sum_i c_i ln c_i; checked at n = 32 (a = b = 16, k = 16)
+    for a, b, k in [(16, 16, 16), (15, 14, 14), (3, 5, 4)]:
+        n = a + b
+        x = (1 << b) - 1
+        ref = 0.0
+        for i in range(k + 1):
+            c = math.comb(a, i) * math.comb(b, k - i)
+            if c > 1:
+                ref += c * math.log(c)
+        assert abs(oracle.entropy_x(n, k, x) - ref) <= 1e-12 * abs(ref)
+
+
+def test_kl_x_matches_full_kl(oracle):
+    n, k = 10, 4
+    rng = np.random.default_rng(3)
+    q = rng.random(1 << k) + 0.1
+    logq = np.log(q / q.sum())
+    D = oracle.full_kl(n, k, logq)
+    for x in [0, 1, 77, 512, 1023]:
+        assert abs(oracle.kl_x(n, k, x, logq) - D[x]) <= 1e-13
+    # uniform q: D(x||q) = H(x)/C - ln C + k ln 2
+    C = math.comb(n, k)
+    u = np.full(1 << k, -k * math.log(2.0))
+    for x in [5, 300]:
+        ref = oracle.entropy_x(n, k, x) / C - math.log(C) + k * math.log(2.0)
+        assert abs(oracle.kl_x(n, k, x, u) - ref) <= 1e-13
