@@ -415,6 +415,114 @@ __device__ inline void build_levels(const uint8_t *Th, const uint8_t *Tl, uint32
   }
 }
 
+typedef unsigned short ushort2_ __attribute__((ext_vector_type(2)));
+__device__ inline uint32_t half_of(const uint32_t *w, int i) { return (w[i >> 1] >> (16 * (i & 1))) & 0xffffu; }
+
+// Counts of the 16 outputs y_e = ys | (e << R), e < 16 (ys has bits R..R+3
+// clear) and their sum of c ln c.  R = 1: quadrant layout (A holds the entries
+// whose top bit is the quadrant's, B those whose bottom bit is).
+//  - splits with m = k - j >= R + 4: the A entry is shared by the 16 outputs
+//    and their B entries are 16 consecutive u16 (two 16-byte LDS loads); two
+//    splits go through one v_dot2_u32_u16 per output (A pair . B pair);
+//  - splits with m < R + 4 (unrolled): the outputs' A entries are consecutive
+//    (or every other one for m < R), B level m has at most 8 entries;
+//  - the identity factors (y_hi = x1 or y_lo = x2) are compares.
+template <int R>
+__device__ inline double sweep16(const EntropyArgs &ea, const uint16_t *__restrict__ A, const uint16_t *__restrict__ B,
+                                 uint32_t ys, uint32_t x1, uint32_t x2, const double *lnF, const double *invF) {
+  const int k = ea.k, n1 = ea.n1, n2 = ea.n2;
+  const bool idA = k >= n1, idB = k >= n2;
+  const int j0 = ea.jlo + (idB ? 1 : 0), j1 = ea.jhi - (idA ? 1 : 0);
+  uint32_t c[16];
+#pragma unroll
+  for (int e = 0; e < 16; ++e) c[e] = 0u;
+  const int jb = min(j1, k - R - 4);
+  int j = j0;
+  for (; j + 1 <= jb; j += 2) {
+    const int m = k - j;
+    const uint32_t a0 = A[lev_off(j, R) + ((ys >> m) & lev_mask_top(j, R))];
+    const uint32_t a1 = A[lev_off(j + 1, R) + ((ys >> (m - 1)) & lev_mask_top(j + 1, R))];
+    const ushort2_ ap = __builtin_bit_cast(ushort2_, a0 | (a1 << 16));
+    const uint4 *p0 = reinterpret_cast<const uint4 *>(B + lev_off(m, R) + ((ys & ((1u << m) - 1u)) >> R));
+    const uint4 *p1 = reinterpret_cast<const uint4 *>(B + lev_off(m - 1, R) + ((ys & ((1u << (m - 1)) - 1u)) >> R));
+    const uint4 u0 = p0[0], u1 = p0[1], v0 = p1[0], v1 = p1[1];
+    const uint32_t w0[8] = {u0.x, u0.y, u0.z, u0.w, u1.x, u1.y, u1.z, u1.w};
+    const uint32_t w1[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+      // low half: B_m(y_e) (split j), high half: B_{m-1}(y_e) (split j + 1)
+      const uint32_t pk = __builtin_amdgcn_perm(w1[e >> 1], w0[e >> 1], (e & 1) ? 0x07060302u : 0x05040100u);
+      c[e] = __builtin_amdgcn_udot2(ap, __builtin_bit_cast(ushort2_, pk), c[e], false);
+    }
+  }
+  if (j <= jb) {
+    const int m = k - j;
+    const uint32_t a = A[lev_off(j, R) + ((ys >> m) & lev_mask_top(j, R))];
+    const uint4 *p0 = reinterpret_cast<const uint4 *>(B + lev_off(m, R) + ((ys & ((1u << m) - 1u)) >> R));
+    const uint4 u0 = p0[0], u1 = p0[1];
+    const uint32_t w0[8] = {u0.x, u0.y, u0.z, u0.w, u1.x, u1.y, u1.z, u1.w};
+#pragma unroll
+    for (int e = 0; e < 16; ++e) c[e] += __umul24(a, half_of(w0, e));
+  }
+#pragma unroll
+  for (int mm = 0; mm < R + 4; ++mm) {
+    const int jj = k - mm;
+    if (jj < j0 || jj > j1 || jj <= jb) continue;
+    const int abase = lev_off(jj, R) + (int)((ys >> mm) & lev_mask_top(jj, R));
+    uint32_t aw[8], bw[4];
+    // A entries of the 16 outputs: abase + ((e << R) >> mm)
+    if (mm < R) {
+#pragma unroll
+      for (int i = 0; i < 8; ++i) aw[i] = (uint32_t)A[abase + ((2 * i) << R)] | ((uint32_t)A[abase + ((2 * i + 1) << R)] << 16);
+    } else if (mm == R) {
+      const uint4 *pa = reinterpret_cast<const uint4 *>(A + abase);
+      const uint4 q0 = pa[0], q1 = pa[1];
+      aw[0] = q0.x; aw[1] = q0.y; aw[2] = q0.z; aw[3] = q0.w; aw[4] = q1.x; aw[5] = q1.y; aw[6] = q1.z; aw[7] = q1.w;
+    } else if (mm == R + 1) {
+      const uint4 q0 = *reinterpret_cast<const uint4 *>(A + abase);
+      aw[0] = q0.x; aw[1] = q0.y; aw[2] = q0.z; aw[3] = q0.w;
+    } else if (mm == R + 2) {
+      const uint2 q0 = *reinterpret_cast<const uint2 *>(A + abase);
+      aw[0] = q0.x; aw[1] = q0.y;
+    } else {
+      aw[0] = *reinterpret_cast<const uint32_t *>(A + abase);
+    }
+    // B level mm (restricted): entry e mod 2^(mm - R), or its single entry
+    const int boff = lev_off(mm, R);
+    if (mm <= R) {
+      bw[0] = B[boff];
+    } else if (mm == R + 1) {
+      bw[0] = *reinterpret_cast<const uint32_t *>(B + boff);
+    } else if (mm == R + 2) {
+      const uint2 q = *reinterpret_cast<const uint2 *>(B + boff);
+      bw[0] = q.x; bw[1] = q.y;
+    } else {
+      const uint4 q = *reinterpret_cast<const uint4 *>(B + boff);
+      bw[0] = q.x; bw[1] = q.y; bw[2] = q.z; bw[3] = q.w;
+    }
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+      const uint32_t av = mm < R ? half_of(aw, e) : half_of(aw, e >> (mm - R));
+      const uint32_t bv = mm <= R ? bw[0] : half_of(bw, e & ((1 << (mm - R)) - 1));
+      c[e] += __umul24(av, bv);
+    }
+  }
+  double h = 0.0;
+  const uint32_t m2 = (1u << n2) - 1u;
+#pragma unroll
+  for (int e = 0; e < 16; ++e) {
+    const uint32_t y = ys | ((uint32_t)e << R);
+    uint32_t cy = c[e];
+    if (idA && (y >> (k - n1)) == x1) {
+      const int m = k - n1;
+      cy += B[lev_off(m, R) + ((y & ((1u << m) - 1u)) >> R)];
+    }
+    if (idB && (y & m2) == x2) cy += A[lev_off(ea.jlo, R) + ((y >> n2) & lev_mask_top(ea.jlo, R))];
+    if (cy > 1u) h = fma((double)cy, log_tang_int(cy, lnF, invF), h);
+  }
+  return h;
+}
+
 __global__ void __launch_bounds__(1024) k_entropy_reps(EntropyArgs ea, const uint64_t *__restrict__ mask,
                                                       const uint32_t *__restrict__ wordbase,
                                                       const double *__restrict__ logtab,
@@ -459,6 +567,13 @@ __global__ void __launch_bounds__(1024) k_entropy_reps(EntropyArgs ea, const uin
                  ea.bHi, rb, 0, (uint32_t)ea.b, B);
     __syncthreads();
     double h = 0.0;
+    if (ea.groups >= 4) {  // 16 outputs per thread (k >= 2 rb + 4)
+      const int nsg = ea.groups >> 2;
+      for (int sg = threadIdx.x; sg < nsg; sg += blockDim.x) {
+        const uint32_t ys = ybase | ((uint32_t)sg << (4 + rb));
+        h += rb ? sweep16<1>(ea, A, B, ys, x1, x2, lnF, invF) : sweep16<0>(ea, A, B, ys, x1, x2, lnF, invF);
+      }
+    } else
     for (int g = threadIdx.x; g < ea.groups; g += blockDim.x) {
       const uint32_t y0 = ybase | ((uint32_t)g << (2 + rb));
       uint32_t c0 = 0, c1 = 0, c2 = 0, c3 = 0;
@@ -512,9 +627,9 @@ int launch_entropy_reps(int n, int k, int n1, int n2, const uint64_t *mask, cons
   ea.aHi = std::min(ea.jhi, n1 - 1);
   ea.bLo = k - ea.jhi;
   ea.bHi = std::min(k - ea.jlo, n2 - 1);
-  auto words = [](int hi, int r) {
+  auto words = [](int hi, int r) {  // multiples of 8 entries: 16-byte aligned B region
     const int w = r ? (hi ? 1 << hi : 1) : (2 << hi);
-    return (w + 3) & ~3;
+    return (w + 7) & ~7;
   };
   // restrict to output quadrants only when the full row sets do not fit
   const size_t kLdsCap = 136 * 1024;
@@ -546,8 +661,9 @@ int launch_entropy_reps(int n, int k, int n1, int n2, const uint64_t *mask, cons
     }
   const int free_bits = k - 2 * r - 2;
   ea.groups = free_bits > 0 ? 1 << free_bits : 1;
+  const int tasks = ea.groups >= 4 ? ea.groups / 4 : ea.groups;  // 16 or 4 outputs per thread
   int threads = 64;
-  while (threads < 1024 && threads < ea.groups) threads *= 2;
+  while (threads < 1024 && threads < tasks) threads *= 2;
   // representatives have top bit 0 (x <= ~x), so only x1 < N1/2 holds any
   const char *v = getenv("BDC_VERBOSE");
   const bool verbose = v && v[0] == '1';

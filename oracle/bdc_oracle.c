@@ -481,24 +481,26 @@ int or_num_threads(void) { return omp_get_max_threads(); }
  * log_W_term (backend/BAA.cu:401-411) times C(n,k), for a single input x
  * (n <= 32).  Terms with c <= 1 contribute 0. */
 double or_entropy_x(int n, int k, uint32_t x) {
+  /* extended-precision accumulation: at n = 32 the sum has 2^16 terms and
+   * reaches ~6e9, where a plain FP64 running sum loses ~1e-12 relative */
   const uint64_t ny = 1ull << k;
-  double h = 0.0;
+  long double h = 0.0L;
   for (uint64_t y = 0; y < ny; ++y) {
     const uint32_t c = or_count(x, n, (uint32_t)y, k);
-    if (c > 1) h += (double)c * log((double)c);
+    if (c > 1) h += (long double)c * logl((long double)c);
   }
-  return h;
+  return (double)h;
 }
 
 /* D(x||q) = sum_y W(y|x) (ln W(y|x) - ln q(y)) in nats for a single input x,
  * W = c / C(n,k) (compute_KL_kernel / KL_term, backend/BAA.cu:419-425, 533-571) */
 double or_kl_x(int n, int k, uint32_t x, const double *logq) {
   const uint64_t ny = 1ull << k;
-  const double C = (double)or_nck((uint32_t)n, (uint32_t)k), lnC = log(C);
-  double d = 0.0;
+  const long double C = (long double)or_nck((uint32_t)n, (uint32_t)k), lnC = logl(C);
+  long double d = 0.0L;
   for (uint64_t y = 0; y < ny; ++y) {
     const uint32_t c = or_count(x, n, (uint32_t)y, k);
-    if (c) d += (c / C) * (log((double)c) - lnC - logq[y]);
+    if (c) d += (c / C) * (logl((long double)c) - lnC - (long double)logq[y]);
   }
-  return d;
+  return (double)d;
 }
