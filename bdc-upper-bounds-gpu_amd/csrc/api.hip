@@ -922,24 +922,51 @@ int bdc_sample_inputs(bdc_ctx *c, int what, const uint32_t *x, uint64_t len, dou
 }  // extern "C"
 
 namespace bdc {
+// Is theta (2^k, host) invariant under y -> ~y and y -> rev y within a relative
+// tolerance (0: exactly)?  Only meaningful when n1 = n2.
+static bool theta_symmetric(const bdc_ctx *c, const double *theta, double rtol) {
+  if (c->n1 != c->n2) return false;
+  const int k = c->k;
+  const uint32_t ny = 1u << k;
+  for (uint32_t y = 0; y < ny; ++y) {
+    const uint32_t o[3] = {comp_bits(y, k), rev_bits(y, k), rev_bits(comp_bits(y, k), k)};
+    for (uint32_t z : o) {
+      const double a = theta[y], b = theta[z];
+      if (rtol == 0.0 ? a != b : !(std::fabs(a - b) <= rtol * (1.0 + std::fabs(a)))) return false;
+    }
+  }
+  return true;
+}
+
 // U_beta(theta) and its gradient, theta resident in c->theta.  Writes U to
-// devU (device) and the gradient to c->grad.
-static int dual_device(bdc_ctx *c, int prec, double beta, double *devU, bool want_grad) {
+// devU (device) and the gradient to c->grad.  sym: theta is complement- and
+// reversal-symmetric, so q and D(x||q) are too, and the D pass and W^T pi run
+// on the half space x1 < N1/2 with the mirror rule, as the BA step does; the
+// gradient (symmetric in exact arithmetic) is averaged over each output orbit
+// so that Adam keeps theta exactly symmetric.
+static int dual_device(bdc_ctx *c, int prec, double beta, double *devU, bool want_grad, bool sym) {
   const size_t ny = (size_t)1 << c->k;
   double *q = c->qv.as<double>(), *L = q + ny, *wq = q + 2 * ny, *wl = q + 3 * ny;
+  const int rows = sym ? ba_rows(c, prec) : c->N1;
+  const bool half = rows < c->N1;
   BDC_LAUNCH(launch_softmax_log(c->theta.as<double>(), c->k, q, L, c->scal.as<double>(), c->stream));
   if (prec == BDC_PREC_FP64)
-    BDC_TRY(cross_operands(c, L));
+    BDC_TRY(cross_operands(c, L, half, half ? rows : 0));
   else
-    BDC_TRY(split_cross_operands(c, L, false, c->N1));
+    BDC_TRY(split_cross_operands(c, L, half, rows));
   int nb = 0;
-  BDC_TRY(cross_term(c, prec, EPI_D, c->dvec.as<double>(), &nb));
-  BDC_LAUNCH(launch_dual_weights(c->dvec.as<double>(), c->nx, beta, c->part.as<double>(), nb, c->scal.as<double>(), devU,
-                                 c->stream));
+  BDC_TRY(cross_term(c, prec, EPI_D, c->dvec.as<double>(), &nb, rows));
+  BDC_LAUNCH(launch_dual_weights(c->dvec.as<double>(), (uint64_t)rows * c->N2, beta, (double)c->N1 / rows,
+                                 c->part.as<double>(), nb, c->scal.as<double>(), devU, c->stream));
   if (!want_grad) return BDC_OK;
   // W^T pi through the output-law contraction (adjoint of the cross term)
-  BDC_TRY(output_law(c, prec, c->dvec.as<double>(), kScalDualScale, false, c->N1, wq, wl));
-  BDC_LAUNCH(launch_vec_sub(q, wq, 1.0, c->grad.as<double>(), ny, c->stream));
+  BDC_TRY(output_law(c, prec, c->dvec.as<double>(), kScalDualScale, half, rows, wq, wl));
+  if (half) {
+    BDC_LAUNCH(launch_vec_sub(q, wq, 1.0, wl, ny, c->stream));
+    BDC_LAUNCH(launch_orbit_mean(wl, c->k, c->grad.as<double>(), c->stream));
+  } else {
+    BDC_LAUNCH(launch_vec_sub(q, wq, 1.0, c->grad.as<double>(), ny, c->stream));
+  }
   return BDC_OK;
 }
 static int ensure_dual(bdc_ctx *c) {
@@ -963,7 +990,7 @@ int bdc_dual(bdc_ctx *c, int prec, const double *theta, double beta, double *u, 
   BDC_HIP(hipMemcpyAsync(c->theta.p, theta, ny * 8, hipMemcpyHostToDevice, c->stream));
   // (a scalar slot of its own: the split-K scratch is reused by the gradient's output law)
   double *devU = c->scal.as<double>() + kScalDualU;
-  BDC_TRY(dual_device(c, prec, beta, devU, grad != nullptr));
+  BDC_TRY(dual_device(c, prec, beta, devU, grad != nullptr, theta_symmetric(c, theta, 0.0)));
   BDC_HIP(hipMemcpyAsync(u, devU, 8, hipMemcpyDeviceToHost, c->stream));
   if (grad) BDC_HIP(hipMemcpyAsync(grad, c->grad.p, ny * 8, hipMemcpyDeviceToHost, c->stream));
   BDC_HIP(hipStreamSynchronize(c->stream));
@@ -975,13 +1002,22 @@ int bdc_refine_dual(bdc_ctx *c, int prec, double *theta, double beta, int steps,
   if (prec == BDC_PREC_SPLIT16) BDC_TRY(split_prepare(c));
   BDC_TRY(ensure_dual(c));
   const size_t ny = (size_t)1 << c->k;
+  // a theta symmetric to 1e-9 relative (ln q of a BA iterate: the split path's
+  // limb truncation differs between an input and its reversal) is made exactly
+  // symmetric (orbit mean) and refined on the half space; the refined bound is
+  // certified in FP64 whatever the starting point
+  const bool sym = theta_symmetric(c, theta, 1e-9);
   BDC_HIP(hipMemcpyAsync(c->theta.p, theta, ny * 8, hipMemcpyHostToDevice, c->stream));
+  if (sym) {
+    BDC_HIP(hipMemcpyAsync(c->grad.p, c->theta.p, ny * 8, hipMemcpyDeviceToDevice, c->stream));
+    BDC_LAUNCH(launch_orbit_mean(c->grad.as<double>(), c->k, c->theta.as<double>(), c->stream));
+  }
   BDC_HIP(hipMemsetAsync(c->adam_m.p, 0, ny * 8, c->stream));
   BDC_HIP(hipMemsetAsync(c->adam_v.p, 0, ny * 8, c->stream));
   DevBuf uh;
   BDC_TRY(uh.alloc((size_t)std::max(steps, 1) * 8));
   for (int s = 0; s < steps; ++s) {
-    BDC_TRY(dual_device(c, prec, beta, uh.as<double>() + s, true));
+    BDC_TRY(dual_device(c, prec, beta, uh.as<double>() + s, true, sym));
     BDC_LAUNCH(launch_adam(c->theta.as<double>(), c->grad.as<double>(), c->adam_m.as<double>(), c->adam_v.as<double>(), ny,
                            lr, 0.9, 0.999, 1e-8, s + 1, c->stream));
   }
@@ -1110,7 +1146,17 @@ int split_prepare(bdc_ctx *c) {
   BDC_HIP(hipMemcpyAsync(a.data(), c->abar16.p, a.size() * 2, hipMemcpyDeviceToHost, c->stream));
   if (KRp) BDC_HIP(hipMemcpyAsync(b.data(), c->bbar16.p, b.size() * 2, hipMemcpyDeviceToHost, c->stream));
   BDC_HIP(hipStreamSynchronize(c->stream));
-  auto round16 = [](uint16_t v) -> _Float16 { return (_Float16)(float)v; };
+  // counts fp16 cannot hold are rounded toward zero (the exact residual r >= 0 comes
+  // from the sparse lists), so the count mass behind each exactness bound below is
+  // at most the true mass (2^16 per row of [Abar | Bbar] at half length 16)
+  auto round16 = [](uint16_t v) -> _Float16 {
+    _Float16 h = (_Float16)(float)v;
+    if ((float)h > (float)v) {  // next fp16 toward zero: one step down in the bit pattern (v > 0)
+      const uint16_t b = __builtin_bit_cast(uint16_t, h);
+      h = __builtin_bit_cast(_Float16, (uint16_t)(b - 1u));
+    }
+    return h;
+  };
   std::vector<_Float16> ah(N1 * KLp), aTh(KLp * N1), bh(N2 * KRp);
   std::vector<int32_t> resA, rowptrA(N1 + 1, 0), entA, rowptrB(N2 + 1, 0), entB;
   double rowA = 0, rowB = 0, chunk = 0;
@@ -1164,10 +1210,13 @@ int split_prepare(bdc_ctx *c) {
     return fail(BDC_ERR_NUMERIC, "exact FP32 accumulation needs limbs of %d / %d bits, too narrow", sp.g1_bits, sp.g2_bits);
   sp.g1_bits = std::min(sp.g1_bits, env_int("BDC_SPLIT_G1_BITS", sp.g1_bits));
   sp.g2_bits = std::min(sp.g2_bits, env_int("BDC_SPLIT_G2_BITS", sp.g2_bits));
-  sp.g1_limbs = env_int("BDC_SPLIT_G1_LIMBS", std::min(4, (36 + sp.g1_bits - 1) / sp.g1_bits));
+  // G1: 4 limbs (36 bits of p per tile at 9 bits), 5 when the limbs are 7 bits or
+  // narrower (half length >= 15); G2: 4 limbs
+  sp.g1_limbs = env_int("BDC_SPLIT_G1_LIMBS", std::min(sp.g1_bits <= 7 ? 5 : 4, (36 + sp.g1_bits - 1) / sp.g1_bits));
   sp.g2_limbs = env_int("BDC_SPLIT_G2_LIMBS", std::min(4, (42 + sp.g2_bits - 1) / sp.g2_bits));
-  if (sp.g1_limbs < 2 || sp.g1_limbs > 4 || sp.g2_limbs < 2 || sp.g2_limbs > 4)
-    return fail(BDC_ERR_ARG, "limb counts must be in [2, 4] (got %d, %d)", sp.g1_limbs, sp.g2_limbs);
+  if (sp.g1_limbs < 2 || sp.g1_limbs > 5 || sp.g2_limbs < 2 || sp.g2_limbs > 4)
+    return fail(BDC_ERR_ARG, "limb counts must be in [2, 5] for G1 and [2, 4] for G2 (got %d, %d)", sp.g1_limbs,
+                sp.g2_limbs);
   std::vector<uint8_t> fA(N1 / 128, 0), fB(N2 / 64, 0);
   for (size_t x1 = 0; x1 < N1; ++x1)
     if (rowptrA[x1 + 1] > rowptrA[x1]) fA[x1 / 128] = 1;

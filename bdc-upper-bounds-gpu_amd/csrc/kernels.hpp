@@ -82,9 +82,12 @@ int launch_q_combine(const QCombine &qc, double inv_c, double *scal, double *qou
                      hipStream_t s);
 // theta -> q = softmax(theta), L = ln q - mu (dual bound)
 int launch_softmax_log(const double *theta, int k, double *q, double *L, double *scal, hipStream_t s);
-// D-pass reductions for the dual: max D, then pi = softmax(beta D) and U
-int launch_dual_weights(double *D, uint64_t nx, double beta, double *part, int nblocks, double *scal, double *u_out,
-                        hipStream_t s);
+// D-pass reductions for the dual: max D, then pi = softmax(beta D) and U over
+// nx inputs, each standing for `mult` inputs (2: the half space of a symmetric q)
+int launch_dual_weights(double *D, uint64_t nx, double beta, double mult, double *part, int nblocks, double *scal,
+                        double *u_out, hipStream_t s);
+// dst[y] = mean of src over the complement/reversal orbit of y (2^k entries)
+int launch_orbit_mean(const double *src, int k, double *dst, hipStream_t s);
 int launch_adam(double *theta, const double *grad, double *m, double *v, uint64_t len, double lr, double b1, double b2,
                 double eps, int step, hipStream_t s);
 int launch_copy2d(const double *src, int64_t srs, int64_t scs, double *dst, int64_t drs, int64_t dcs, int64_t R,

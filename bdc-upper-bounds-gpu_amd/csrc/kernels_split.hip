@@ -601,7 +601,12 @@ int launch_split_g1(int KLp, int Kdim, int lda, int ncols, const _Float16 *aT, c
 #define LG(LL)                                                                                                  \
   hipLaunchKernelGGL(k_split_g1<LL>, dim3(blocks), dim3(512), 0, s, KLp, Kdim, lda, ncols, aT, X, texp, bits, ks, \
                      Tpart)
-  BDC_SPLIT_L(limbs, LG)
+  // five limbs for the narrow limbs of half length >= 15 (7 bits: 35 bits of p per tile)
+  if (limbs == 5) {
+    LG(5);
+  } else {
+    BDC_SPLIT_L(limbs, LG)
+  }
 #undef LG
   return hipGetLastError() == hipSuccess ? BDC_OK : BDC_ERR_HIP;
 }

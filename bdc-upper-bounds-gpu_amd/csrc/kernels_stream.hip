@@ -200,24 +200,43 @@ __global__ void __launch_bounds__(256) k_dual_exp(double *__restrict__ D, uint64
   s = block_sum(s, red);
   if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
-__global__ void __launch_bounds__(256) k_dual_final(const double *__restrict__ part, int np, double beta,
+__global__ void __launch_bounds__(256) k_dual_final(const double *__restrict__ part, int np, double beta, double mult,
                                                     double *__restrict__ scal, double *__restrict__ u_out) {
   __shared__ double red[16];
   double s = 0.0;
   for (int i = threadIdx.x; i < np; i += blockDim.x) s += part[i];
-  s = block_sum(s, red);
+  s = mult * block_sum(s, red);
   if (threadIdx.x == 0) {
     scal[kScalSumW] = s;
     scal[kScalDualScale] = 1.0 / s;
     *u_out = scal[kScalDMax] + log(s) / beta;
   }
 }
-int launch_dual_weights(double *D, uint64_t nx, double beta, double *part, int nblocks, double *scal, double *u_out,
-                        hipStream_t s) {
+int launch_dual_weights(double *D, uint64_t nx, double beta, double mult, double *part, int nblocks, double *scal,
+                        double *u_out, hipStream_t s) {
   hipLaunchKernelGGL(k_dual_max, dim3(1), dim3(256), 0, s, part, nblocks, scal);
   const int np = 1024;
   hipLaunchKernelGGL(k_dual_exp, dim3(np), dim3(256), 0, s, D, nx, beta, scal, part);
-  hipLaunchKernelGGL(k_dual_final, dim3(1), dim3(256), 0, s, part, np, beta, scal, u_out);
+  hipLaunchKernelGGL(k_dual_final, dim3(1), dim3(256), 0, s, part, np, beta, mult, scal, u_out);
+  return hipGetLastError() == hipSuccess ? BDC_OK : BDC_ERR_HIP;
+}
+
+// dst[y] = mean of src over the complement/reversal orbit of y (the members
+// summed in ascending index order, so every member gets the same value)
+__global__ void k_orbit_mean(const double *__restrict__ src, int k, double *__restrict__ dst) {
+  const uint32_t y = blockIdx.x * blockDim.x + threadIdx.x;
+  if (y >= (1u << k)) return;
+  uint32_t m[4] = {y, comp_bits(y, k), rev_bits(y, k), rev_bits(comp_bits(y, k), k)};
+  for (int i = 1; i < 4; ++i)  // sort the four indices
+    for (int j = i; j > 0 && m[j] < m[j - 1]; --j) {
+      const uint32_t t = m[j];
+      m[j] = m[j - 1];
+      m[j - 1] = t;
+    }
+  dst[y] = 0.25 * (((src[m[0]] + src[m[1]]) + src[m[2]]) + src[m[3]]);
+}
+int launch_orbit_mean(const double *src, int k, double *dst, hipStream_t s) {
+  hipLaunchKernelGGL(k_orbit_mean, dim3(((1u << k) + 255) / 256), dim3(256), 0, s, src, k, dst);
   return hipGetLastError() == hipSuccess ? BDC_OK : BDC_ERR_HIP;
 }
 

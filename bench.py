@@ -93,13 +93,22 @@ def run_dual(bdc, n, k, steps, lr, init_iters, betas):
     e.sync()
     init_s = time.perf_counter() - t
     theta0 = e.log_output()
+    # asymmetry of theta0 under y -> ~y and y -> rev y (the refinement takes the
+    # half-space path below 1e-9 relative)
+    ny = 1 << k
+    ys = np.arange(ny)
+    rev = np.array([int(format(v, f"0{k}b")[::-1], 2) for v in range(ny)])
+    asym = max(float(np.max(np.abs(theta0 - theta0[ys ^ (ny - 1)]) / (1 + np.abs(theta0)))),
+               float(np.max(np.abs(theta0 - theta0[rev]) / (1 + np.abs(theta0)))))
+    log(f"theta0 relative asymmetry {asym:.2e}")
     ub_init = e.certify()["upper_cert"] / LN2
     log(f"{init_iters} BA steps in {init_s:.2f} s, certified UB {ub_init:.10f} bits")
     rng = np.random.default_rng(2)
     coords = rng.choice(1 << k, 64, replace=False)
     h = 1e-6
     out = {"metric": f"dual-bound refinement at BDC_({n},{k})", "n": n, "k": k, "setup_s": setup_s,
-           "init_iters": init_iters, "init_s": init_s, "ub_after_init_bits": ub_init, "betas": {}}
+           "init_iters": init_iters, "init_s": init_s, "ub_after_init_bits": ub_init, "theta0_asymmetry": asym,
+           "half_space_refinement": asym < 1e-9, "betas": {}}
     for beta in betas:
         u64, g64 = e.dual(theta0, beta, bdc.PREC_FP64)
         u16, g16 = e.dual(theta0, beta, bdc.PREC_SPLIT16)
@@ -295,6 +304,7 @@ def run_published(bdc, cases, max_iters, precision):
             rec["lb_below_published"] = bool(rec["lb_bits"] <= pub + 5e-5)
         log(f"BDC_({n},{k}) a={tol}: {len(st)} steps ({secs:.2f} s, setup {setup:.1f} s) LB {rec['lb_bits']:.6f} "
             f"UB_ref {rec['ub_ref_bits']:.6f} UB_cert {rec['ub_cert_bits']:.6f} published {pub}")
+        print(json.dumps({"case": f"{n}_{k}", **rec}), flush=True)  # one line per case as it completes
         out["cases"][f"{n}_{k}"] = rec
     return out
 

@@ -128,7 +128,11 @@ int bdc_sample_inputs(bdc_ctx *ctx, int what, const uint32_t *x, uint64_t len, d
 int bdc_dual(bdc_ctx *ctx, int precision, const double *theta, double beta, double *u, double *grad);
 /* Adam on theta = log q (device resident): `steps` steps at learning rate lr with
  * the default Adam constants (0.9, 0.999, 1e-8).  theta is read and written
- * (2^k).  u_hist (may be NULL) receives U_beta before each step. */
+ * (2^k).  u_hist (may be NULL) receives U_beta before each step.  When n is even
+ * and theta is invariant under y -> ~y and y -> rev y to 1e-9 relative (ln q of a
+ * BA iterate is), theta is first averaged over each such output orbit and the
+ * steps run on the half input space with the BA step's mirror rule, the
+ * gradient averaged over each orbit so theta stays symmetric. */
 int bdc_refine_dual(bdc_ctx *ctx, int precision, double *theta, double beta, int steps, double lr, double *u_hist);
 /* Certified upper bound max_x D(x||softmax theta) in FP64, nats. */
 int bdc_certify_q(bdc_ctx *ctx, const double *theta, double *upper);

@@ -352,3 +352,46 @@ def test_dual_value_with_gradient(bdc):
             u_g, _ = e.dual(theta, 256.0, prec)
             u = e.dual(theta, 256.0, prec, want_grad=False)
             assert abs(u_g - u) < 1e-12 * max(1.0, abs(u)), (prec, u_g, u)
+
+
+def _output_orbits(k):
+    """the four images (y, ~y, rev y, rev ~y) of every output y"""
+    ny = 1 << k
+    y = np.arange(ny)
+    rev = np.array([int(format(v, f"0{k}b")[::-1], 2) for v in range(ny)])
+    comp = y ^ (ny - 1)
+    return np.stack([y, comp, rev, rev[comp]])
+
+
+def _symmetric_theta(k, seed):
+    """theta with one value per complement/reversal orbit of the outputs"""
+    imgs = _output_orbits(k)
+    vals = np.random.default_rng(seed).normal(scale=0.5, size=1 << k)
+    return vals[imgs.min(axis=0)]
+
+
+def _is_symmetric(theta, k):
+    imgs = _output_orbits(k)
+    return all(np.array_equal(theta, theta[m]) for m in imgs[1:])
+
+
+@pytest.mark.parametrize("n,k", [(14, 7), (16, 8)])
+def test_dual_symmetric_half_space(bdc, oracle, monkeypatch, n, k):
+    """A complement/reversal-symmetric theta takes the half-space path (the BA
+    step's mirror rule over x1 < N1/2); U and dU/dtheta against the oracle's
+    full-space sums, and the refinement against the full-space refinement."""
+    say(f"dual symmetric n={n} k={k}")
+    theta = _symmetric_theta(k, 4)
+    assert _is_symmetric(theta, k)
+    with bdc.Engine(n, k) as e:
+        for prec, tol in ((bdc.PREC_FP64, 1e-12), (bdc.PREC_SPLIT16, 1e-9)):
+            U, g = e.dual(theta, 256.0, prec)
+            Uo, go = oracle.full_dual(n, k, theta, 256.0)
+            assert abs(U - Uo) < tol
+            np.testing.assert_allclose(g, go, rtol=0, atol=tol)
+        th_half, u_half = e.refine_dual(theta, 256.0, 6, 1e-2)
+        assert _is_symmetric(th_half, k)  # the orbit-averaged gradient keeps theta symmetric
+        monkeypatch.setenv("BDC_HALF_SPACE", "0")
+        th_full, u_full = e.refine_dual(theta, 256.0, 6, 1e-2)
+        np.testing.assert_allclose(th_half, th_full, rtol=0, atol=1e-10)
+        np.testing.assert_allclose(u_half, u_full, rtol=0, atol=1e-12)
