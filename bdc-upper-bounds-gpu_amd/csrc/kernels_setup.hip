@@ -667,7 +667,10 @@ int launch_entropy_reps(int n, int k, int n1, int n2, const uint64_t *mask, cons
   // representatives have top bit 0 (x <= ~x), so only x1 < N1/2 holds any
   const char *v = getenv("BDC_VERBOSE");
   const bool verbose = v && v[0] == '1';
-  const uint32_t X1 = 1u << (n1 - 1), chunk = X1 >= 16 ? X1 / 16 : X1;
+  // one launch per quadrant (low x1 rows hold the most representatives; the block
+  // scheduler balances them against the light rows), or 16 chunks of rows when
+  // progress is reported
+  const uint32_t X1 = 1u << (n1 - 1), chunk = (verbose && X1 >= 16) ? X1 / 16 : X1;
   const int quads = r ? 4 : 1;
   for (int q = 0; q < quads; ++q) {
     ea.t = q >> 1;

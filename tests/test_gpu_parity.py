@@ -395,3 +395,22 @@ def test_dual_symmetric_half_space(bdc, oracle, monkeypatch, n, k):
         th_full, u_full = e.refine_dual(theta, 256.0, 6, 1e-2)
         np.testing.assert_allclose(th_half, th_full, rtol=0, atol=1e-10)
         np.testing.assert_allclose(u_half, u_full, rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("n,k", [(16, 8), (20, 10)])
+def test_runs_are_bitwise_deterministic(bdc, n, k):
+    """No reduction uses atomics (block partials are combined in a fixed order),
+    so two runs of the same steps give bitwise identical statistics, p and
+    certificate at both precisions; the reference's FP64 atomicAdd into the
+    output law (BAA.cu:477-480) does not have this property."""
+    say(f"determinism n={n} k={k}")
+    with bdc.Engine(n, k) as e:
+        for prec in (bdc.PREC_FP64, bdc.PREC_SPLIT16):
+            runs = []
+            for _ in range(2):
+                e.set_p_dirichlet(1.0, 3)
+                st = e.iterate(8, prec)
+                runs.append((st, e.get_p_orbits(), e.certify()))
+            (s0, p0, c0), (s1, p1, c1) = runs
+            assert np.array_equal(s0, s1) and np.array_equal(p0, p1), prec
+            assert c0 == c1, prec
