@@ -1210,9 +1210,11 @@ int split_prepare(bdc_ctx *c) {
     return fail(BDC_ERR_NUMERIC, "exact FP32 accumulation needs limbs of %d / %d bits, too narrow", sp.g1_bits, sp.g2_bits);
   sp.g1_bits = std::min(sp.g1_bits, env_int("BDC_SPLIT_G1_BITS", sp.g1_bits));
   sp.g2_bits = std::min(sp.g2_bits, env_int("BDC_SPLIT_G2_BITS", sp.g2_bits));
-  // G1: 4 limbs (36 bits of p per tile at 9 bits), 5 when the limbs are 7 bits or
-  // narrower (half length >= 15); G2: 4 limbs
-  sp.g1_limbs = env_int("BDC_SPLIT_G1_LIMBS", std::min(sp.g1_bits <= 7 ? 5 : 4, (36 + sp.g1_bits - 1) / sp.g1_bits));
+  // G1: enough limbs for 44 bits of p relative to its tile maximum, at most 5
+  // (5 x 9 = 45 bits at (28,14): the per-step error against FP64 stays ~1e-12 bits
+  // over 2000 steps, against ~2e-9 with 4 x 9 = 36 bits, for ~4% of the step time);
+  // G2: 4 limbs (4 x 11 = 44 bits at (28,14), 4 x 9 = 36 at half length 16)
+  sp.g1_limbs = env_int("BDC_SPLIT_G1_LIMBS", std::min(5, (44 + sp.g1_bits - 1) / sp.g1_bits));
   sp.g2_limbs = env_int("BDC_SPLIT_G2_LIMBS", std::min(4, (42 + sp.g2_bits - 1) / sp.g2_bits));
   if (sp.g1_limbs < 2 || sp.g1_limbs > 5 || sp.g2_limbs < 2 || sp.g2_limbs > 4)
     return fail(BDC_ERR_ARG, "limb counts must be in [2, 5] for G1 and [2, 4] for G2 (got %d, %d)", sp.g1_limbs,
