@@ -178,27 +178,40 @@ __global__ void __launch_bounds__(512) k_split_g1(int KLp, int Kdim, int lda, in
   const char *cX = reinterpret_cast<const char *>(X + (int64_t)kbase * ncols + n0);
   const uint32_t offA = (uint32_t)(rbase * lda + q8) * 2u;
   const uint32_t offX = (uint32_t)(2 * rp * ncols + 2 * cp) * 8u;
-  half8 ra[4];
-  double2 rx[2];
-  int etile = 0;
-  auto gload = [&](int s) {
+  // Register ring of D stages: while stage s is multiplied the loads of stages
+  // s + 1 .. s + D - 1 are in flight (one block of 8 waves per CU holds too few
+  // bytes in flight to cover HBM latency with a single stage of prefetch).  The
+  // slots are compile-time (the loop is unrolled by D), and each stage's counts
+  // and p are issued together, so waiting for stage s leaves the later stages'
+  // loads outstanding.
+  constexpr int D = 3;
+  half8 ra[D][4];
+  double2 rx[D][2];
+  int et[D];
+  auto gload = [&](int s, auto slot) {
+    constexpr int u = decltype(slot)::value;
     const uint32_t k2 = (uint32_t)(s * BK * 2);
+    // rows past KLp (padding of the last block) read row KLp - 1 instead: they
+    // only feed MFMA output rows that are never stored, and an unconditional load
+    // keeps the loop body branch-free for the wait counts
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      const bool ok = m0 + rbase + 64 * i < KLp;
-      ra[i] = ok ? *reinterpret_cast<const half8 *>(cA + (int64_t)(64 * i) * lda * 2 + (offA + k2)) : (half8){0};
+      const int dr = min(64 * i, KLp - 1 - m0 - rbase);  // row offset from this thread's first row
+      ra[u][i] = *reinterpret_cast<const half8 *>(cA + (int64_t)dr * lda * 2 + (offA + k2));
     }
     const char *xs = cX + (int64_t)s * BK * ncols * 8;  // stage base (64-bit, uniform)
 #pragma unroll
-    for (int i = 0; i < 2; ++i) rx[i] = *reinterpret_cast<const double2 *>(xs + (int64_t)i * ncols * 8 + offX);
-    etile = texp[((kbase + s * BK) / 128) * tcols + n0 / 32];
+    for (int i = 0; i < 2; ++i) rx[u][i] = *reinterpret_cast<const double2 *>(xs + (int64_t)i * ncols * 8 + offX);
+    et[u] = texp[((kbase + s * BK) / 128) * tcols + n0 / 32];
   };
   const double radix = ldexp(1.0, bits);
-  auto sstore = [&]() {
+  auto sstore = [&](auto slot) {
+    constexpr int u = decltype(slot)::value;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) *reinterpret_cast<half8 *>(&sm[(rbase + 64 * i) * LD + q8]) = ra[i];
+    for (int i = 0; i < 4; ++i) *reinterpret_cast<half8 *>(&sm[(rbase + 64 * i) * LD + q8]) = ra[u][i];
     // limbs of rows (2 rp, 2 rp + 1) of column 2 cp + q: one packed half2 per limb
-    double v[2][2] = {{ldexp(rx[0].x, -etile), ldexp(rx[1].x, -etile)}, {ldexp(rx[0].y, -etile), ldexp(rx[1].y, -etile)}};
+    const int e = et[u];
+    double v[2][2] = {{ldexp(rx[u][0].x, -e), ldexp(rx[u][1].x, -e)}, {ldexp(rx[u][0].y, -e), ldexp(rx[u][1].y, -e)}};
 #pragma unroll
     for (int l = 0; l < L; ++l)
 #pragma unroll
@@ -218,14 +231,17 @@ __global__ void __launch_bounds__(512) k_split_g1(int KLp, int Kdim, int lda, in
 #pragma unroll
   for (int l = 0; l < L; ++l) acc[l] = (float16_){0};
   const int nst = krows / BK;
-  gload(0);
-#pragma unroll 1
-  for (int s = 0; s < nst; ++s) {
+  // one stage: store slot `cur` (stage s) to LDS, prefetch stage s + D - 1 into
+  // slot `nxt` (the slot stage s - 1 used; near the end the last stage is loaded
+  // again into that free slot, so the loop body has no branch and the waits
+  // before the LDS stores leave the later stages' loads outstanding), multiply,
+  // promote every 128 rows
+  auto step = [&](int s, auto cur, auto nxt) {
     if (s > 0) __syncthreads();
-    const int et = etile;
-    sstore();
+    const int e = et[decltype(cur)::value];
+    sstore(cur);
     __syncthreads();
-    if (s + 1 < nst) gload(s + 1);
+    gload(min(s + D - 1, nst - 1), nxt);
 #pragma unroll
     for (int kk = 0; kk < BK / 16; ++kk) {
       const half8 a = opnd(wave * 32 + lr, kk);
@@ -235,7 +251,7 @@ __global__ void __launch_bounds__(512) k_split_g1(int KLp, int Kdim, int lda, in
     if (s & 1) {  // 128 K rows done: promote the exact integers to FP64
       double sc[L];
 #pragma unroll
-      for (int l = 0; l < L; ++l) sc[l] = ldexp(1.0, et - bits * l);
+      for (int l = 0; l < L; ++l) sc[l] = ldexp(1.0, e - bits * l);
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
         double t = 0.0;
@@ -246,7 +262,22 @@ __global__ void __launch_bounds__(512) k_split_g1(int KLp, int Kdim, int lda, in
 #pragma unroll
       for (int l = 0; l < L; ++l) acc[l] = (float16_){0};
     }
+  };
+  using U0 = std::integral_constant<int, 0>;
+  using U1 = std::integral_constant<int, 1>;
+  using U2 = std::integral_constant<int, 2>;
+  static_assert(D == 3, "the unrolled ring below has three slots");
+  gload(0, U0{});
+  gload(min(1, nst - 1), U1{});
+  int s0 = 0;
+#pragma unroll 1
+  for (; s0 + D <= nst; s0 += D) {
+    step(s0, U0{}, U2{});
+    step(s0 + 1, U1{}, U0{});
+    step(s0 + 2, U2{}, U1{});
   }
+  if (s0 < nst) step(s0, U0{}, U2{});  // the last nst mod 3 stages
+  if (s0 + 1 < nst) step(s0 + 1, U1{}, U0{});
   double *T = Tpart + (int64_t)kz * KLp * ncols;
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
