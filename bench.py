@@ -352,6 +352,8 @@ def main():
                     help="dual-bound refinement config instead (default BDC_(28,18), 200 BA steps, 500 Adam steps)")
     ap.add_argument("--dual-k", type=int, default=18)
     ap.add_argument("--adam-steps", type=int, default=500)
+    ap.add_argument("--dual-betas", default="64,256,1024,4096,16384,65536",
+                    help="smoothing parameters of U_beta, comma-separated (nats^-1)")
     ap.add_argument("--sweep", action="store_true",
                     help="k-sweep config instead: BDC_(24,k), k = 8..20 step 2, Dirichlet(1) seed 0, tol 1e-8")
     ap.add_argument("--sweep-n", type=int, default=24)
@@ -393,7 +395,8 @@ def main():
         return
     if args.dual:
         import bdc_gpu as bdc
-        print(json.dumps(run_dual(bdc, args.n, args.dual_k, args.adam_steps, 1e-3, 200, (64.0, 256.0, 1024.0))),
+        print(json.dumps(run_dual(bdc, args.n, args.dual_k, args.adam_steps, 1e-3, 200,
+                                  [float(b) for b in args.dual_betas.split(",")])),
               flush=True)
         return
     if args.gpus != 1:

@@ -132,7 +132,11 @@ int bdc_dual(bdc_ctx *ctx, int precision, const double *theta, double beta, doub
  * and theta is invariant under y -> ~y and y -> rev y to 1e-9 relative (ln q of a
  * BA iterate is), theta is first averaged over each such output orbit and the
  * steps run on the half input space with the BA step's mirror rule, the
- * gradient averaged over each orbit so theta stays symmetric. */
+ * gradient averaged over each orbit so theta stays symmetric.  U_beta exceeds
+ * max_x D by at most n ln 2 / beta nats, so beta must make that small against the
+ * improvement sought: from ln q of 200 BA steps at n = 28, k = 18, beta = 16384
+ * and lr 1e-3 lower the certified bound faster per pass than BA; beta <= 1024
+ * does not. */
 int bdc_refine_dual(bdc_ctx *ctx, int precision, double *theta, double beta, int steps, double lr, double *u_hist);
 /* Certified upper bound max_x D(x||softmax theta) in FP64, nats. */
 int bdc_certify_q(bdc_ctx *ctx, const double *theta, double *upper);

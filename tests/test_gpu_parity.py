@@ -397,6 +397,29 @@ def test_dual_symmetric_half_space(bdc, oracle, monkeypatch, n, k):
         np.testing.assert_allclose(u_half, u_full, rtol=0, atol=1e-12)
 
 
+@pytest.mark.parametrize("n,k", [(16, 10)])
+def test_dual_refinement_beats_ba_per_pass(bdc, oracle, n, k):
+    """From theta = ln q after 50 split BA steps from the uniform p, 150 Adam
+    steps on U_beta (beta = 4096, lr 1e-3) certify a lower max_x D(x || q) than
+    150 more BA steps from the same p (each step of either is one pass over
+    the inputs; dense CPU runs of the same workload give a margin of 5e-3
+    bits).  The refined bound is checked against the oracle's D(x || q)."""
+    say(f"dual vs BA per pass n={n} k={k}")
+    with bdc.Engine(n, k) as e:
+        e.set_p_uniform()
+        e.iterate(50, bdc.PREC_SPLIT16)
+        theta0 = e.log_output()
+        th, uh = e.refine_dual(theta0, 4096.0, 150, 1e-3, bdc.PREC_SPLIT16)
+        ub_dual = e.certify_q(th)
+        e.iterate(150, bdc.PREC_SPLIT16)  # the refinement leaves p as it was
+        ub_ba = e.certify()["upper_cert"]
+    assert uh[-1] < uh[0]
+    q = np.exp(th - th.max())
+    D = oracle.full_kl(n, k, np.log(q / q.sum()))
+    assert abs(D.max() - ub_dual) < 1e-11
+    assert ub_dual < ub_ba - 1e-3 * LN2, (ub_dual / LN2, ub_ba / LN2)
+
+
 @pytest.mark.parametrize("n,k", [(16, 8), (20, 10)])
 def test_runs_are_bitwise_deterministic(bdc, n, k):
     """No reduction uses atomics (block partials are combined in a fixed order),
