@@ -147,7 +147,7 @@ __global__ void k_split_w(const double *__restrict__ W, int64_t len, int KR, int
 // Block: 256 kappa x 32 columns (all of Tbar's rows for KLp <= 256, so X is
 // read once), 8 waves of 32 x 32 with all L limbs.  A K stage covers 64 rows
 // of X: the counts come as one 128-byte line per row of aT (8 lanes of 16
-// bytes); the 64 x 32 block of X is read as 16-byte pairs, split on the fly
+// bytes); the 64 x 32 block of X is read as 16-byte pairs, split (one stage ahead)
 // into L integer limbs with the exponent of its 128 x 32 tile, and written to
 // LDS transposed (column-major) next to the counts; rows of 144 bytes keep the
 // MFMA operand reads conflict-free.  The FP32 sums (exact integers) are
@@ -170,26 +170,33 @@ __global__ void __launch_bounds__(512) k_split_g1(int KLp, int Kdim, int lda, in
   const int q8 = (tid & 7) * 8, rbase = tid >> 3, lr = lane & 31, lk = 8 * (lane >> 5);
 
   // counts: thread rows rbase + 64 i (i < 4) of aT.  X: thread takes rows
-  // 2 rp, 2 rp + 1 and columns 2 cp, 2 cp + 1 of the stage (rp = tid / 16,
-  // cp = tid % 16).  Wave-uniform bases plus 32-bit per-lane byte offsets that
-  // carry the stage offset (a loop-invariant sum would be hoisted as VGPR pointers).
-  const int rp = tid >> 4, cp = tid & 15;
+  // 2 rp, 2 rp + 1 and columns 2 cp, 2 cp + 1 of the stage; a wave covers 8
+  // values of each (rp = 8 (wave / 2) + lane % 8, cp = 8 (wave % 2) + lane / 8),
+  // so the transposed 4-byte limb stores (LDS word 36 (2 cp + q) + rp) fall in
+  // 8 (lane / 8) + lane % 8 + const: a different bank for every lane, whether
+  // the LDS has 32 or 64 banks.  Wave-uniform bases plus 32-bit per-lane byte
+  // offsets that carry the stage offset (a loop-invariant sum would be hoisted
+  // as VGPR pointers).
+  const int rp = 8 * (wave >> 1) + (lane & 7), cp = 8 * (wave & 1) + (lane >> 3);
   const char *cA = reinterpret_cast<const char *>(aT + (int64_t)m0 * lda + kbase);
   const char *cX = reinterpret_cast<const char *>(X + (int64_t)kbase * ncols + n0);
   const uint32_t offA = (uint32_t)(rbase * lda + q8) * 2u;
   const uint32_t offX = (uint32_t)(2 * rp * ncols + 2 * cp) * 8u;
-  // Register ring of D stages: while stage s is multiplied the loads of stages
-  // s + 1 .. s + D - 1 are in flight (one block of 8 waves per CU holds too few
-  // bytes in flight to cover HBM latency with a single stage of prefetch).  The
-  // slots are compile-time (the loop is unrolled by D), and each stage's counts
-  // and p are issued together, so waiting for stage s leaves the later stages'
-  // loads outstanding.
+  // Prefetch: the counts (L2 hits: every block reads the same rows of aT) are
+  // loaded one stage ahead into one register slot; p (an HBM stream, too few
+  // bytes in flight per CU to cover its latency with one stage) comes through
+  // a ring of three slots loaded three stages ahead, and is split into limbs
+  // one stage before it is stored, while the previous stage is multiplied (the
+  // FP64 split then overlaps the matrix cores instead of sitting between the
+  // two barriers of a stage).  Stage s's p is in slot s % 3; the loop is
+  // unrolled so slots are compile-time, and counts and p are issued in stage
+  // order, so waiting for the counts of stage s leaves the later p loads
+  // outstanding.
   constexpr int D = 3;
-  half8 ra[D][4];
+  half8 ra[4];
   double2 rx[D][2];
   int et[D];
-  auto gload = [&](int s, auto slot) {
-    constexpr int u = decltype(slot)::value;
+  auto gload_a = [&](int s) {
     const uint32_t k2 = (uint32_t)(s * BK * 2);
     // rows past KLp (padding of the last block) read row KLp - 1 instead: they
     // only feed MFMA output rows that are never stored, and an unconditional load
@@ -197,30 +204,45 @@ __global__ void __launch_bounds__(512) k_split_g1(int KLp, int Kdim, int lda, in
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int dr = min(64 * i, KLp - 1 - m0 - rbase);  // row offset from this thread's first row
-      ra[u][i] = *reinterpret_cast<const half8 *>(cA + (int64_t)dr * lda * 2 + (offA + k2));
+      ra[i] = *reinterpret_cast<const half8 *>(cA + (int64_t)dr * lda * 2 + (offA + k2));
     }
+  };
+  auto gload_x = [&](int s, auto slot) {
+    constexpr int u = decltype(slot)::value;
     const char *xs = cX + (int64_t)s * BK * ncols * 8;  // stage base (64-bit, uniform)
 #pragma unroll
     for (int i = 0; i < 2; ++i) rx[u][i] = *reinterpret_cast<const double2 *>(xs + (int64_t)i * ncols * 8 + offX);
     et[u] = texp[((kbase + s * BK) / 128) * tcols + n0 / 32];
   };
   const double radix = ldexp(1.0, bits);
-  auto sstore = [&](auto slot) {
+  // limbs of rows (2 rp, 2 rp + 1) of column 2 cp + q, one packed half2 per
+  // limb: split_begin scales the p values of a slot by its tile exponent,
+  // split_limb(l) takes limb l off them (called for l = 0, 1, ... in order)
+  half2_ hl[L][2];
+  auto split_begin = [&](auto slot, double(&v)[2][2]) {
     constexpr int u = decltype(slot)::value;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) *reinterpret_cast<half8 *>(&sm[(rbase + 64 * i) * LD + q8]) = ra[u][i];
-    // limbs of rows (2 rp, 2 rp + 1) of column 2 cp + q: one packed half2 per limb
     const int e = et[u];
-    double v[2][2] = {{ldexp(rx[u][0].x, -e), ldexp(rx[u][1].x, -e)}, {ldexp(rx[u][0].y, -e), ldexp(rx[u][1].y, -e)}};
+    v[0][0] = ldexp(rx[u][0].x, -e);
+    v[0][1] = ldexp(rx[u][1].x, -e);
+    v[1][0] = ldexp(rx[u][0].y, -e);
+    v[1][1] = ldexp(rx[u][1].y, -e);
+  };
+  auto split_limb = [&](int l, double(&v)[2][2]) {
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const double l0 = rint(v[q][0]), l1 = rint(v[q][1]);
+      hl[l][q] = int_to_half2(l0, l1);
+      v[q][0] = (v[q][0] - l0) * radix;
+      v[q][1] = (v[q][1] - l1) * radix;
+    }
+  };
+  auto sstore = [&]() {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) *reinterpret_cast<half8 *>(&sm[(rbase + 64 * i) * LD + q8]) = ra[i];
 #pragma unroll
     for (int l = 0; l < L; ++l)
 #pragma unroll
-      for (int q = 0; q < 2; ++q) {
-        const double l0 = rint(v[q][0]), l1 = rint(v[q][1]);
-        *reinterpret_cast<half2_ *>(&sm[(BM + l * BN + 2 * cp + q) * LD + 2 * rp]) = int_to_half2(l0, l1);
-        v[q][0] = (v[q][0] - l0) * radix;
-        v[q][1] = (v[q][1] - l1) * radix;
-      }
+      for (int q = 0; q < 2; ++q) *reinterpret_cast<half2_ *>(&sm[(BM + l * BN + 2 * cp + q) * LD + 2 * rp]) = hl[l][q];
   };
   auto opnd = [&](int row, int kk) -> half8 { return *reinterpret_cast<const half8 *>(&sm[row * LD + kk * 16 + lk]); };
 
@@ -231,24 +253,34 @@ __global__ void __launch_bounds__(512) k_split_g1(int KLp, int Kdim, int lda, in
 #pragma unroll
   for (int l = 0; l < L; ++l) acc[l] = (float16_){0};
   const int nst = krows / BK;
-  // one stage: store slot `cur` (stage s) to LDS, prefetch stage s + D - 1 into
-  // slot `nxt` (the slot stage s - 1 used; near the end the last stage is loaded
-  // again into that free slot, so the loop body has no branch and the waits
-  // before the LDS stores leave the later stages' loads outstanding), multiply,
-  // promote every 128 rows
-  auto step = [&](int s, auto cur, auto nxt) {
+  // one stage s: store its counts and limbs to LDS; load the counts of stage
+  // s + 1 and p of stage s + 3 into slot `cur` (stage s's p was split in the
+  // previous stage); multiply, splitting p of stage s + 1 (slot `nxt`) between
+  // the groups of multiplications; promote every 128 rows (odd s).  Past the end the last stage is loaded again, so
+  // the body has no branch (`odd` is a compile-time parity: a stage is one
+  // basic block, and the split can be scheduled beside the multiplications).
+  auto step = [&](int s, auto cur, auto nxt, auto odd) {
     if (s > 0) __syncthreads();
     const int e = et[decltype(cur)::value];
-    sstore(cur);
+    sstore();
     __syncthreads();
-    gload(min(s + D - 1, nst - 1), nxt);
+    gload_a(min(s + 1, nst - 1));
+    gload_x(min(s + 3, nst - 1), cur);
+    double v[2][2];
+    split_begin(nxt, v);
 #pragma unroll
     for (int kk = 0; kk < BK / 16; ++kk) {
       const half8 a = opnd(wave * 32 + lr, kk);
 #pragma unroll
       for (int l = 0; l < L; ++l) acc[l] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, opnd(BM + l * BN + lr, kk), acc[l], 0, 0, 0);
+      // a share of the next stage's limbs per group of multiplications; the
+      // scheduling barrier keeps it there (left alone, the scheduler sinks the
+      // split past the next barrier, to the LDS stores that consume it)
+#pragma unroll
+      for (int l = (kk * L) / (BK / 16); l < ((kk + 1) * L) / (BK / 16); ++l) split_limb(l, v);
+      __builtin_amdgcn_sched_barrier(0);
     }
-    if (s & 1) {  // 128 K rows done: promote the exact integers to FP64
+    if constexpr (decltype(odd)::value) {  // 128 K rows done: promote the exact integers to FP64
       double sc[L];
 #pragma unroll
       for (int l = 0; l < L; ++l) sc[l] = ldexp(1.0, e - bits * l);
@@ -267,17 +299,39 @@ __global__ void __launch_bounds__(512) k_split_g1(int KLp, int Kdim, int lda, in
   using U1 = std::integral_constant<int, 1>;
   using U2 = std::integral_constant<int, 2>;
   static_assert(D == 3, "the unrolled ring below has three slots");
-  gload(0, U0{});
-  gload(min(1, nst - 1), U1{});
+  using E0 = std::false_type;
+  using E1 = std::true_type;
+  gload_x(0, U0{});
+  gload_x(min(1, nst - 1), U1{});
+  gload_x(min(2, nst - 1), U2{});
+  gload_a(0);
+  {
+    double v[2][2];
+    split_begin(U0{}, v);
+#pragma unroll
+    for (int l = 0; l < L; ++l) split_limb(l, v);
+  }
+  // six stages per iteration (three ring slots, promotion every second stage):
+  // stage s0 + i uses slot i % 3 and has parity i % 2 since s0 is a multiple of 6
   int s0 = 0;
 #pragma unroll 1
-  for (; s0 + D <= nst; s0 += D) {
-    step(s0, U0{}, U2{});
-    step(s0 + 1, U1{}, U0{});
-    step(s0 + 2, U2{}, U1{});
+  for (; s0 + 6 <= nst; s0 += 6) {
+    step(s0, U0{}, U1{}, E0{});
+    step(s0 + 1, U1{}, U2{}, E1{});
+    step(s0 + 2, U2{}, U0{}, E0{});
+    step(s0 + 3, U0{}, U1{}, E1{});
+    step(s0 + 4, U1{}, U2{}, E0{});
+    step(s0 + 5, U2{}, U0{}, E1{});
   }
-  if (s0 < nst) step(s0, U0{}, U2{});  // the last nst mod 3 stages
-  if (s0 + 1 < nst) step(s0 + 1, U1{}, U0{});
+  // the last nst mod 6 stages (nst is even: K rows come in multiples of 128)
+  if (s0 < nst) {
+    step(s0, U0{}, U1{}, E0{});
+    step(s0 + 1, U1{}, U2{}, E1{});
+  }
+  if (s0 + 2 < nst) {
+    step(s0 + 2, U2{}, U0{}, E0{});
+    step(s0 + 3, U0{}, U1{}, E1{});
+  }
   double *T = Tpart + (int64_t)kz * KLp * ncols;
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
