@@ -536,8 +536,13 @@ static int output_law(bdc_ctx *c, int prec, const double *X, int scale_slot, boo
 
 // stage 1: Vbar rows (j in JL) and Wbar columns (j in JR) from L (FP64).
 // mirror: L is reversal-symmetric and n1 = n2, so Wbar is a permutation of Vbar;
-// only the first `rows` rows of Wbar are formed then.
-static int cross_operands(bdc_ctx *c, const double *L, bool mirror = false, int rows = 0) {
+// only the first `rows` rows of Wbar are formed then.  amax (may be null):
+// receives one maximum of |Vbar| per row block of the deletion-chain launch,
+// and *namax their number, when that launch formed Vbar and Wbar is its
+// permutation (else *namax = 0).
+static int cross_operands(bdc_ctx *c, const double *L, bool mirror = false, int rows = 0, double *amax = nullptr,
+                          size_t amax_cap = 0, int *namax = nullptr) {
+  if (namax) *namax = 0;
   const int k = c->k, N1 = c->N1, N2 = c->N2;
   // V_j = L_j B_{k-j}^T for all j in JL in one launch of factored
   // single-deletion steps (row y1 of L_j, 2^(k-j) entries, to 2^n2 entries)
@@ -558,9 +563,11 @@ static int cross_operands(bdc_ctx *c, const double *L, bool mirror = false, int 
     }
     if (jb.count <= kMaxChainJobs) {
       jb.row0[jb.count] = nrows;
-      const int rc = launch_chain_del(L, c->g2b.as<double>(), c->n2, jb, c->stream);
+      const bool fuse_max = amax && mirror && c->n1 == c->n2 && (size_t)nrows <= amax_cap;
+      const int rc = launch_chain_del(L, c->g2b.as<double>(), c->n2, jb, fuse_max ? amax : nullptr, c->stream);
       if (rc == BDC_ERR_HIP) return fail(rc, "chain_del launch failed");
       staged = rc == BDC_OK;
+      if (staged && fuse_max && namax) *namax = nrows;
     }
   }
   for (size_t i = 0; i < c->JL.size() && !staged; ++i) {
@@ -1298,11 +1305,14 @@ int split_output_law_g1(bdc_ctx *c, const double *X, int rows) {
 }
 
 int split_cross_operands(bdc_ctx *c, const double *L, bool mirror, int rows) {
-  BDC_TRY(cross_operands(c, L, mirror, rows));  // V, W in FP64 (stage GEMMs)
+  // V, W in FP64 (stage contractions); the deletion chain also leaves the block
+  // maxima of |V| for the limb exponent when W is a permutation of V
+  int namax = 0;
+  BDC_TRY(cross_operands(c, L, mirror, rows, c->part.as<double>(), c->part.bytes / 8, &namax));
   const Split &sp = c->split;
   BDC_LAUNCH(launch_split_vw(c->g2b.as<double>(), c->KL, c->KLp, c->N2, c->g2a.as<double>(), rows, c->KR, c->KRp,
                              sp.g2_limbs, sp.g2_bits, c->vlimb.as<_Float16>(), c->wlimb.as<_Float16>(),
-                             c->part.as<double>(), c->scal.as<double>(), c->stream));
+                             c->part.as<double>(), namax, c->scal.as<double>(), c->stream));
   return BDC_OK;
 }
 

@@ -124,7 +124,8 @@ struct ChainJobs {
   int64_t src_off[kMaxChainJobs], src_ld[kMaxChainJobs], dst_off[kMaxChainJobs], dst_ld[kMaxChainJobs];
 };
 int launch_chain_ins(const double *src, double *dst, int L, const ChainJobs &jobs, hipStream_t s);
-int launch_chain_del(const double *src, double *dst, int L, const ChainJobs &jobs, hipStream_t s);
+// amax (may be null): one double per block (row), the largest |entry| written
+int launch_chain_del(const double *src, double *dst, int L, const ChainJobs &jobs, double *amax, hipStream_t s);
 struct MirrorW {  // for each j in JR: column offset in Wbar, m = k - j, row offset of V_{k-j} in Vbar
   int count;
   int offR[18], m[18], offL[18];
@@ -135,8 +136,10 @@ int launch_w_from_v(const MirrorW &mw, const double *V, int N2, int n2, double *
 // ---- split-FP16 contractions (kernels_split.hip) ----
 // exponents of the 128 x 32 tiles of a P-like array (rows x1 < M, N2 columns), row-major by tile
 int launch_tile_exp(const double *P, int M, int N2, int bits, int32_t *texp, hipStream_t s);
+// np_pre > 0: part[0 .. np_pre) already holds block maxima of |V| and |W| (W a
+// permutation of entries of V), so the max pass over V and W is skipped
 int launch_split_vw(const double *V, int KL, int KLp, int N2, const double *W, int M, int KR, int KRp, int limbs,
-                    int bits, _Float16 *vT, _Float16 *wl, double *part, double *scal, hipStream_t s);
+                    int bits, _Float16 *vT, _Float16 *wl, double *part, int np_pre, double *scal, hipStream_t s);
 // ks partial slabs (ks * KLp * ncols FP64) of Tbar = aT X over the first Kdim rows of the FP64 array X
 // (row stride ncols; split into limbs on the fly with the tile exponents texp), aT row stride lda
 int launch_split_g1(int KLp, int Kdim, int lda, int ncols, const _Float16 *aT, const double *X, const int32_t *texp,

@@ -660,9 +660,12 @@ int launch_tile_exp(const double *P, int M, int N2, int bits, int32_t *texp, hip
 }
 
 int launch_split_vw(const double *V, int KL, int KLp, int N2, const double *W, int M, int KR, int KRp, int limbs,
-                    int bits, _Float16 *vT, _Float16 *wl, double *part, double *scal, hipStream_t s) {
-  const int np = 512;
-  hipLaunchKernelGGL(k_absmax_vw, dim3(np), dim3(256), 0, s, V, (int64_t)KL * N2, W, M, KR, KRp, part);
+                    int bits, _Float16 *vT, _Float16 *wl, double *part, int np_pre, double *scal, hipStream_t s) {
+  int np = np_pre;
+  if (np <= 0) {
+    np = 512;
+    hipLaunchKernelGGL(k_absmax_vw, dim3(np), dim3(256), 0, s, V, (int64_t)KL * N2, W, M, KR, KRp, part);
+  }
   hipLaunchKernelGGL(k_vw_exponent, dim3(1), dim3(256), 0, s, part, np, bits, scal);
   if (N2 % 64 || KLp % 64) return BDC_ERR_ARG;
   dim3 gv(N2 / 64, KLp / 64);

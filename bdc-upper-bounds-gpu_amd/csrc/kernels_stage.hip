@@ -207,9 +207,17 @@ __device__ inline int chain_job(const ChainJobs &jb, int b) {
   return i;
 }
 
+// (the two insertions at position 0 of v are the adjacent entries 2v, 2v + 1:
+// one 16-byte LDS load, where two 8-byte loads at a stride of 16 bytes would
+// conflict)
+__device__ inline void add_pos0(double &a, const double *row, uint32_t v) {
+  const double2 t = *reinterpret_cast<const double2 *>(row + 2 * v);
+  a += t.x;
+  a += t.y;
+}
 __global__ void __launch_bounds__(1024) k_chain_ins(const double *__restrict__ src, double *__restrict__ dst, int L,
                                                     ChainJobs jb) {
-  extern __shared__ double lds[];
+  extern __shared__ __attribute__((aligned(16))) double lds[];
   const int i = chain_job(jb, blockIdx.x), y = blockIdx.x - jb.row0[i], d = jb.d[i];
   const double *in = src + jb.src_off[i] + (int64_t)y * jb.src_ld[i];
   double *out = dst + jb.dst_off[i] + (int64_t)y * jb.dst_ld[i];
@@ -232,11 +240,13 @@ __global__ void __launch_bounds__(1024) k_chain_ins(const double *__restrict__ s
     for (uint32_t v = tid; v < H; v += bd) {
       double a = h ? acc[v] : 0.0;
       a += half[v];
-      if (L >= 2 && (v >> (L - 2)) == h)
-        for (int p = 0; p < L - 1; ++p) {
+      if (L >= 2 && (v >> (L - 2)) == h) {
+        add_pos0(a, half, v - h * (H >> 1));  // ins_bit(v, 0, b) - h H = 2 (v - h H / 2) + b
+        for (int p = 1; p < L - 1; ++p) {
           a += half[ins_bit(v, p, 0u) - h * H];
           a += half[ins_bit(v, p, 1u) - h * H];
         }
+      }
       acc[v] = a;
     }
   }
@@ -251,7 +261,8 @@ __global__ void __launch_bounds__(1024) k_chain_ins(const double *__restrict__ s
     const uint32_t nout = 1u << (l - 1);
     for (uint32_t v = tid; v < nout; v += bd) {
       double a = 0.0;
-      for (int p = 0; p < l; ++p) {
+      add_pos0(a, cur, v);
+      for (int p = 1; p < l; ++p) {
         a += cur[ins_bit(v, p, 0u)];
         a += cur[ins_bit(v, p, 1u)];
       }
@@ -267,36 +278,48 @@ __global__ void __launch_bounds__(1024) k_chain_ins(const double *__restrict__ s
   }
 }
 
+// amax (may be null): amax[block] = max |out| over the row the block writes
 __global__ void __launch_bounds__(1024) k_chain_del(const double *__restrict__ src, double *__restrict__ dst, int L,
-                                                    ChainJobs jb) {
+                                                    ChainJobs jb, double *__restrict__ amax) {
   extern __shared__ double lds[];
+  __shared__ double red[16];
   const int i = chain_job(jb, blockIdx.x), y = blockIdx.x - jb.row0[i], d = jb.d[i], m = L - d;
   const double *in = src + jb.src_off[i] + (int64_t)y * jb.src_ld[i];
   double *out = dst + jb.dst_off[i] + (int64_t)y * jb.dst_ld[i];
   const uint32_t tid = threadIdx.x, bd = blockDim.x;
+  double mx = 0.0;
   if (d == 0) {
-    for (uint32_t t = tid; t < (1u << L); t += bd) out[t] = in[t];
-    return;
-  }
-  const double f = factorial(d);
-  double *cur = lds, *nxt = lds + (1u << (L - 1));  // each region holds up to 2^(L-1) entries
-  for (uint32_t t = tid; t < (1u << m); t += bd) cur[t] = in[t];
-  __syncthreads();
-  int l = m;  // bits of cur
-  for (int s = 1; s <= d; ++s, ++l) {
-    const uint32_t nout = 1u << (l + 1);
-    for (uint32_t u = tid; u < nout; u += bd) {
-      double a = 0.0;
-      for (int q = 0; q <= l; ++q) a += cur[del_bit(u, q)];
-      if (s == d)
-        out[u] = a / f;
-      else
-        nxt[u] = a;
+    for (uint32_t t = tid; t < (1u << L); t += bd) {
+      out[t] = in[t];
+      mx = fmax(mx, fabs(in[t]));
     }
-    double *t = cur;
-    cur = nxt;
-    nxt = t;
+  } else {
+    const double f = factorial(d);
+    double *cur = lds, *nxt = lds + (1u << (L - 1));  // each region holds up to 2^(L-1) entries
+    for (uint32_t t = tid; t < (1u << m); t += bd) cur[t] = in[t];
     __syncthreads();
+    int l = m;  // bits of cur
+    for (int s = 1; s <= d; ++s, ++l) {
+      const uint32_t nout = 1u << (l + 1);
+      for (uint32_t u = tid; u < nout; u += bd) {
+        double a = 0.0;
+        for (int q = 0; q <= l; ++q) a += cur[del_bit(u, q)];
+        if (s == d) {
+          out[u] = a / f;
+          mx = fmax(mx, fabs(a / f));
+        } else {
+          nxt[u] = a;
+        }
+      }
+      double *t = cur;
+      cur = nxt;
+      nxt = t;
+      __syncthreads();
+    }
+  }
+  if (amax) {  // uniform: every thread of the block takes it
+    mx = block_max(mx, red);
+    if (tid == 0) amax[blockIdx.x] = mx;
   }
 }
 
@@ -311,9 +334,10 @@ int launch_chain_ins(const double *src, double *dst, int L, const ChainJobs &job
   hipLaunchKernelGGL(k_chain_ins, dim3(jobs.row0[jobs.count]), dim3(1024), sizeof(double) << L, s, src, dst, L, jobs);
   return hipGetLastError() == hipSuccess ? BDC_OK : BDC_ERR_HIP;
 }
-int launch_chain_del(const double *src, double *dst, int L, const ChainJobs &jobs, hipStream_t s) {
+int launch_chain_del(const double *src, double *dst, int L, const ChainJobs &jobs, double *amax, hipStream_t s) {
   if (chain_check(L, jobs) != BDC_OK) return BDC_ERR_ARG;
-  hipLaunchKernelGGL(k_chain_del, dim3(jobs.row0[jobs.count]), dim3(1024), sizeof(double) << L, s, src, dst, L, jobs);
+  hipLaunchKernelGGL(k_chain_del, dim3(jobs.row0[jobs.count]), dim3(1024), sizeof(double) << L, s, src, dst, L, jobs,
+                     amax);
   return hipGetLastError() == hipSuccess ? BDC_OK : BDC_ERR_HIP;
 }
 
