@@ -16,6 +16,13 @@ namespace bdc {
 
 static thread_local std::string g_err;
 
+// bdc_iterate with a stopping rule: steps per batch, and the largest updated
+// part of P (bytes) for which batching (and its snapshot copy) is used.  A
+// round trip per step costs tens of microseconds; copying 256 MiB costs
+// about 0.1 ms, once per batch.
+constexpr int kStopBatch = 32;
+constexpr size_t kStopBatchMaxBytes = (size_t)256 << 20;
+
 void set_error(const char *fmt, ...) {
   char buf[1024];
   va_list ap;
@@ -363,7 +370,7 @@ int bdc_destroy(bdc_ctx *c) {
                          &c->entA, &c->rowptrB, &c->entB, &c->blkA, &c->blkB, &c->restiles[0], &c->restiles[1], &c->g1part, &c->repmask, &c->wordbase, &c->orbits, &c->P,
                          &c->Hc, &c->T, &c->U, &c->qpart, &c->qv, &c->g2a, &c->g2b, &c->splitk, &c->part, &c->scal, &c->exptab,
                          &c->hist, &c->ptexp, &c->vlimb, &c->wlimb, &c->theta, &c->adam_m,
-                         &c->adam_v, &c->grad, &c->dvec})
+                         &c->adam_v, &c->grad, &c->dvec, &c->ck_p, &c->ck_s, &c->ck_t})
     b->release();
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -808,14 +815,56 @@ int bdc_iterate(bdc_ctx *c, int prec, int iters, double tol, bdc_iter_stats *sta
   BDC_TRY(ensure_hist(c, std::max(iters, 1)));
   const int rows = ba_rows(c, prec);
   int ran = 0;
-  for (int it = 0; it < iters; ++it) {
-    BDC_TRY(ba_step(c, prec, it, rows));
-    ++ran;
-    if (tol > 0.0) {
-      double mx = 0.0;
-      BDC_HIP(hipMemcpyAsync(&mx, c->hist.as<double>() + 4 * it, 8, hipMemcpyDeviceToHost, c->stream));
+  if (tol <= 0.0) {
+    for (int it = 0; it < iters; ++it) BDC_TRY(ba_step(c, prec, it, rows));
+    ran = iters;
+  } else {
+    // The stopping rule (main_BAA.cu:73: the loop runs while mx >= a) ends at
+    // the first step with mx < tol.  Reading mx back after every step costs a
+    // host round trip per step, which dominates small problems; there, steps
+    // run in batches from a snapshot of the step state, and a batch that
+    // crosses the stopping point is rerun from the snapshot up to that step.
+    // No reduction uses atomics, so the rerun is bitwise the same.
+    const size_t pbytes = (size_t)rows * c->N2 * 8;
+    const bool split = prec == BDC_PREC_SPLIT16;
+    const int batch = pbytes <= kStopBatchMaxBytes ? kStopBatch : 1;
+    std::vector<double> h;
+    while (ran < iters) {
+      const int nb = std::min(batch, iters - ran);
+      const bool snap = nb > 1;
+      bool limb_valid = c->plimb_valid;
+      int limb_rows = c->plimb_rows;
+      if (snap) {
+        if (c->ck_p.bytes < pbytes) BDC_TRY(c->ck_p.alloc(pbytes));
+        if (c->ck_s.bytes < c->scal.bytes) BDC_TRY(c->ck_s.alloc(c->scal.bytes));
+        BDC_HIP(hipMemcpyAsync(c->ck_p.p, c->P.p, pbytes, hipMemcpyDeviceToDevice, c->stream));
+        BDC_HIP(hipMemcpyAsync(c->ck_s.p, c->scal.p, c->scal.bytes, hipMemcpyDeviceToDevice, c->stream));
+        if (split) {
+          if (c->ck_t.bytes < c->ptexp.bytes) BDC_TRY(c->ck_t.alloc(c->ptexp.bytes));
+          BDC_HIP(hipMemcpyAsync(c->ck_t.p, c->ptexp.p, c->ptexp.bytes, hipMemcpyDeviceToDevice, c->stream));
+        }
+      }
+      for (int it = 0; it < nb; ++it) BDC_TRY(ba_step(c, prec, ran + it, rows));
+      h.resize((size_t)4 * nb);
+      BDC_HIP(hipMemcpyAsync(h.data(), c->hist.as<double>() + 4 * ran, h.size() * 8, hipMemcpyDeviceToHost, c->stream));
       BDC_HIP(hipStreamSynchronize(c->stream));
-      if (mx < tol) break;  // main_BAA.cu:73 (loop runs while mx >= a)
+      int stop = -1;
+      for (int it = 0; it < nb && stop < 0; ++it)
+        if (h[4 * it] < tol) stop = it;
+      if (stop < 0) {
+        ran += nb;
+        continue;
+      }
+      if (stop < nb - 1) {  // the batch ran past the stopping point
+        BDC_HIP(hipMemcpyAsync(c->P.p, c->ck_p.p, pbytes, hipMemcpyDeviceToDevice, c->stream));
+        BDC_HIP(hipMemcpyAsync(c->scal.p, c->ck_s.p, c->scal.bytes, hipMemcpyDeviceToDevice, c->stream));
+        if (split) BDC_HIP(hipMemcpyAsync(c->ptexp.p, c->ck_t.p, c->ptexp.bytes, hipMemcpyDeviceToDevice, c->stream));
+        c->plimb_valid = limb_valid;
+        c->plimb_rows = limb_rows;
+        for (int it = 0; it <= stop; ++it) BDC_TRY(ba_step(c, prec, ran + it, rows));
+      }
+      ran += stop + 1;
+      break;
     }
   }
   if (ran && rows < c->N1) BDC_LAUNCH(launch_reflect_half(c->P.as<double>(), c->nx, c->stream));

@@ -138,6 +138,10 @@ struct bdc_ctx {
   bdc::DevBuf exptab; // 2^(j/32), j < 32, for the update's exp
   bdc::DevBuf hist;   // per-iteration stats (device)
   int hist_cap = 0;
+  // snapshot of what a BA step changes (the rows of P it updates, the device
+  // scalars, the P tile exponents), taken before a batch of steps that a
+  // stopping rule may cut short
+  bdc::DevBuf ck_p, ck_s, ck_t;
 
   // split-FP16 buffers
   bdc::Split split;

@@ -98,7 +98,11 @@ int bdc_set_p_dirichlet(bdc_ctx *ctx, double alpha, uint64_t seed);
 /* ---- iteration: baa_iteration (BAA.cu:590-639) + compute_max_deviation (:916-946) ---- */
 /* Runs `iters` BA steps at the given precision; stats (may be NULL) receives one
  * record per step.  If tol > 0 it stops early after the first step with
- * mx_bits < tol (main_BAA.cu:73) and *done receives the steps run. */
+ * mx_bits < tol (main_BAA.cu:73) and *done receives the steps run.  With
+ * tol > 0 on small problems the steps run in batches from a device snapshot
+ * of the step state (up to 256 MiB, kept by the context) and the batch that
+ * crosses the stopping point is rerun up to it: the result is bitwise that
+ * of one step per call. */
 int bdc_iterate(bdc_ctx *ctx, int precision, int iters, double tol, bdc_iter_stats *stats, int *done);
 
 /* ---- bounds: compute_all_log_D + compute_rate (BAA.cu:642-724, 836-913) ---- */

@@ -421,6 +421,39 @@ def test_dual_refinement_beats_ba_per_pass(bdc, oracle, n, k):
 
 
 @pytest.mark.parametrize("n,k", [(16, 8), (20, 10)])
+def test_stopping_rule_batches_match_single_steps(bdc, n, k):
+    """With a stopping rule (main_BAA.cu:73: the loop runs while mx >= a)
+    bdc_iterate runs steps in batches from a snapshot of the step state and
+    reruns a batch that crosses the stopping point.  It must stop at the same
+    step, with bitwise the same statistics and p, as one call per step and as
+    the same number of steps without the rule."""
+    say(f"stopping rule n={n} k={k}")
+    with bdc.Engine(n, k) as e:
+        for prec in (bdc.PREC_FP64, bdc.PREC_SPLIT16):
+            e.set_p_dirichlet(1.0, 5)
+            mx = e.iterate(90, prec)[:, 0]
+            # the first new running minimum after step 40 that is not the last
+            # step of a batch of 32 (so the batch is rerun up to it): tol
+            # between it and the minimum before it
+            run_min = np.minimum.accumulate(mx)
+            stop = next((i for i in range(40, 90) if i % 32 != 31 and mx[i] < run_min[i - 1]), None)
+            assert stop is not None, mx
+            tol = 0.5 * (mx[stop] + run_min[stop - 1])
+            e.set_p_dirichlet(1.0, 5)
+            batched = e.iterate(90, prec, tol=tol)
+            p_batched = e.get_p_orbits()
+            e.set_p_dirichlet(1.0, 5)
+            single = np.concatenate([e.iterate(1, prec, tol=tol) for _ in range(stop + 1)])
+            p_single = e.get_p_orbits()
+            e.set_p_dirichlet(1.0, 5)
+            free = e.iterate(stop + 1, prec)
+            p_free = e.get_p_orbits()
+            assert len(batched) == stop + 1, prec
+            assert np.array_equal(batched, single) and np.array_equal(batched, free), prec
+            assert np.array_equal(p_batched, p_single) and np.array_equal(p_batched, p_free), prec
+
+
+@pytest.mark.parametrize("n,k", [(16, 8), (20, 10)])
 def test_runs_are_bitwise_deterministic(bdc, n, k):
     """No reduction uses atomics (block partials are combined in a fixed order),
     so two runs of the same steps give bitwise identical statistics, p and
