@@ -113,6 +113,29 @@ def test_kl_general_q(bdc, oracle, n, k):
     np.testing.assert_allclose(d, oracle.full_kl(n, k, np.log(q)), rtol=0, atol=1e-12)
 
 
+@pytest.mark.parametrize("n,k", [(14, 1), (15, 1), (14, 2), (15, 2)])
+def test_small_k_matches_oracle(bdc, oracle, n, k):
+    """The narrowest output spaces (2 and 4 outputs), where the stage and
+    entropy kernels run their smallest levels: per-step mx, p and both bounds
+    against the oracle at both precisions, and for k = 1 the closed form
+    C_{n,1} = 1 bit (Upper_Bounds.txt:6,42) between I(p) and max_x D(x||q)."""
+    say(f"small k n={n} k={k}")
+    for prec, tol in ((bdc.PREC_FP64, 1e-10), (bdc.PREC_SPLIT16, 1e-9)):
+        ref = oracle.BA(n, k)
+        with bdc.Engine(n, k) as e:
+            e.set_p_uniform()
+            stats = e.iterate(12, prec)
+            for t in range(12):
+                mx = ref.iterate()
+                assert abs(stats[t, 0] - mx) < tol, (prec, t, stats[t, 0], mx)
+            np.testing.assert_allclose(e.get_p_orbits(), ref.get_q(), rtol=tol)
+            cert = e.certify()
+        assert abs(cert["lower"] - ref.rate()) / LN2 < tol
+        assert abs(cert["upper_cert"] - ref.max_kl()) / LN2 < tol
+        if k == 1:
+            assert cert["lower"] / LN2 <= 1.0 + 1e-12 and cert["upper_cert"] / LN2 >= 1.0 - 1e-12
+
+
 @pytest.mark.parametrize("n,k,iters", [(16, 8, 25), (13, 6, 15), (12, 6, 15), (18, 9, 4)])
 def test_ba_fp64_matches_oracle(bdc, oracle, n, k, iters):
     say(f"ba fp64 n={n} k={k} iters={iters}")
