@@ -267,7 +267,8 @@ PUBLISHED_UB = {
          11: 3.9247, 12: 4.2841, 13: 4.6727, 14: 5.0930, 15: 5.5573, 16: 6.0713, 17: 6.6464, 18: 7.2964,
          19: 8.0364, 20: 8.8850},
     31: {1: 1.0000, 2: 1.1888, 3: 1.5136, 4: 1.8086, 5: 2.0831, 6: 2.3632, 7: 2.6458, 8: 2.9332, 9: 3.2298, 10: 3.5384,
-         11: 3.8605, 12: 4.2005, 13: 4.5623, 14: 4.9511, 15: 5.3895, 16: 5.8826, 17: 6.3947, 18: 6.9592},
+         11: 3.8605, 12: 4.2005, 13: 4.5623, 14: 4.9511, 15: 5.3895, 16: 5.8826, 17: 6.3947, 18: 6.9592,
+         19: 8.3882, 20: 9.1247},
 }
 
 
