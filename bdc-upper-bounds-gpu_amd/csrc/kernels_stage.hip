@@ -171,10 +171,46 @@ __global__ void k_w_from_v(MirrorW mw, const double *__restrict__ V, int N2, int
   const uint32_t rz = rev_bits((uint32_t)z, m), rx = rev_bits((uint32_t)x1, n2);
   W[(int64_t)x1 * KRp + kap] = V[(int64_t)(mw.offL[s] + rz) * N2 + rx];
 }
+// The same copy tiled: a block covers 64 columns kappa' of Wbar and 32 of its
+// rows, taken as x1 = rev(c << sh) for 32 consecutive c (rows = 2^(n2 - sh)), so
+// that rev x1 = c << sh: the Vbar reads are 32 entries of one row (stride 2^sh)
+// and the Wbar writes 64-entry row segments, transposed through LDS.  (The
+// one-thread-per-entry form reads one Vbar row per entry.)
+__global__ void __launch_bounds__(256) k_w_from_v_tiled(MirrorW mw, const double *__restrict__ V, int N2, int n2,
+                                                        double *__restrict__ W, int KR, int KRp, int sh) {
+  __shared__ double t[64][33];
+  const int k0 = blockIdx.x * 64, c0 = blockIdx.y * 32, tid = threadIdx.x;
+  const int cl = tid & 31;
+  for (int i = 0; i < 8; ++i) {
+    const int kl = (tid >> 5) + 8 * i, kap = k0 + kl;
+    if (kap < KR) {
+      int s = 0;
+      while (s + 1 < mw.count && kap >= mw.offR[s + 1]) ++s;
+      const uint32_t rz = rev_bits((uint32_t)(kap - mw.offR[s]), mw.m[s]);
+      t[kl][cl] = V[(int64_t)(mw.offL[s] + rz) * N2 + ((uint32_t)(c0 + cl) << sh)];
+    }
+  }
+  __syncthreads();
+  const int kl = tid & 63, kap = k0 + kl;
+  if (kap < KR)
+    for (int i = 0; i < 8; ++i) {
+      const int r = (tid >> 6) + 4 * i;
+      const uint32_t x1 = rev_bits((uint32_t)(c0 + r) << sh, n2);
+      W[(int64_t)x1 * KRp + kap] = t[kl][r];
+    }
+}
 int launch_w_from_v(const MirrorW &mw, const double *V, int N2, int n2, double *W, int N1, int KR, int KRp,
                     hipStream_t s) {
-  const int64_t tot = (int64_t)N1 * KR;
-  hipLaunchKernelGGL(k_w_from_v, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, mw, V, N2, n2, W, N1, KR, KRp);
+  // N1: rows of Wbar formed (all, or the half space x1 < 2^(n2 - 1))
+  int b = 0;
+  while (b < 31 && (1 << b) < N1) ++b;
+  if ((1 << b) == N1 && N1 >= 32 && b <= n2 && N2 == (1 << n2)) {
+    hipLaunchKernelGGL(k_w_from_v_tiled, dim3((unsigned)((KR + 63) / 64), (unsigned)(N1 / 32)), dim3(256), 0, s, mw, V,
+                       N2, n2, W, KR, KRp, n2 - b);
+  } else {
+    const int64_t tot = (int64_t)N1 * KR;
+    hipLaunchKernelGGL(k_w_from_v, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, mw, V, N2, n2, W, N1, KR, KRp);
+  }
   return hipGetLastError() == hipSuccess ? BDC_OK : BDC_ERR_HIP;
 }
 
