@@ -95,24 +95,35 @@ __global__ void __launch_bounds__(256) k_q_combine(QCombine qc, double inv_c, co
   const uint32_t y = blockIdx.x * blockDim.x + threadIdx.x;
   double lq = 0.0;
   if (y < ny) {
+    // Slice sums in a fixed order.  The loops have no branch in their bodies
+    // and are unrolled, so the slice loads are issued ahead of the adds (one
+    // grid of 2^k threads has too few waves to hide a load per add); in the
+    // mirror loop an unselected slice adds +0.0, which leaves the sum of
+    // nonnegative terms (from +0.0) bitwise unchanged.
     double s = 0.0;
-    for (int i = 0; i < qc.nL; ++i) {
-      s += qc.qL[(int64_t)i * ny + y];
-      if (qc.comp) s += qc.qL[(int64_t)i * ny + (y ^ cm)];
-    }
-    if (qc.mirror) {
-      const uint32_t ry = rev_bits(y, qc.k);
-      for (int i = 0; i < qc.nL; ++i)
-        if ((qc.mirrorSel >> i) & 1ull) {
-          s += qc.qL[(int64_t)i * ny + ry];
-          if (qc.comp) s += qc.qL[(int64_t)i * ny + (ry ^ cm)];
+    auto slices = [&](const double *__restrict__ base, int n, uint32_t yy, uint64_t sel) {
+      if (qc.comp) {
+#pragma unroll 4
+        for (int i = 0; i < n; ++i) {
+          const bool on = (sel >> i) & 1ull;
+          const double a = base[(int64_t)i * ny + yy], b = base[(int64_t)i * ny + (yy ^ cm)];
+          s += on ? a : 0.0;
+          s += on ? b : 0.0;
         }
-    } else {
-      for (int i = 0; i < qc.nR; ++i) {
-        s += qc.qR[(int64_t)i * ny + y];
-        if (qc.comp) s += qc.qR[(int64_t)i * ny + (y ^ cm)];
+      } else {
+#pragma unroll 4
+        for (int i = 0; i < n; ++i) {
+          const bool on = (sel >> i) & 1ull;
+          const double a = base[(int64_t)i * ny + yy];
+          s += on ? a : 0.0;
+        }
       }
-    }
+    };
+    slices(qc.qL, qc.nL, y, ~0ull);
+    if (qc.mirror)
+      slices(qc.qL, qc.nL, rev_bits(y, qc.k), qc.mirrorSel);
+    else
+      slices(qc.qR, qc.nR, y, ~0ull);
     const double q = s * f;
     qout[y] = q;
     lq = log(q);
