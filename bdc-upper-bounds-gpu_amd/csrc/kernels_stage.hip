@@ -337,26 +337,43 @@ __global__ void __launch_bounds__(1024) k_chain_del(const double *__restrict__ s
     int l = m;  // bits of cur
     for (int s = 1; s <= d; ++s, ++l) {
       const uint32_t nout = 1u << (l + 1);
-      for (uint32_t u = tid; u < nout; u += bd) {
-        // the l + 1 terms in order, loaded four at a time ahead of their adds
-        // (one accumulator: the same sum as term by term, without an LDS
-        // round trip per term)
-        double a = 0.0;
+      for (uint32_t u = tid; u < nout; u += 2 * bd) {
+        // two outputs per pass, u and v = u + bd (v repeats u, unstored, past
+        // the end), each with its l + 1 terms in order in its own accumulator,
+        // loaded four at a time ahead of the adds: the same sums as term by
+        // term, with eight LDS loads in flight instead of a round trip per term
+        const uint32_t v = u + bd < nout ? u + bd : u;
+        double a = 0.0, b = 0.0;
         int q = 0;
         for (; q + 3 <= l; q += 4) {
           const double t0 = cur[del_bit(u, q)], t1 = cur[del_bit(u, q + 1)], t2 = cur[del_bit(u, q + 2)],
                        t3 = cur[del_bit(u, q + 3)];
+          const double w0 = cur[del_bit(v, q)], w1 = cur[del_bit(v, q + 1)], w2 = cur[del_bit(v, q + 2)],
+                       w3 = cur[del_bit(v, q + 3)];
           a += t0;
           a += t1;
           a += t2;
           a += t3;
+          b += w0;
+          b += w1;
+          b += w2;
+          b += w3;
         }
-        for (; q <= l; ++q) a += cur[del_bit(u, q)];
+        for (; q <= l; ++q) {
+          const double t0 = cur[del_bit(u, q)], w0 = cur[del_bit(v, q)];
+          a += t0;
+          b += w0;
+        }
         if (s == d) {
           out[u] = a / f;
           mx = fmax(mx, fabs(a / f));
+          if (v != u) {
+            out[v] = b / f;
+            mx = fmax(mx, fabs(b / f));
+          }
         } else {
           nxt[u] = a;
+          if (v != u) nxt[v] = b;
         }
       }
       double *t = cur;
