@@ -42,3 +42,26 @@ def test_supported_validation_without_gpu():
     assert b"[2, 32]" in L.bdc_last_error()
     assert L.bdc_supported(32, 21) == 0
     assert L.bdc_version() >= 1
+
+
+def test_argument_errors_without_gpu():
+    """Every entry point checks its arguments before any GPU call: a status
+    code and a message, never an exit (the reference exits, BAA.cuh:5-13)."""
+    import bdc_gpu
+    L = bdc_gpu.lib()
+    ctx = ctypes.c_void_p(1234)
+    assert L.bdc_create(33, 16, None, ctypes.byref(ctx)) == bdc_gpu.ERR_ARG
+    assert ctx.value is None  # cleared on failure
+    assert b"n must be in [2, 32]" in L.bdc_last_error()
+    assert L.bdc_create(28, 21, None, ctypes.byref(ctx)) == bdc_gpu.ERR_ARG
+    assert b"k <= 20" in L.bdc_last_error()
+    assert L.bdc_create(28, 14, None, None) == bdc_gpu.ERR_ARG
+    assert L.bdc_iterate(None, bdc_gpu.PREC_FP64, 1, 0.0, None, None) == bdc_gpu.ERR_ARG
+    assert L.bdc_certify(None, None) == bdc_gpu.ERR_ARG
+    assert L.bdc_set_p_uniform(None) == bdc_gpu.ERR_ARG
+    assert L.bdc_set_p_dirichlet(None, 1.0, 0) == bdc_gpu.ERR_ARG
+    u = ctypes.c_double()
+    theta = (ctypes.c_double * 4)()
+    assert L.bdc_dual(None, bdc_gpu.PREC_FP64, theta, 64.0, ctypes.byref(u), None) == bdc_gpu.ERR_ARG
+    assert L.bdc_refine_dual(None, bdc_gpu.PREC_FP64, theta, 64.0, 1, 1e-3, None) == bdc_gpu.ERR_ARG
+    assert L.bdc_destroy(None) == bdc_gpu.OK
