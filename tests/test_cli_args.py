@@ -46,3 +46,15 @@ def test_generator_usage(tmp_path):
     assert r.returncode == 1 and "Usage" in r.stderr
     r = run("generate_bit_transition_cache", ["1", "2"], tmp_path)
     assert r.returncode == 1 and "Usage" in r.stderr
+
+
+def test_bench_rejects_several_gpus():
+    """bench.py runs on one GPU; --gpus above 1 exits with a message (and a
+    JSON line saying so) before any GPU work."""
+    import json
+    import sys
+    repo = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(repo, "bench.py"), "--gpus", "2", "--steps", "1", "--warmup", "0"],
+                       cwd=repo, capture_output=True, text=True, timeout=300)
+    assert r.returncode != 0
+    assert "one GPU" in json.loads(r.stdout.strip().splitlines()[-1])["error"]
