@@ -1,6 +1,8 @@
 // C ABI of the engine (include/bdc.h): context lifetime, caches, distribution
 // I/O, the BA step pipeline and the bounds.  Every entry point returns a
 // status; nothing exits.
+#include <sys/stat.h>
+
 #include <algorithm>
 #include <cmath>
 #include <cstdarg>
@@ -108,27 +110,98 @@ static int timer_end(bdc_ctx *c, Timer &t) {
 }
 
 // ------------------------------------------------------------- caches
+// cache_io.h:7-14: <dir>/cache_<n'>_<k'>
+static std::string cache_path(const char *dir, int len, int m) {
+  return std::string(dir) + "/cache_" + std::to_string(len) + "_" + std::to_string(m);
+}
+
+// Entries per chunk when a cache file is scanned or converted (64 MiB of u32).
+constexpr size_t kCacheChunk = (size_t)1 << 24;
+
+// The reference loads whatever bytes the file holds (cache_io.h:29-44).  Here a
+// file must exist, hold exactly 2^(n'+k') raw uint32 counts (cache_io.h:17-19)
+// and every count must fit the u16 device layout; checked before any device
+// work, in chunks (cache_16_15 alone is 8 GiB).
+static int validate_cache_file(const char *dir, int len, int m) {
+  const std::string path = cache_path(dir, len, m);
+  struct stat st;
+  if (stat(path.c_str(), &st) != 0) return fail(BDC_ERR_IO, "cannot open %s", path.c_str());
+  const size_t cnt = (size_t)1 << (len + m);
+  if ((uint64_t)st.st_size != cnt * sizeof(uint32_t))
+    return fail(BDC_ERR_IO, "%s: %lld bytes, expected %zu uint32 counts (%zu bytes)", path.c_str(),
+                (long long)st.st_size, cnt, cnt * sizeof(uint32_t));
+  FILE *f = fopen(path.c_str(), "rb");
+  if (!f) return fail(BDC_ERR_IO, "cannot open %s", path.c_str());
+  std::vector<uint32_t> h(std::min(cnt, kCacheChunk));
+  for (size_t done = 0; done < cnt;) {
+    const size_t want = std::min(h.size(), cnt - done);
+    if (fread(h.data(), sizeof(uint32_t), want, f) != want) {
+      fclose(f);
+      return fail(BDC_ERR_IO, "%s: short read at count %zu", path.c_str(), done);
+    }
+    for (size_t i = 0; i < want; ++i)
+      if (h[i] > 65535u) {
+        fclose(f);
+        return fail(BDC_ERR_IO, "%s: count %u at index %zu does not fit the u16 device layout", path.c_str(), h[i],
+                    done + i);
+      }
+    done += want;
+  }
+  fclose(f);
+  return BDC_OK;
+}
+
+// Half-length caches a context for (n, k) reads: cache_{n1,j} for the splits j
+// (j < n1) and cache_{n2,k-j} (k - j < n2); the identity factors j = n1 and
+// k - j = n2 are implicit (backend/BAA.cu:371-384).
+static std::vector<std::pair<int, int>> needed_caches(int n, int k) {
+  const int n1 = (n + 1) / 2, n2 = n - n1;
+  std::vector<std::pair<int, int>> out;
+  auto add = [&](int len, int m) {
+    for (auto &p : out)
+      if (p.first == len && p.second == m) return;
+    out.push_back({len, m});
+  };
+  for (int j = std::max(0, k - n2); j <= std::min(k, n1); ++j) {
+    if (j < n1) add(n1, j);
+    if (k - j < n2) add(n2, k - j);
+  }
+  return out;
+}
+
 static int make_cache_u16(bdc_ctx *c, int len, int m, DevBuf &dst, const char *dir) {
   BDC_TRY(dst.alloc(sizeof(uint16_t) << (len + m)));
   if (dir && dir[0]) {
-    // cache_io.h:7-14 name, :29-44 raw uint32 content; size is validated here
-    std::string path = std::string(dir) + "/cache_" + std::to_string(len) + "_" + std::to_string(m);
+    // validated by bdc_create before any device work; converted chunk by chunk
+    const std::string path = cache_path(dir, len, m);
     FILE *f = fopen(path.c_str(), "rb");
     if (!f) return fail(BDC_ERR_IO, "cannot open %s", path.c_str());
     const size_t cnt = (size_t)1 << (len + m);
-    std::vector<uint32_t> h(cnt);
-    const size_t got = fread(h.data(), sizeof(uint32_t), cnt, f);
-    const int extra = fgetc(f);
-    fclose(f);
-    if (got != cnt || extra != EOF)
-      return fail(BDC_ERR_IO, "%s: expected %zu uint32 counts, file size differs", path.c_str(), cnt);
-    std::vector<uint16_t> h16(cnt);
-    for (size_t i = 0; i < cnt; ++i) {
-      if (h[i] > 65535u) return fail(BDC_ERR_IO, "%s: count %u does not fit the u16 device layout", path.c_str(), h[i]);
-      h16[i] = (uint16_t)h[i];
+    std::vector<uint32_t> h(std::min(cnt, kCacheChunk));
+    std::vector<uint16_t> h16(h.size());
+    for (size_t done = 0; done < cnt;) {
+      const size_t want = std::min(h.size(), cnt - done);
+      if (fread(h.data(), sizeof(uint32_t), want, f) != want) {
+        fclose(f);
+        return fail(BDC_ERR_IO, "%s: short read at count %zu", path.c_str(), done);
+      }
+      for (size_t i = 0; i < want; ++i) {
+        if (h[i] > 65535u) {
+          fclose(f);
+          return fail(BDC_ERR_IO, "%s: count %u does not fit the u16 device layout", path.c_str(), h[i]);
+        }
+        h16[i] = (uint16_t)h[i];
+      }
+      const hipError_t e = hipMemcpyAsync(dst.as<uint16_t>() + done, h16.data(), want * sizeof(uint16_t),
+                                          hipMemcpyHostToDevice, c->stream);
+      const hipError_t e2 = e == hipSuccess ? hipStreamSynchronize(c->stream) : e;  // h16 is reused
+      if (e2 != hipSuccess) {
+        fclose(f);
+        return fail(BDC_ERR_HIP, "cache upload: %s", hipGetErrorString(e2));
+      }
+      done += want;
     }
-    BDC_HIP(hipMemcpyAsync(dst.p, h16.data(), cnt * sizeof(uint16_t), hipMemcpyHostToDevice, c->stream));
-    BDC_HIP(hipStreamSynchronize(c->stream));
+    fclose(f);
     return BDC_OK;
   }
   BDC_LAUNCH(launch_count_cache_u16(len, m, dst.as<uint16_t>(), c->stream));
@@ -194,6 +267,8 @@ int bdc_create(int n, int k, const char *cache_dir, bdc_ctx **out) {
   if (!out) return fail(BDC_ERR_ARG, "out is NULL");
   *out = nullptr;
   if (!bdc_supported(n, k)) return BDC_ERR_ARG;
+  if (cache_dir && cache_dir[0])
+    for (auto &lm : needed_caches(n, k)) BDC_TRY(validate_cache_file(cache_dir, lm.first, lm.second));
   bdc_ctx *c = new bdc_ctx();
   auto cleanup = [&](int rc) {
     bdc_destroy(c);
@@ -683,7 +758,7 @@ static int ba_rows(bdc_ctx *c, int prec) {
 }
 
 // one BA step on the device over the rows x1 < rows; stats land in hist[idx]
-static int ba_step(bdc_ctx *c, int prec, int idx, int rows) {
+static int ba_step(bdc_ctx *c, int prec, int idx, int rows, double *hist) {
   double *q = c->qv.as<double>(), *L = q + ((size_t)1 << c->k);
   BDC_TRY(output_law(c, prec, c->P.as<double>(), kScalPScale, true, rows, q, L));
   if (prec == BDC_PREC_FP64) {
@@ -695,7 +770,7 @@ static int ba_step(bdc_ctx *c, int prec, int idx, int rows) {
   BDC_TRY(cross_term(c, prec, EPI_BA, nullptr, &nb, rows));
   if (prec == BDC_PREC_FP64) c->plimb_valid = false;  // P changed without new limbs
   BDC_LAUNCH(launch_ba_finalize(c->part.as<double>(), nb, (double)c->N1 / rows, c->scal.as<double>(),
-                                c->hist.as<double>(), idx, c->part.as<double>() + c->part_blocks * kPartStride, c->stream));
+                                hist, idx, c->part.as<double>() + c->part_blocks * kPartStride, c->stream));
   return BDC_OK;
 }
 
@@ -816,7 +891,7 @@ int bdc_iterate(bdc_ctx *c, int prec, int iters, double tol, bdc_iter_stats *sta
   const int rows = ba_rows(c, prec);
   int ran = 0;
   if (tol <= 0.0) {
-    for (int it = 0; it < iters; ++it) BDC_TRY(ba_step(c, prec, it, rows));
+    for (int it = 0; it < iters; ++it) BDC_TRY(ba_step(c, prec, it, rows, c->hist.as<double>()));
     ran = iters;
   } else {
     // The stopping rule (main_BAA.cu:73: the loop runs while mx >= a) ends at
@@ -844,7 +919,7 @@ int bdc_iterate(bdc_ctx *c, int prec, int iters, double tol, bdc_iter_stats *sta
           BDC_HIP(hipMemcpyAsync(c->ck_t.p, c->ptexp.p, c->ptexp.bytes, hipMemcpyDeviceToDevice, c->stream));
         }
       }
-      for (int it = 0; it < nb; ++it) BDC_TRY(ba_step(c, prec, ran + it, rows));
+      for (int it = 0; it < nb; ++it) BDC_TRY(ba_step(c, prec, ran + it, rows, c->hist.as<double>()));
       h.resize((size_t)4 * nb);
       BDC_HIP(hipMemcpyAsync(h.data(), c->hist.as<double>() + 4 * ran, h.size() * 8, hipMemcpyDeviceToHost, c->stream));
       BDC_HIP(hipStreamSynchronize(c->stream));
@@ -861,7 +936,7 @@ int bdc_iterate(bdc_ctx *c, int prec, int iters, double tol, bdc_iter_stats *sta
         if (split) BDC_HIP(hipMemcpyAsync(c->ptexp.p, c->ck_t.p, c->ptexp.bytes, hipMemcpyDeviceToDevice, c->stream));
         c->plimb_valid = limb_valid;
         c->plimb_rows = limb_rows;
-        for (int it = 0; it <= stop; ++it) BDC_TRY(ba_step(c, prec, ran + it, rows));
+        for (int it = 0; it <= stop; ++it) BDC_TRY(ba_step(c, prec, ran + it, rows, c->hist.as<double>()));
       }
       ran += stop + 1;
       break;
@@ -1093,6 +1168,7 @@ int bdc_certify_q(bdc_ctx *c, const double *theta, double *upper) {
   BDC_TRY(cross_operands(c, L));
   int nb = 0;
   BDC_TRY(cross_term(c, BDC_PREC_FP64, EPI_D, c->dvec.as<double>(), &nb));
+  c->have_d = true;  // D(x||softmax theta), readable through bdc_sample_inputs
   double r[3];
   BDC_LAUNCH(launch_dot(c->dvec.as<double>(), c->dvec.as<double>(), c->nx, c->part.as<double>(), c->splitk.as<double>(), c->stream));
   BDC_HIP(hipMemcpyAsync(r, c->splitk.p, 24, hipMemcpyDeviceToHost, c->stream));
@@ -1190,6 +1266,13 @@ static int exact_bits(double mass) {
   for (int b = 11; b >= 2; --b)
     if (mass * std::ldexp(1.0, b - 1) <= 16777216.0) return b;
   return 0;
+}
+
+// G1 split-K: enough blocks (>= 512) for the 256 CUs, each K range >= 1024 rows
+static int g1_splits(int rows, int blocks) {
+  int ks = 1;
+  while (ks < 8 && blocks * ks < 512 && (rows / 128) % (ks * 2) == 0 && rows / (ks * 2) >= 1024) ks *= 2;
+  return env_int("BDC_G1_KSPLIT", ks);
 }
 
 int split_prepare(bdc_ctx *c) {
@@ -1317,19 +1400,21 @@ int split_prepare(bdc_ctx *c) {
   BDC_HIP(hipMemcpyAsync(c->entA.p, entA.data(), entA.size() * 4, hipMemcpyHostToDevice, c->stream));
   BDC_HIP(hipMemcpyAsync(c->rowptrB.p, rowptrB.data(), rowptrB.size() * 4, hipMemcpyHostToDevice, c->stream));
   BDC_HIP(hipMemcpyAsync(c->entB.p, entB.data(), entB.size() * 4, hipMemcpyHostToDevice, c->stream));
+  {  // split-K slabs of G1 for every row count a call may use (full and half space)
+    size_t need = 0;
+    for (size_t rows : {N1, N1 / 2})
+      if (rows >= 128 && rows % 128 == 0) {
+        const int ks = g1_splits((int)rows, (int)((N2 / 32) * ((KLp + 255) / 256)));
+        if (ks > 1) need = std::max(need, (size_t)ks * KLp * N2 * 8);
+      }
+    if (need > c->g1part.bytes) BDC_TRY(c->g1part.alloc(need));
+  }
   BDC_TRY(c->ptexp.alloc((N1 / 128) * (N2 / 32) * 4));
   BDC_TRY(c->vlimb.alloc((size_t)sp.g2_limbs * N2 * KLp * 2));
   if (KRp) BDC_TRY(c->wlimb.alloc((size_t)sp.g2_limbs * N1 * KRp * 2));
   BDC_HIP(hipStreamSynchronize(c->stream));
   sp.ok = true;
   return BDC_OK;
-}
-
-// G1 split-K: enough blocks (>= 512) for the 256 CUs, each K range >= 1024 rows
-static int g1_splits(int rows, int blocks) {
-  int ks = 1;
-  while (ks < 8 && blocks * ks < 512 && (rows / 128) % (ks * 2) == 0 && rows / (ks * 2) >= 1024) ks *= 2;
-  return env_int("BDC_G1_KSPLIT", ks);
 }
 
 int split_output_law_g1(bdc_ctx *c, const double *X, int rows) {

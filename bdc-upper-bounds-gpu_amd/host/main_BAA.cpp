@@ -37,6 +37,13 @@
 #include <vector>
 
 #include "../../include/bdc.h"
+#include "dist_io.hpp"
+
+using dist_io::load_dist;
+using dist_io::load_meta;
+using dist_io::Meta;
+using dist_io::save_dist;
+using dist_io::save_meta;
 
 static int die(const char *what) {
   fprintf(stderr, "Error: %s: %s\n", what, bdc_last_error());
@@ -46,116 +53,6 @@ static int die(const char *what) {
 static bool file_exists(const std::string &p) {
   struct stat st;
   return stat(p.c_str(), &st) == 0;
-}
-
-static unsigned host_threads() { return std::max(1u, std::min(64u, std::thread::hardware_concurrency())); }
-
-// one "%.17g\n" line per value, formatted in parallel chunks
-static bool save_dist(const std::string &file, const std::vector<double> &Q) {
-  const size_t nv = Q.size();
-  const unsigned nt = (unsigned)std::min<size_t>(host_threads(), std::max<size_t>(1, nv / 65536));
-  std::vector<std::string> parts(nt);
-  std::vector<std::thread> pool;
-  for (unsigned t = 0; t < nt; ++t)
-    pool.emplace_back([&, t] {
-      const size_t a = nv * t / nt, b = nv * (t + 1) / nt;
-      std::string &out = parts[t];
-      out.resize((b - a) * 26);
-      char *p = out.data();
-      for (size_t i = a; i < b; ++i) {
-        p = std::to_chars(p, out.data() + out.size(), Q[i], std::chars_format::general, 17).ptr;
-        *p++ = '\n';
-      }
-      out.resize(p - out.data());
-    });
-  for (auto &th : pool) th.join();
-  const std::string tmp = file + ".tmp";
-  FILE *f = fopen(tmp.c_str(), "wb");
-  if (!f) return false;
-  bool ok = true;
-  for (const auto &s : parts) ok = ok && fwrite(s.data(), 1, s.size(), f) == s.size();
-  ok = (fclose(f) == 0) && ok;
-  return ok && rename(tmp.c_str(), file.c_str()) == 0;
-}
-
-// whitespace-separated values, parsed in parallel chunks split at whitespace
-static bool load_dist(const std::string &file, std::vector<double> &Q) {
-  FILE *f = fopen(file.c_str(), "rb");
-  if (!f) return false;
-  std::string buf;
-  fseek(f, 0, SEEK_END);
-  const long len = ftell(f);
-  fseek(f, 0, SEEK_SET);
-  buf.resize(len > 0 ? (size_t)len : 0);
-  const bool rd = buf.empty() || fread(buf.data(), 1, buf.size(), f) == buf.size();
-  fclose(f);
-  if (!rd) return false;
-  const unsigned nt = (unsigned)std::min<size_t>(host_threads(), std::max<size_t>(1, buf.size() / (1 << 20)));
-  std::vector<size_t> cut(nt + 1, buf.size());
-  cut[0] = 0;
-  auto space = [](char ch) { return ch == ' ' || ch == '\n' || ch == '\r' || ch == '\t'; };
-  for (unsigned t = 1; t < nt; ++t) {
-    size_t c = buf.size() * t / nt;
-    while (c < buf.size() && !space(buf[c])) ++c;
-    cut[t] = c;
-  }
-  std::vector<std::vector<double>> parts(nt);
-  std::vector<int> bad(nt, 0);
-  std::vector<std::thread> pool;
-  for (unsigned t = 0; t < nt; ++t)
-    pool.emplace_back([&, t] {
-      const char *p = buf.data() + cut[t], *e = buf.data() + cut[t + 1];
-      while (true) {
-        while (p < e && space(*p)) ++p;
-        if (p >= e) break;
-        double v;
-        auto r = std::from_chars(p, e, v);
-        if (r.ec != std::errc()) {
-          bad[t] = 1;
-          break;
-        }
-        parts[t].push_back(v);
-        p = r.ptr;
-      }
-    });
-  for (auto &th : pool) th.join();
-  Q.clear();
-  for (unsigned t = 0; t < nt; ++t) {
-    if (bad[t]) return false;
-    Q.insert(Q.end(), parts[t].begin(), parts[t].end());
-  }
-  return true;
-}
-
-struct Meta {
-  int n = -1, k = -1;
-  long iteration = 0;
-  double mx = NAN;
-  std::string precision;
-};
-
-static bool save_meta(const std::string &file, const Meta &m) {
-  const std::string tmp = file + ".tmp";
-  FILE *f = fopen(tmp.c_str(), "w");
-  if (!f) return false;
-  fprintf(f, "n %d\nk %d\nprecision %s\niteration %ld\nmx %.17g\n", m.n, m.k, m.precision.c_str(), m.iteration, m.mx);
-  if (fclose(f) != 0) return false;
-  return rename(tmp.c_str(), file.c_str()) == 0;
-}
-
-static bool load_meta(const std::string &file, Meta &m) {
-  FILE *f = fopen(file.c_str(), "r");
-  if (!f) return false;
-  char key[32], val[64];
-  while (fscanf(f, "%31s %63s", key, val) == 2) {
-    if (!strcmp(key, "n")) m.n = atoi(val);
-    else if (!strcmp(key, "k")) m.k = atoi(val);
-    else if (!strcmp(key, "iteration")) m.iteration = atol(val);
-    else if (!strcmp(key, "mx")) m.mx = strtod(val, nullptr);
-    else if (!strcmp(key, "precision")) m.precision = val;
-  }
-  fclose(f);
-  return true;
 }
 
 int main(int argc, char **argv) {
@@ -233,22 +130,30 @@ int main(int argc, char **argv) {
     if (!save_meta(metafile, meta)) fprintf(stderr, "Warning: could not write %s\n", metafile.c_str());
     return true;
   };
+  // Steps run in calls of up to save_iterations steps (to the next save point)
+  // with the stopping rule inside the library, which stops after the first step
+  // with mx < a (main_BAA.cu:73) without a host round trip per step.
+  const long chunk_max = save_iterations > 0 ? save_iterations : 1000;
+  std::vector<bdc_iter_stats> st((size_t)chunk_max);
   while (mx >= a) {
+    const long to_save = save_iterations > 0 ? save_iterations - ((it - 1) % save_iterations) : chunk_max;
+    const int want = (int)std::min(to_save, chunk_max);
     const auto t0 = std::chrono::steady_clock::now();
-    bdc_iter_stats st;
     int done = 0;
-    if (bdc_iterate(ctx, prec, 1, 0.0, &st, &done) != BDC_OK) return die("iterate");
-    mx = st.mx_bits;
-    if (save_iterations > 0 && it % save_iterations == 0) {
+    if (bdc_iterate(ctx, prec, want, a, st.data(), &done) != BDC_OK) return die("iterate");
+    const double per = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() / std::max(done, 1);
+    for (int s = 0; s < done; ++s, ++it) {
+      mx = st[(size_t)s].mx_bits;
+      printf("Iteration %ld: mx = %.6f, time = %.3f s\n", it, mx, per);
+    }
+    if (done == 0) break;
+    if (save_iterations > 0 && (it - 1) % save_iterations == 0) {
       const auto s0 = std::chrono::steady_clock::now();
       if (bdc_get_p_orbits(ctx, Q.data(), norb) != BDC_OK) return die("get p");
-      if (checkpoint(it))
+      if (checkpoint(it - 1))
         printf("Distribution saved in %.3f seconds\n",
                std::chrono::duration<double>(std::chrono::steady_clock::now() - s0).count());
     }
-    printf("Iteration %ld: mx = %.6f, time = %.3f s\n", it, mx,
-           std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
-    ++it;
   }
   if (bdc_get_p_orbits(ctx, Q.data(), norb) != BDC_OK) return die("get p");
   if (checkpoint(it - 1)) printf("Final distribution saved to %s\n", file.c_str());

@@ -51,6 +51,51 @@ def run_precision(bdc, e, prec, steps, warmup):
             "cert": cert, "g2_limbs": e.split_config().get("g2_limbs", 1) if prec == bdc.PREC_SPLIT16 else 1}
 
 
+def err_windows(s16, s64):
+    """per-iteration error of the split path against FP64 (bits): LB = I(p_t),
+    UB_ref = I(p_t) + mx_t, mx_t; maxima over windows of iterations"""
+    t = min(len(s16), len(s64))
+    lb16, lb64 = s16[:t, 3] / LN2, s64[:t, 3] / LN2
+    err = {"lb": np.abs(lb16 - lb64), "ub_ref": np.abs((lb16 + s16[:t, 0]) - (lb64 + s64[:t, 0])),
+           "mx": np.abs(s16[:t, 0] - s64[:t, 0])}
+    wins = [(a, min(b, t)) for a, b in ((0, 10), (10, 100), (100, t)) if min(b, t) > a]
+    return err, {f"{a}-{b}": {key: float(v[a:b].max()) for key, v in err.items()} for a, b in wins}
+
+
+def accuracy_leg(bdc, e, steps):
+    """Untimed: `steps` BA steps from the uniform p at both precisions (the
+    precision config's 2000 iterations) and the per-iteration error of the
+    split path against FP64, with the certified bounds after the last step."""
+    log(f"accuracy: {steps} steps at each precision (untimed)")
+    out = {}
+    for name, prec in (("fp64", bdc.PREC_FP64), ("split", bdc.PREC_SPLIT16)):
+        e.set_p_uniform()
+        st = e.iterate(steps, prec)
+        out[name] = (st, e.certify())
+    (s64, c64), (s16, c16) = out["fp64"], out["split"]
+    err, wins = err_windows(s16, s64)
+    res = {"steps": steps, "per_iteration_max_err_bits": {key: float(v.max()) for key, v in err.items()},
+           "per_iteration_max_err_bits_by_window": wins,
+           "fp64_ub_cert_bits": c64["upper_cert"] / LN2, "split_ub_cert_bits": c16["upper_cert"] / LN2,
+           "ub_cert_gap_bits": abs(c16["upper_cert"] - c64["upper_cert"]) / LN2,
+           "lb_gap_bits": abs(c16["lower"] - c64["lower"]) / LN2,
+           "fp64_lb_bits": c64["lower"] / LN2, "fp64_mx_bits_last": float(s64[-1, 0])}
+    log(f"accuracy: max per-iteration error {res['per_iteration_max_err_bits']}, UB_cert gap "
+        f"{res['ub_cert_gap_bits']:.2e} bits after {steps} steps")
+    return res
+
+
+def host_cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for row in f:
+                if row.startswith("model name"):
+                    return row.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def measured_traffic(n, k, key):
     """HBM bytes per launch of a kernel from the committed rocprofv3 --pmc
     summary (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by what the
@@ -339,7 +384,8 @@ def cpu_baseline(n, k, seconds):
         count *= 2
     ba.close()
     frac = done / norb
-    return {"value": frac / spent, "unit": "iter/s", "cores": threads, "kind": "port",
+    return {"value": frac / spent, "unit": "iter/s", "cores": threads, "kind": "port", "cpu_model": host_cpu_model(),
+            "host_cpus": os.cpu_count(),
             "sample": f"both reference enumeration passes over {done} of {norb} orbits ({spent:.1f} s, "
                       f"setup {setup:.1f} s excluded), extrapolated to one full iteration"}
 
@@ -354,6 +400,9 @@ def main():
     ap.add_argument("--precision", default="both", choices=["both", "fp64", "split"])
     ap.add_argument("--fp64-steps", type=int, default=None, help="FP64 steps (default: same as --steps)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--accuracy-steps", type=int, default=2000,
+                    help="untimed steps at both precisions for the per-iteration error against FP64 "
+                         "(run when above --steps)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (traffic-counter calibration under rocprofv3 --pmc)")
@@ -433,20 +482,32 @@ def main():
     r = res[head]
     prec = bdc.PREC_SPLIT16 if head == "split" else bdc.PREC_FP64
     g1f, g2f, g2bytes = e.work_model(prec)
-    # dominant kernel: the cross-term contraction G2 with the fused BA update.
-    # split16: HBM-bound (its FP16 matrix-core work is a small fraction of peak,
-    # reported beside it); FP64: bound by the FP64 matrix cores.
+    # Dominant kernel: the cross-term contraction G2 with the fused BA update.
+    # Algorithmic bytes are orbit-reduced: per complement/reversal orbit the step
+    # reads p and H(x) and writes p once (24 B), as the reference's orbit list
+    # does (main_BAA.cu:46-50, BAA.cu:230-241).  The kernel's own layout moves
+    # 24 B per input of the half space it updates (both members of every reversal
+    # pair), reported beside them as layout bytes.
+    orbit_bytes = 24.0 * e.num_orbits
     g2s = r["g2_ms"] * 1e-3
+    traffic = measured_traffic(n, k, "g2_split" if head == "split" else "g2_fp64")
     if head == "split":
         limb_flops = g2f * r["g2_limbs"]  # every FP64-equivalent multiply-add runs once per limb
-        roof = {"bound": "hbm", "achieved": g2bytes / g2s / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
-                "frac": g2bytes / g2s / PEAK_HBM, "traffic": measured_traffic(n, k, "g2_split"),
+        roof = {"bound": "hbm", "achieved": orbit_bytes / g2s / 1e9, "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+                "frac": orbit_bytes / g2s / PEAK_HBM, "traffic": traffic,
                 "kernel": "k_split_g2 (cross term + BA update + p tile exponents)",
-                "algorithmic_bytes_per_launch": g2bytes,
+                "algorithmic_bytes_per_launch": orbit_bytes,
+                "algorithmic_bytes_rule": f"24 B x {e.num_orbits} orbits (p read, p written, H read)",
+                "layout_bytes_per_launch": g2bytes,
+                "layout_frac": g2bytes / g2s / PEAK_HBM,
+                "traffic_source": "profiles/traffic_bdc28_14.json (rocprofv3 --pmc FETCH_SIZE + WRITE_SIZE, "
+                                  "calibrated on a 1 GiB copy)" if traffic else None,
+                "traffic_over_algorithmic": traffic / orbit_bytes if traffic else None,
+                "traffic_over_layout": traffic / g2bytes if traffic else None,
                 "mfma_tflops": limb_flops / g2s / 1e12, "mfma_frac": limb_flops / g2s / PEAK_FP16}
     else:
         roof = {"bound": "mfma", "achieved": g2f / g2s / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
-                "frac": g2f / g2s / PEAK_FP64, "traffic": measured_traffic(n, k, "g2_fp64"),
+                "frac": g2f / g2s / PEAK_FP64, "traffic": traffic,
                 "kernel": "k_gemm64 cross term + BA update", "algorithmic_flop_per_launch": g2f}
     line = {
         "metric": "BAA iter/s at n=28,k=14: split-FP16 MFMA vs FP64 GPU, bound within 1e-9 bits",
@@ -461,7 +522,8 @@ def main():
         "vs_baseline": None,
         "dtype": "fp16 limbs x fp32 accumulate + fp64" if head == "split" else "f64",
         "data": "synthetic",
-        "config": {"workload": f"BDC_({n},{k}) Blahut-Arimoto, uniform initial p, full 2^{n} input space",
+        "config": {"workload": f"BDC_({n},{k}) Blahut-Arimoto step from the uniform p: 2^{n} inputs, "
+                               f"{e.num_orbits} complement/reversal orbits, updated on the half space x1 < 2^{n // 2 - 1}",
                    "n": n, "k": k, "iterations": args.steps, "precision": head},
         "roofline": roof,
         "setup_s": setup_s,
@@ -478,17 +540,12 @@ def main():
         line["ub_cert_gap_bits"] = abs(res["split"]["cert"]["upper_cert"] - res["fp64"]["cert"]["upper_cert"]) / LN2
         line["lb_gap_bits"] = abs(res["split"]["cert"]["lower"] - res["fp64"]["cert"]["lower"]) / LN2
         # per-iteration error against the FP64 path: LB = I(p_t), UB_ref = I(p_t) + mx_t
-        s16, s64 = res["split"]["stats"], res["fp64"]["stats"]
-        t = min(len(s16), len(s64))
-        lb16, lb64 = s16[:t, 3] / LN2, s64[:t, 3] / LN2
-        err = {"lb": np.abs(lb16 - lb64), "ub_ref": np.abs((lb16 + s16[:t, 0]) - (lb64 + s64[:t, 0])),
-               "mx": np.abs(s16[:t, 0] - s64[:t, 0])}
+        err, wins = err_windows(res["split"]["stats"], res["fp64"]["stats"])
         line["per_iteration_max_err_bits"] = {key: float(v.max()) for key, v in err.items()}
         line["per_iteration_argmax"] = {key: int(v.argmax()) for key, v in err.items()}
-        # by window of iterations: the early steps move p by orders of magnitude
-        wins = [(a, min(b, t)) for a, b in ((0, 10), (10, 100), (100, t)) if min(b, t) > a]
-        line["per_iteration_max_err_bits_by_window"] = {
-            f"{a}-{b}": {key: float(v[a:b].max()) for key, v in err.items()} for a, b in wins}
+        line["per_iteration_max_err_bits_by_window"] = wins
+    if args.accuracy_steps > args.steps and args.precision == "both":
+        line["accuracy"] = accuracy_leg(bdc, e, args.accuracy_steps)
     e.close()
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle, reference method, sampled)")

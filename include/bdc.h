@@ -119,7 +119,8 @@ int bdc_entropy_term(bdc_ctx *ctx, double *h);
 /* Values at selected inputs x[0..len) (len <= 2^24), for checks where the
  * full 2^n arrays are too large to copy (n up to 32): what = BDC_SAMPLE_ENTROPY
  * gives H(x)/C(n,k) - ln C(n,k) (as bdc_entropy_term), BDC_SAMPLE_P gives p(x)
- * (per string), BDC_SAMPLE_KL gives D(x||q) from the last bdc_certify.  No
+ * (per string), BDC_SAMPLE_KL gives D(x||q) from the last bdc_certify or
+ * bdc_certify_q.  No
  * reference counterpart (the reference only ever copies whole arrays,
  * BAA.cu:642-724). */
 enum { BDC_SAMPLE_ENTROPY = 0, BDC_SAMPLE_P = 1, BDC_SAMPLE_KL = 2 };

@@ -21,6 +21,13 @@
  *     (tests/test_oracle_ba.py).
  *   - or_full_dual: central finite differences and the closed form
  *     dU/dtheta = q - W^T pi (tests/test_oracle_dual.py).
+ *   - or_entropy_x / or_kl_x: the dense channel, a brute-force sweep over every
+ *     output (k close to n and k = n), the closed forms for x = 0^a 1^b and for
+ *     one deletion (sum over runs of l ln l, at n = 29), and D(x||uniform)
+ *     (tests/test_oracle_counts.py).
+ *   - or_sparse_output_law: or_full_output_law on the same weights.
+ *   - or_ctx_sample_pass: its checksum against a dense textbook BA over all
+ *     orbits and over a slice (tests/test_oracle_ba.py).
  */
 #include "bdc_oracle.h"
 
@@ -477,30 +484,81 @@ int or_num_threads(void) { return omp_get_max_threads(); }
 
 /* ------------------------------------------------------- per-input sums */
 
-/* sum_y c(x,y) ln c(x,y) in nats over all 2^k outputs: the W ln W part of
- * log_W_term (backend/BAA.cu:401-411) times C(n,k), for a single input x
- * (n <= 32).  Terms with c <= 1 contribute 0. */
+/* Single-input sums over the outputs y with c(x,y) > 0, visited in ascending
+ * order through the reference's own enumeration of distinct subsequences
+ * (fill_DP_tables + unrank_k, backend/BAA.cu:261-335): the same nonzero terms in
+ * the same order as a sweep over all 2^k outputs, at a cost of (number of
+ * distinct subsequences) x O(nk) instead of 2^k x O(nk), so k close to n is
+ * practical (a few thousand outputs per input at n = 29, k = 24). */
+
+/* sum_y c(x,y) ln c(x,y) in nats: the W ln W part of log_W_term
+ * (backend/BAA.cu:401-411) times C(n,k), for a single input x (n <= 32).
+ * Terms with c <= 1 contribute 0. */
 double or_entropy_x(int n, int k, uint32_t x) {
-  /* extended-precision accumulation: at n = 32 the sum has 2^16 terms and
+  /* extended-precision accumulation: at n = 32 the sum has up to 2^16 terms and
    * reaches ~6e9, where a plain FP64 running sum loses ~1e-12 relative */
-  const uint64_t ny = 1ull << k;
+  dp_tab t;
+  fill_dp(x, n, k, &t);
+  const uint32_t nd = t.cnt[0][k];
   long double h = 0.0L;
-  for (uint64_t y = 0; y < ny; ++y) {
-    const uint32_t c = or_count(x, n, (uint32_t)y, k);
+  for (uint32_t r = 0; r < nd; ++r) {
+    const uint32_t c = or_count(x, n, unrank(n, k, r, &t), k);
     if (c > 1) h += (long double)c * logl((long double)c);
   }
   return (double)h;
 }
 
 /* D(x||q) = sum_y W(y|x) (ln W(y|x) - ln q(y)) in nats for a single input x,
- * W = c / C(n,k) (compute_KL_kernel / KL_term, backend/BAA.cu:419-425, 533-571) */
+ * W = c / C(n,k) (compute_KL_kernel / KL_term, backend/BAA.cu:419-425, 533-571);
+ * logq holds ln q over all 2^k outputs */
 double or_kl_x(int n, int k, uint32_t x, const double *logq) {
-  const uint64_t ny = 1ull << k;
+  dp_tab t;
+  fill_dp(x, n, k, &t);
+  const uint32_t nd = t.cnt[0][k];
   const long double C = (long double)or_nck((uint32_t)n, (uint32_t)k), lnC = logl(C);
   long double d = 0.0L;
-  for (uint64_t y = 0; y < ny; ++y) {
-    const uint32_t c = or_count(x, n, (uint32_t)y, k);
-    if (c) d += (c / C) * (logl((long double)c) - lnC - (long double)logq[y]);
+  for (uint32_t r = 0; r < nd; ++r) {
+    const uint32_t y = unrank(n, k, r, &t);
+    const uint32_t c = or_count(x, n, y, k);
+    d += (c / C) * (logl((long double)c) - lnC - (long double)logq[y]);
   }
   return (double)d;
+}
+
+/* q(y) = sum_i w[i] c(xs[i], y) / C(n,k) over all 2^k outputs for a weight
+ * vector supported on the inputs xs[0..count) (compute_den_kernel,
+ * backend/BAA.cu:429-485, restricted to a support).  The counts of one input
+ * for all outputs come from one pass over its symbols: cnt_t[z] = embeddings
+ * of z (t symbols) in the prefix read so far, and reading symbol b adds
+ * cnt_{t-1}[z'] to cnt_t[z' b] (longest t first), the recurrence of
+ * generate_bit_transition_cache.cpp:23-44 run for every output at once. */
+void or_sparse_output_law(int n, int k, uint64_t count, const uint32_t *xs, const double *w, double *q) {
+  const uint64_t ny = 1ull << k;
+  const double inv_c = 1.0 / (double)or_nck((uint32_t)n, (uint32_t)k);
+  memset(q, 0, ny * sizeof(double));
+#pragma omp parallel
+  {
+    double *acc = (double *)calloc(ny, sizeof(double));
+    uint32_t *cnt = (uint32_t *)malloc(sizeof(uint32_t) << (k + 1)); /* level t at offset 2^t */
+#pragma omp for schedule(dynamic, 4)
+    for (uint64_t i = 0; i < count; ++i) {
+      memset(cnt, 0, sizeof(uint32_t) << (k + 1));
+      cnt[1] = 1; /* the empty string */
+      for (int pos = 0; pos < n; ++pos) {
+        const uint32_t b = (xs[i] >> (n - 1 - pos)) & 1u;
+        for (int tl = (pos + 1 < k ? pos + 1 : k); tl >= 1; --tl) {
+          uint32_t *dst = cnt + (1ull << tl);
+          const uint32_t *src = cnt + (1ull << (tl - 1));
+          for (uint64_t z = 0; z < (1ull << (tl - 1)); ++z) dst[(z << 1) | b] += src[z];
+        }
+      }
+      const uint32_t *ck = cnt + ny;
+      for (uint64_t y = 0; y < ny; ++y)
+        if (ck[y]) acc[y] += w[i] * (double)ck[y] * inv_c;
+    }
+#pragma omp critical
+    for (uint64_t y = 0; y < ny; ++y) q[y] += acc[y];
+    free(acc);
+    free(cnt);
+  }
 }

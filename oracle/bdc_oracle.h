@@ -65,9 +65,12 @@ void or_full_kl(int n, int k, const double *logq, double *D);
  * writes dU/dtheta (2^k) when grad != NULL.  Returns U. */
 double or_full_dual(int n, int k, const double *theta, double beta, double *grad);
 
-/* single-input sums (any n <= 32): sum_y c ln c, and D(x||q) given ln q (2^k) */
+/* single-input sums (any n <= 32, k <= n): sum_y c ln c, and D(x||q) given ln q
+ * (2^k), over the distinct subsequences of x in ascending order */
 double or_entropy_x(int n, int k, uint32_t x);
 double or_kl_x(int n, int k, uint32_t x, const double *logq);
+/* output law of a weight vector supported on count inputs xs (k <= 24) */
+void or_sparse_output_law(int n, int k, uint64_t count, const uint32_t *xs, const double *w, double *q);
 
 int or_num_threads(void);
 

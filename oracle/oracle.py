@@ -51,6 +51,7 @@ def lib():
         L.or_num_threads.argtypes, L.or_num_threads.restype = [], i32
         L.or_entropy_x.argtypes, L.or_entropy_x.restype = [i32, i32, u32], dbl
         L.or_kl_x.argtypes, L.or_kl_x.restype = [i32, i32, u32, vp], dbl
+        L.or_sparse_output_law.argtypes, L.or_sparse_output_law.restype = [i32, i32, u64, vp, vp, vp], None
         _lib = L
     return _lib
 
@@ -118,6 +119,16 @@ def kl_x(n, k, x, logq):
     """D(x||q) (nats) for one input x given ln q over all 2^k outputs"""
     logq = np.ascontiguousarray(logq, dtype=np.float64)
     return lib().or_kl_x(n, k, int(x), _ptr(logq))
+
+
+def sparse_output_law(n, k, xs, w):
+    """q(y) = sum_i w[i] c(xs[i], y) / C(n,k) over all 2^k outputs"""
+    xs = np.ascontiguousarray(xs, dtype=np.uint32)
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    assert xs.shape == w.shape
+    q = np.empty(1 << k)
+    lib().or_sparse_output_law(n, k, xs.size, _ptr(xs), _ptr(w), _ptr(q))
+    return q
 
 
 class BA:

@@ -96,6 +96,33 @@ def test_main_baa_split16(oracle, tmp_path):
     assert read_meta(str(tmp_path / "distributions" / f"dist_{n}_{k}.txt.meta"))["precision"] == "split16"
 
 
+def test_main_baa_loads_transition_counts(tmp_path):
+    """With the half-length cache files in ./transition_counts (the reference's
+    cache directory, BAA.cu:27-93) main_BAA loads them instead of generating
+    the counts, and the run is bitwise that of a run without them."""
+    import bdc_gpu
+    n, k = 16, 8
+    say("main_BAA 16 8 with and without ./transition_counts")
+    plain = tmp_path / "plain"
+    cached = tmp_path / "cached"
+    plain.mkdir()
+    (cached / "transition_counts").mkdir(parents=True)
+    bdc_gpu.write_caches(8, str(cached / "transition_counts"))
+    r1 = run(plain, n, k, 0, 1e-3, 0)
+    r2 = run(cached, n, k, 0, 1e-3, 0)
+    assert r1.returncode == 0 and r2.returncode == 0, (r1.stderr, r2.stderr)
+    assert "Cache files not found" in r1.stdout
+    assert "Cache files not found" not in r2.stdout
+    d1 = (plain / "distributions" / f"dist_{n}_{k}.txt").read_bytes()
+    d2 = (cached / "distributions" / f"dist_{n}_{k}.txt").read_bytes()
+    assert d1 == d2
+    # a truncated cache file is an error, not a silent regeneration
+    f = cached / "transition_counts" / "cache_8_3"
+    f.write_bytes(f.read_bytes()[:-4])
+    r3 = run(cached, n, k, 0, 1e-3, 0)
+    assert r3.returncode != 0 and "cache_8_3" in r3.stderr
+
+
 @pytest.mark.parametrize("case", ["short", "unnormalised", "negative", "garbage"])
 def test_main_baa_rejects_bad_distribution(oracle, tmp_path, case):
     n, k = 10, 5

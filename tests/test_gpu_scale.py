@@ -1,7 +1,8 @@
 """The scale configuration BDC_(32,16): 2^32 inputs (p and the entropy term
 32 GiB each in HBM), half length 16 (the quadrant layout of the entropy
-kernel, counts up to C(16,8) = 12870).  Setup takes minutes, so this module
-runs only when BDC_SCALE_TESTS=1.
+kernel, counts up to C(16,8) = 12870).  Setup takes about six minutes (the
+entropy term over 2^30 representatives x 2^16 outputs); the kernel's progress
+lines (one per sixteenth of the rows) are passed through to the terminal.
 
 Checks, with the full arrays never copied: the entropy term at sampled inputs
 against the oracle's single-input sums; one step from the uniform p against
@@ -17,9 +18,7 @@ import sys
 import numpy as np
 import pytest
 
-pytestmark = [pytest.mark.gpu,
-              pytest.mark.skipif(os.environ.get("BDC_SCALE_TESTS") != "1",
-                                 reason="BDC_(32,16) setup takes minutes; set BDC_SCALE_TESTS=1")]
+pytestmark = pytest.mark.gpu
 LN2 = math.log(2.0)
 
 
@@ -28,7 +27,7 @@ def say(msg):
     sys.stdout.flush()
 
 
-def test_scale_bdc32_16(oracle):
+def test_scale_bdc32_16(oracle, capfd):
     import bdc_gpu as bdc
     n, k = 32, 16
     C = math.comb(n, k)
@@ -39,7 +38,9 @@ def test_scale_bdc32_16(oracle):
     D0 = H / C - math.log(C) + k * LN2
     os.environ["BDC_VERBOSE"] = "1"
     say(f"scale n={n} k={k}: create")
-    with bdc.Engine(n, k) as e:
+    with capfd.disabled():  # the library's progress lines of the entropy precompute
+        e = bdc.Engine(n, k)
+    with e:
         assert e.num_orbits == 1073774592
         # H/C - ln C is a sum of 2^16 terms of total up to ~6e9 divided by C(32,16) ~ 6e8:
         # FP64 summation alone leaves ~1e-12 (4.7e-12 seen against a plain FP64 oracle sum)

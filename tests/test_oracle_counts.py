@@ -119,3 +119,47 @@ def test_kl_x_matches_full_kl(oracle):
     for x in [5, 300]:
         ref = oracle.entropy_x(n, k, x) / C - math.log(C) + k * math.log(2.0)
         assert abs(oracle.kl_x(n, k, x, u) - ref) <= 1e-13
+
+
+def test_single_input_sums_for_k_near_n(oracle):
+    """entropy_x / kl_x visit the distinct subsequences (the reference's
+    enumeration, BAA.cu:261-335): against a brute-force sweep over every output
+    for k close to n and k = n, and the single-deletion closed form at n = 29:
+    the distinct y of length n - 1 are one per run of x, reached ell times for a
+    run of length ell, so H(x) = sum over runs of ell ln ell."""
+    rng = np.random.default_rng(11)
+    for n, k in [(13, 11), (12, 12), (11, 10)]:
+        logq = np.log(rng.dirichlet(np.ones(1 << k)))
+        C = math.comb(n, k)
+        for x in [0, (1 << n) - 1, int(rng.integers(1 << n)), int(rng.integers(1 << n))]:
+            cs = np.array([oracle.count(x, n, y, k) for y in range(1 << k)], dtype=np.float64)
+            nz = cs > 0
+            H = float(np.sum(cs[cs > 1] * np.log(cs[cs > 1])))
+            D = float(np.sum(cs[nz] / C * (np.log(cs[nz]) - math.log(C) - logq[nz])))
+            assert abs(oracle.entropy_x(n, k, x) - H) <= 1e-12 * max(1.0, H)
+            assert abs(oracle.kl_x(n, k, x, logq) - D) <= 1e-12
+    n = 29
+    for x in [0x0ABCDEF1, 0x155AA55A, 0, 0x1FFFFFFF, 0x0F0F0F0F]:
+        bits = format(x, f"0{n}b")
+        runs, cur = [], 1
+        for i in range(1, n):
+            if bits[i] == bits[i - 1]:
+                cur += 1
+            else:
+                runs.append(cur)
+                cur = 1
+        runs.append(cur)
+        ref = sum(l * math.log(l) for l in runs if l > 1)
+        assert abs(oracle.entropy_x(n, n - 1, x) - ref) <= 1e-12 * max(1.0, ref)
+
+
+def test_sparse_output_law(oracle):
+    """q of a weight vector on a few inputs against the full-space sum"""
+    rng = np.random.default_rng(5)
+    for n, k in [(10, 5), (12, 7), (11, 11)]:
+        xs = rng.choice(1 << n, 40, replace=False).astype(np.uint32)
+        w = rng.random(40)
+        full = np.zeros(1 << n)
+        full[xs] = w
+        np.testing.assert_allclose(oracle.sparse_output_law(n, k, xs, w), oracle.full_output_law(n, k, full),
+                                   rtol=1e-13, atol=1e-300)
