@@ -27,6 +27,7 @@ EXPORTS = [
     "bdc_get_p_orbits", "bdc_set_p_dirichlet", "bdc_iterate", "bdc_certify", "bdc_output_law", "bdc_kl",
     "bdc_entropy_term", "bdc_dual", "bdc_refine_dual", "bdc_certify_q", "bdc_log_output", "bdc_sync",
     "bdc_kernel_time", "bdc_reset_kernel_timers", "bdc_work_model", "bdc_split_config", "bdc_sample_inputs",
+    "bdc_iterate_noalloc",
 ]
 SAMPLE_ENTROPY, SAMPLE_P, SAMPLE_KL = 0, 1, 2
 
@@ -77,6 +78,7 @@ def lib():
         L.bdc_get_p_orbits.argtypes = [vp, vp, u64]
         L.bdc_set_p_dirichlet.argtypes = [vp, dbl, u64]
         L.bdc_iterate.argtypes = [vp, i32, i32, dbl, vp, ctypes.POINTER(i32)]
+        L.bdc_iterate_noalloc.argtypes = [vp, i32, i32, vp]
         L.bdc_certify.argtypes = [vp, ctypes.POINTER(Certificate)]
         L.bdc_output_law.argtypes = [vp, i32, vp, vp]
         L.bdc_kl.argtypes = [vp, i32, vp, vp]
@@ -178,6 +180,13 @@ class Engine:
         _check(lib().bdc_iterate(self._h, precision, iters, float(tol), ctypes.cast(stats, ctypes.c_void_p),
                                  ctypes.byref(done)))
         a = np.array([(s.mx_bits, s.max_d, s.log_z, s.info) for s in stats[:done.value]], dtype=np.float64)
+        return a.reshape(-1, 4)
+
+    def iterate_noalloc(self, iters, precision=PREC_FP64):
+        """bdc_iterate_noalloc: the same steps with no device allocation"""
+        stats = (IterStats * max(iters, 1))()
+        _check(lib().bdc_iterate_noalloc(self._h, precision, iters, ctypes.cast(stats, ctypes.c_void_p)))
+        a = np.array([(s.mx_bits, s.max_d, s.log_z, s.info) for s in stats[:iters]], dtype=np.float64)
         return a.reshape(-1, 4)
 
     def certify(self):

@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -43,7 +44,14 @@ int fail(int code, const char *fmt, ...) {
   return code;
 }
 
+// Set for the duration of a bdc_iterate_noalloc call on this thread: any lazy
+// device allocation on its path fails instead of calling hipMalloc.
+static thread_local bool g_forbid_alloc = false;
+
 int DevBuf::alloc(size_t b) {
+  if (g_forbid_alloc)
+    return fail(BDC_ERR_STATE, "this call allocates no device memory, but %zu bytes were needed (prepare the context "
+                               "with one bdc_iterate at this precision first)", b);
   release();
   if (b == 0) return BDC_OK;
   hipError_t e = hipMalloc(&p, b);
@@ -76,6 +84,36 @@ static int check_launch(int rc, const char *what) {
     if (r_ != BDC_OK) return cleanup(r_); \
   } while (0)
 
+// ------------------------------------------------------- no-allocation steps
+// bdc_iterate_noalloc keeps its per-step statistics (4 doubles per step) in
+// this file-scope device array instead of a buffer the context grows with
+// hipMalloc.  Every GPU the library runs on holds one copy until the process
+// ends; its address on each device is looked up once, with that device
+// current, when the first context on it is created.  Calls on one GPU share
+// the array: calls that overlap on one GPU race (each call writes the rows of
+// its own steps before reading them, so nothing needs clearing).
+constexpr int kNoAllocMaxSteps = 4096;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) double scratch[kNoAllocMaxSteps * 4];
+static std::mutex g_scratch_mu;
+static double *g_scratch_dev[kMaxDevices] = {};
+
+static int scratch_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) return fail(BDC_ERR_HIP, "device %d out of range", device);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_dev[device]) return BDC_OK;
+  void *ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr)
+    return fail(BDC_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", device, hipGetErrorString(e));
+  g_scratch_dev[device] = static_cast<double *>(ptr);
+  return BDC_OK;
+}
+static double *scratch_ptr(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return (device >= 0 && device < kMaxDevices) ? g_scratch_dev[device] : nullptr;
+}
+
 static uint32_t nck_u32(uint32_t n, uint32_t k) {  // backend/BAA.cu:15-24
   if (k > n) return 0;
   if (k > n - k) k = n - k;
@@ -91,7 +129,7 @@ static uint64_t burnside_orbits(int n) {
 }
 
 static int timer_begin(bdc_ctx *c, Timer &t) {
-  if (!c->timing) return BDC_OK;
+  if (!c->timing || g_forbid_alloc) return BDC_OK;
   if ((int)t.start.size() <= t.used) {
     hipEvent_t a, b;
     BDC_HIP(hipEventCreate(&a));
@@ -103,7 +141,7 @@ static int timer_begin(bdc_ctx *c, Timer &t) {
   return BDC_OK;
 }
 static int timer_end(bdc_ctx *c, Timer &t) {
-  if (!c->timing) return BDC_OK;
+  if (!c->timing || g_forbid_alloc) return BDC_OK;
   BDC_HIP(hipEventRecord(t.stop[t.used], c->stream));
   ++t.used;
   return BDC_OK;
@@ -285,6 +323,7 @@ int bdc_create(int n, int k, const char *cache_dir, bdc_ctx **out) {
   c->inv_c = 1.0 / c->nck;  // backend/BAA.cu:96-111
   c->log_c = std::log(c->nck);
   if (hipGetDevice(&c->device) != hipSuccess) return cleanup(fail(BDC_ERR_HIP, "no HIP device"));
+  BDC_TRY_C(scratch_setup(c->device));
   if (hipStreamCreate(&c->stream) != hipSuccess) return cleanup(fail(BDC_ERR_HIP, "hipStreamCreate failed"));
   {
     const char *e = getenv("BDC_STAGE_FACTORED");
@@ -949,6 +988,31 @@ int bdc_iterate(bdc_ctx *c, int prec, int iters, double tol, bdc_iter_stats *sta
   BDC_HIP(hipStreamSynchronize(c->stream));
   c->last_mx_bits = last_mx;
   if (done) *done = ran;
+  return BDC_OK;
+}
+
+int bdc_iterate_noalloc(bdc_ctx *c, int prec, int iters, bdc_iter_stats *stats) {
+  if (!c) return fail(BDC_ERR_ARG, "null context");
+  if (prec != BDC_PREC_FP64 && prec != BDC_PREC_SPLIT16) return fail(BDC_ERR_ARG, "unknown precision %d", prec);
+  if (iters < 0 || iters > kNoAllocMaxSteps)
+    return fail(BDC_ERR_ARG, "bdc_iterate_noalloc runs 0..%d steps per call (got %d)", kNoAllocMaxSteps, iters);
+  if (!c->have_p) return fail(BDC_ERR_STATE, "p is not set");
+  if (prec == BDC_PREC_SPLIT16 && !c->split.ok)
+    return fail(BDC_ERR_STATE, "split16 operands are not prepared (one bdc_iterate or bdc_split_config call does it)");
+  double *hist = scratch_ptr(c->device);
+  if (!hist) return fail(BDC_ERR_STATE, "no device scratch for device %d", c->device);
+  const int rows = ba_rows(c, prec);
+  g_forbid_alloc = true;
+  int rc = BDC_OK;
+  for (int it = 0; it < iters && rc == BDC_OK; ++it) rc = ba_step(c, prec, it, rows, hist);
+  g_forbid_alloc = false;
+  if (rc != BDC_OK) return rc;
+  if (iters && rows < c->N1) BDC_LAUNCH(launch_reflect_half(c->P.as<double>(), c->nx, c->stream));
+  if (stats && iters) BDC_HIP(hipMemcpyAsync(stats, hist, (size_t)iters * sizeof(bdc_iter_stats), hipMemcpyDeviceToHost, c->stream));
+  double last_mx = c->last_mx_bits;
+  if (iters) BDC_HIP(hipMemcpyAsync(&last_mx, hist + 4 * (iters - 1), 8, hipMemcpyDeviceToHost, c->stream));
+  BDC_HIP(hipStreamSynchronize(c->stream));
+  c->last_mx_bits = last_mx;
   return BDC_OK;
 }
 

@@ -105,6 +105,18 @@ int bdc_set_p_dirichlet(bdc_ctx *ctx, double alpha, uint64_t seed);
  * of one step per call. */
 int bdc_iterate(bdc_ctx *ctx, int precision, int iters, double tol, bdc_iter_stats *stats, int *done);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): `iters` BA steps exactly as bdc_iterate with tol = 0, but
+ * the per-step statistics live in a file-scope device array of 4096 steps
+ * (128 KiB per GPU, held until the process ends) instead of a buffer the
+ * context grows, and no call on its path allocates (a step that would, fails
+ * with BDC_ERR_STATE).  iters must be in [0, 4096].  split16 needs its operands
+ * prepared by one earlier bdc_iterate or bdc_split_config on the context.
+ * The array is shared by every call on one GPU: calls that overlap on one GPU
+ * race.  stats (may be NULL) receives one record per step. */
+int bdc_iterate_noalloc(bdc_ctx *ctx, int precision, int iters, bdc_iter_stats *stats);
+
 /* ---- bounds: compute_all_log_D + compute_rate (BAA.cu:642-724, 836-913) ---- */
 int bdc_certify(bdc_ctx *ctx, bdc_certificate *out);
 
