@@ -246,6 +246,51 @@ static int make_cache_u16(bdc_ctx *c, int len, int m, DevBuf &dst, const char *d
   return BDC_OK;
 }
 
+// The deletion-chain path (no count caches, FP64) serves k close to n: above
+// k = 20 the dense split contractions would hold 2^k outputs per split, and
+// at k = n both halves are consumed whole.  BDC_CHAIN_PATH=1 forces it (tests).
+static bool chain_path(int n, int k) {
+  const char *e = getenv("BDC_CHAIN_PATH");
+  if (e && e[0] == '1') return true;
+  return k > kDenseMaxK || k == n;
+}
+
+// The entropy term by enumerating distinct subsequences (kernels_chain.hip)
+// where they are few against the 2^k outputs of the dense sweep; forced with
+// BDC_ENTROPY_DFS=1 (0 forbids it where the dense sweep applies).
+static bool entropy_by_enumeration(int n, int k) {
+  if (n - k > 16) return false;
+  if (k == n) return true;
+  const char *e = getenv("BDC_ENTROPY_DFS");
+  if (e && e[0]) return e[0] == '1' || k > kDenseMaxK;
+  return k > kDenseMaxK;
+}
+
+// Bit groups of the chain path, top to bottom: the bottom group (at most 13
+// bits) is a contiguous row per block; the bits above it form groups of at
+// most 9, each processed in tiles of 16 low columns.  Level buffer slot t
+// (2^(n-t) doubles) holds the strings after t deletions.
+static int chain_setup(bdc_ctx *c) {
+  const int n = c->n, d = c->n - c->k;
+  const int bottom = std::min(n, 13), rest = n - bottom;
+  const int ntop = (rest + 8) / 9;
+  c->ngrp = 0;
+  int above = 0;
+  for (int g = 0; g < ntop; ++g) {
+    const int L = rest / ntop + (g < rest % ntop ? 1 : 0);
+    above += L;
+    c->grpL[c->ngrp] = L;
+    c->grpB[c->ngrp] = n - above;
+    ++c->ngrp;
+  }
+  c->grpL[c->ngrp] = bottom;
+  c->grpB[c->ngrp] = 0;
+  ++c->ngrp;
+  c->lvl_off.assign(d + 2, 0);
+  for (int t = 0; t <= d; ++t) c->lvl_off[t + 1] = c->lvl_off[t] + ((uint64_t)1 << (n - t));
+  return c->lvl.alloc(c->lvl_off[d + 1] * 8);
+}
+
 }  // namespace bdc
 
 using namespace bdc;
@@ -296,8 +341,8 @@ int bdc_supported(int n, int k) {
   // backend/config.hpp:24 caps n at 32; the reference itself overflows `1 << n`
   // at n >= 31 (main_BAA.cu:47,64), this engine indexes inputs in 64 bits
   if (n < 2 || n > 32) return fail(0, "n must be in [2, 32]"), 0;
-  if (k < 1 || k >= n) return fail(0, "k must be in [1, n-1] (C_{n,n} = n bits needs no iteration)"), 0;
-  if (k > 20) return fail(0, "k <= 20 in this build (2^k outputs held per stage buffer)"), 0;
+  if (k < 1 || k > n) return fail(0, "k must be in [1, n]"), 0;
+  if (k > 30) return fail(0, "k <= 30 in this build (2^k outputs, 8 GiB per FP64 output vector at k = 30)"), 0;
   return 1;
 }
 
@@ -305,7 +350,8 @@ int bdc_create(int n, int k, const char *cache_dir, bdc_ctx **out) {
   if (!out) return fail(BDC_ERR_ARG, "out is NULL");
   *out = nullptr;
   if (!bdc_supported(n, k)) return BDC_ERR_ARG;
-  if (cache_dir && cache_dir[0])
+  const bool chain = chain_path(n, k);
+  if (cache_dir && cache_dir[0] && !chain)  // the chain path reads no count caches
     for (auto &lm : needed_caches(n, k)) BDC_TRY(validate_cache_file(cache_dir, lm.first, lm.second));
   bdc_ctx *c = new bdc_ctx();
   auto cleanup = [&](int rc) {
@@ -330,52 +376,55 @@ int bdc_create(int n, int k, const char *cache_dir, bdc_ctx **out) {
     c->stage_factored = !(e && e[0] == '0');
   }
 
-  // split sets
-  const int jlo = std::max(0, k - c->n2), jhi = std::min(k, c->n1);
-  for (int j = jlo; j <= jhi; ++j) {
-    if (2 * j <= k) {
-      c->offL.push_back(c->KL);
-      c->JL.push_back(j);
-      c->KL += 1 << j;
-    } else {
-      c->offR.push_back(c->KR);
-      c->JR.push_back(j);
-      c->KR += 1 << (k - j);
-    }
-  }
-  for (int j : c->JL)
-    if (j == c->n1) return cleanup(fail(BDC_ERR_ARG, "identity factor in the stacked operand (k too large)"));
-  for (int j : c->JR)
-    if (k - j == c->n2) return cleanup(fail(BDC_ERR_ARG, "identity factor in the stacked operand (k too large)"));
-  c->KLp = (c->KL + 63) / 64 * 64;
-  c->KRp = c->KR ? (c->KR + 63) / 64 * 64 : 0;
-  c->KT = c->KL + c->KR;
-
-  // half-length caches (only the ones a split uses; identities are implicit)
-  for (int j = jlo; j <= jhi; ++j) {
-    if (j < c->n1 && !c->cnt1[j].p) BDC_TRY_C(make_cache_u16(c, c->n1, j, c->cnt1[j], cache_dir));
-    const int m = k - j;
-    if (m < c->n2) {
-      if (c->n2 == c->n1) {
-        if (!c->cnt1[m].p) BDC_TRY_C(make_cache_u16(c, c->n1, m, c->cnt1[m], cache_dir));
-      } else if (!c->cnt2[m].p) {
-        BDC_TRY_C(make_cache_u16(c, c->n2, m, c->cnt2[m], cache_dir));
+  c->chain = chain;
+  if (!chain) {
+    // split sets
+    const int jlo = std::max(0, k - c->n2), jhi = std::min(k, c->n1);
+    for (int j = jlo; j <= jhi; ++j) {
+      if (2 * j <= k) {
+        c->offL.push_back(c->KL);
+        c->JL.push_back(j);
+        c->KL += 1 << j;
+      } else {
+        c->offR.push_back(c->KR);
+        c->JR.push_back(j);
+        c->KR += 1 << (k - j);
       }
     }
-  }
-  // stacked operands
-  BDC_TRY_C(c->abar16.alloc(sizeof(uint16_t) * (size_t)c->N1 * c->KLp));
-  if (hipMemsetAsync(c->abar16.p, 0, c->abar16.bytes, c->stream) != hipSuccess) return cleanup(fail(BDC_ERR_HIP, "memset"));
-  for (size_t i = 0; i < c->JL.size(); ++i)
-    BDC_TRY_C(check_launch(launch_stack_cols(c->cnt1[c->JL[i]].as<uint16_t>(), c->JL[i], c->N1, c->abar16.as<uint16_t>(),
-                                             c->KLp, c->offL[i], c->stream), "stack A"));
-  if (c->KR) {
-    BDC_TRY_C(c->bbar16.alloc(sizeof(uint16_t) * (size_t)c->N2 * c->KRp));
-    if (hipMemsetAsync(c->bbar16.p, 0, c->bbar16.bytes, c->stream) != hipSuccess) return cleanup(fail(BDC_ERR_HIP, "memset"));
-    for (size_t i = 0; i < c->JR.size(); ++i) {
-      const int m = k - c->JR[i];
-      BDC_TRY_C(check_launch(launch_stack_cols(c->c2(m), m, c->N2, c->bbar16.as<uint16_t>(), c->KRp, c->offR[i],
-                                               c->stream), "stack B"));
+    for (int j : c->JL)
+      if (j == c->n1) return cleanup(fail(BDC_ERR_ARG, "identity factor in the stacked operand (k too large)"));
+    for (int j : c->JR)
+      if (k - j == c->n2) return cleanup(fail(BDC_ERR_ARG, "identity factor in the stacked operand (k too large)"));
+    c->KLp = (c->KL + 63) / 64 * 64;
+    c->KRp = c->KR ? (c->KR + 63) / 64 * 64 : 0;
+    c->KT = c->KL + c->KR;
+
+    // half-length caches (only the ones a split uses; identities are implicit)
+    for (int j = jlo; j <= jhi; ++j) {
+      if (j < c->n1 && !c->cnt1[j].p) BDC_TRY_C(make_cache_u16(c, c->n1, j, c->cnt1[j], cache_dir));
+      const int m = k - j;
+      if (m < c->n2) {
+        if (c->n2 == c->n1) {
+          if (!c->cnt1[m].p) BDC_TRY_C(make_cache_u16(c, c->n1, m, c->cnt1[m], cache_dir));
+        } else if (!c->cnt2[m].p) {
+          BDC_TRY_C(make_cache_u16(c, c->n2, m, c->cnt2[m], cache_dir));
+        }
+      }
+    }
+    // stacked operands
+    BDC_TRY_C(c->abar16.alloc(sizeof(uint16_t) * (size_t)c->N1 * c->KLp));
+    if (hipMemsetAsync(c->abar16.p, 0, c->abar16.bytes, c->stream) != hipSuccess) return cleanup(fail(BDC_ERR_HIP, "memset"));
+    for (size_t i = 0; i < c->JL.size(); ++i)
+      BDC_TRY_C(check_launch(launch_stack_cols(c->cnt1[c->JL[i]].as<uint16_t>(), c->JL[i], c->N1, c->abar16.as<uint16_t>(),
+                                               c->KLp, c->offL[i], c->stream), "stack A"));
+    if (c->KR) {
+      BDC_TRY_C(c->bbar16.alloc(sizeof(uint16_t) * (size_t)c->N2 * c->KRp));
+      if (hipMemsetAsync(c->bbar16.p, 0, c->bbar16.bytes, c->stream) != hipSuccess) return cleanup(fail(BDC_ERR_HIP, "memset"));
+      for (size_t i = 0; i < c->JR.size(); ++i) {
+        const int m = k - c->JR[i];
+        BDC_TRY_C(check_launch(launch_stack_cols(c->c2(m), m, c->N2, c->bbar16.as<uint16_t>(), c->KRp, c->offR[i],
+                                                 c->stream), "stack B"));
+      }
     }
   }
 
@@ -410,18 +459,23 @@ int bdc_create(int n, int k, const char *cache_dir, bdc_ctx **out) {
   // work buffers
   BDC_TRY_C(c->P.alloc(c->nx * 8));
   BDC_TRY_C(c->Hc.alloc(c->nx * 8));
-  BDC_TRY_C(c->T.alloc((size_t)c->KLp * c->N2 * 8));
-  if (c->KR) BDC_TRY_C(c->U.alloc((size_t)c->KRp * c->N1 * 8));
   const size_t ny = (size_t)1 << k;
-  BDC_TRY_C(c->qpart.alloc((c->JL.size() + c->JR.size()) * ny * 8));
   BDC_TRY_C(c->qv.alloc(4 * ny * 8));  // q, L, q2 (scratch), grad
-  if (c->KR) {
-    BDC_TRY_C(c->g2a.alloc((size_t)c->N1 * c->KRp * 8));
-    if (hipMemsetAsync(c->g2a.p, 0, c->g2a.bytes, c->stream) != hipSuccess) return cleanup(fail(BDC_ERR_HIP, "memset"));
+  if (!chain) {
+    BDC_TRY_C(c->T.alloc((size_t)c->KLp * c->N2 * 8));
+    if (c->KR) BDC_TRY_C(c->U.alloc((size_t)c->KRp * c->N1 * 8));
+    BDC_TRY_C(c->qpart.alloc((c->JL.size() + c->JR.size()) * ny * 8));
+    if (c->KR) {
+      BDC_TRY_C(c->g2a.alloc((size_t)c->N1 * c->KRp * 8));
+      if (hipMemsetAsync(c->g2a.p, 0, c->g2a.bytes, c->stream) != hipSuccess) return cleanup(fail(BDC_ERR_HIP, "memset"));
+    }
+    BDC_TRY_C(c->g2b.alloc((size_t)c->KLp * c->N2 * 8));
+    if (hipMemsetAsync(c->g2b.p, 0, c->g2b.bytes, c->stream) != hipSuccess) return cleanup(fail(BDC_ERR_HIP, "memset"));
+    c->splitk_elems = (size_t)64 * ny;  // split-K partials for the stage GEMMs: up to 64 splits of 2^k outputs
+  } else {
+    BDC_TRY_C(chain_setup(c));
+    c->splitk_elems = 4096;  // the certificate's three sums
   }
-  BDC_TRY_C(c->g2b.alloc((size_t)c->KLp * c->N2 * 8));
-  if (hipMemsetAsync(c->g2b.p, 0, c->g2b.bytes, c->stream) != hipSuccess) return cleanup(fail(BDC_ERR_HIP, "memset"));
-  c->splitk_elems = (size_t)64 * ny;  // split-K partials for the stage GEMMs: up to 64 splits of 2^k outputs
   BDC_TRY_C(c->splitk.alloc(c->splitk_elems * 8));
   c->part_blocks = std::max<size_t>(4096, ((size_t)c->N1 / 64) * ((size_t)c->N2 / 64));
   BDC_TRY_C(c->part.alloc((c->part_blocks + 256) * kPartStride * 8));  // + first-level finalize sums
@@ -457,8 +511,13 @@ int bdc_create(int n, int k, const char *cache_dir, bdc_ctx **out) {
       if (hipMemcpy(ltab.p, t, sizeof t, hipMemcpyHostToDevice) != hipSuccess)
         return cleanup(fail(BDC_ERR_HIP, "log table"));
     }
-    BDC_TRY_C(check_launch(launch_entropy_reps(n, k, c->n1, c->n2, c->repmask.as<uint64_t>(), c->wordbase.as<uint32_t>(),
-                                               ltab.as<double>(), hrep.as<double>(), c->stream), "entropy"));
+    if (entropy_by_enumeration(n, k))
+      BDC_TRY_C(check_launch(launch_entropy_dfs(n, k, c->norb, c->orbits.as<uint32_t>(), ltab.as<double>(),
+                                                hrep.as<double>(), c->stream), "entropy (distinct subsequences)"));
+    else
+      BDC_TRY_C(check_launch(launch_entropy_reps(n, k, c->n1, c->n2, c->repmask.as<uint64_t>(),
+                                                 c->wordbase.as<uint32_t>(), ltab.as<double>(), hrep.as<double>(),
+                                                 c->stream), "entropy"));
     BDC_TRY_C(check_launch(launch_expand_orbits(n, hrep.as<double>(), c->repmask.as<uint64_t>(), c->wordbase.as<uint32_t>(),
                                                 c->inv_c, -c->log_c, c->Hc.as<double>(), c->stream), "expand H"));
     if (hipStreamSynchronize(c->stream) != hipSuccess) return cleanup(fail(BDC_ERR_HIP, "entropy sync"));
@@ -484,7 +543,7 @@ int bdc_destroy(bdc_ctx *c) {
                          &c->entA, &c->rowptrB, &c->entB, &c->blkA, &c->blkB, &c->restiles[0], &c->restiles[1], &c->g1part, &c->repmask, &c->wordbase, &c->orbits, &c->P,
                          &c->Hc, &c->T, &c->U, &c->qpart, &c->qv, &c->g2a, &c->g2b, &c->splitk, &c->part, &c->scal, &c->exptab,
                          &c->hist, &c->ptexp, &c->vlimb, &c->wlimb, &c->theta, &c->adam_m,
-                         &c->adam_v, &c->grad, &c->dvec, &c->ck_p, &c->ck_s, &c->ck_t})
+                         &c->adam_v, &c->grad, &c->dvec, &c->ck_p, &c->ck_s, &c->ck_t, &c->lvl})
     b->release();
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -547,6 +606,96 @@ static int count_stage(bdc_ctx *c, int M, int N, int K, const double *A, int64_t
   return BDC_OK;
 }
 
+// ---- deletion-chain path (kernels_chain.hip) ----
+// Tile width of a group: one contiguous row for the bottom group, 16 columns
+// of the bits below otherwise.
+static int grp_width(const bdc_ctx *c, int g) { return (int)std::min<uint64_t>(16, (uint64_t)1 << c->grpB[g]); }
+
+// q = (1/C) D^d X / d! over the bit groups (top first), then L = ln q.
+static int chain_output_law(bdc_ctx *c, const double *X, int scale_slot, double *qout, double *Lout) {
+  const int n = c->n, d = c->n - c->k;
+  double *lvl = c->lvl.as<double>();
+  const uint64_t ny = (uint64_t)1 << c->k;
+  if (d == 0) {  // k = n: the channel is the identity
+    BDC_LAUNCH(launch_chain_q(X, ny, c->scal.as<double>(), scale_slot, c->inv_c, qout, Lout, c->stream));
+    return BDC_OK;
+  }
+  std::vector<bool> zero(d + 1, true);
+  zero[0] = false;
+  for (int g = 0; g < c->ngrp; ++g) {
+    const int L = c->grpL[g], B = c->grpB[g];
+    for (int t = d - 1; t >= 0; --t) {
+      const int H = n - t - L - B;  // high bits: the groups above after t deletions
+      if (H < 0 || zero[t]) continue;
+      GrpChain gc{};
+      gc.src = t == 0 ? X : lvl + c->lvl_off[t];
+      gc.m = std::min(d - t, L);
+      double fact = 1.0;
+      for (int j = 1; j <= gc.m; ++j) {
+        fact *= j;
+        gc.lvl[j] = lvl + c->lvl_off[t + j];
+        gc.scale[j] = 1.0 / fact;
+        if (!zero[t + j]) gc.acc |= 1u << j;
+      }
+      gc.L = L;
+      gc.B = B;
+      gc.W = grp_width(c, g);
+      gc.nh = (uint64_t)1 << H;
+      gc.ntiles = ((uint64_t)1 << B) / gc.W;
+      BDC_LAUNCH(launch_grp_del(gc, c->stream));
+      for (int j = 1; j <= gc.m; ++j) zero[t + j] = false;
+    }
+  }
+  if (zero[d]) return fail(BDC_ERR_STATE, "chain output law: no path reached %d deletions", d);
+  BDC_LAUNCH(launch_chain_q(lvl + c->lvl_off[d], ny, c->scal.as<double>(), scale_slot, c->inv_c, qout, Lout,
+                            c->stream));
+  return BDC_OK;
+}
+
+// S(x) = sum_y c(x,y) L(y) into level slot 0: the transpose of the output law
+// (groups bottom first, targets ascending, Horner over the sources).
+static int chain_cross(bdc_ctx *c, const double *L) {
+  const int n = c->n, d = c->n - c->k;
+  double *lvl = c->lvl.as<double>();
+  double *top = lvl + c->lvl_off[d];
+  if (L != top) BDC_HIP(hipMemcpyAsync(top, L, ((size_t)8) << c->k, hipMemcpyDeviceToDevice, c->stream));
+  std::vector<bool> zero(d + 1, true);
+  zero[d] = false;
+  for (int g = c->ngrp - 1; g >= 0; --g) {
+    const int Lg = c->grpL[g], B = c->grpB[g];
+    for (int t = 0; t < d; ++t) {
+      const int H = n - t - Lg - B;
+      const int m = std::min(d - t, Lg);
+      if (H < 0 || m == 0) continue;
+      GrpChain gc{};
+      bool any = false;
+      for (int j = 1; j <= m; ++j) {
+        gc.lvl[j] = zero[t + j] ? nullptr : lvl + c->lvl_off[t + j];
+        gc.scale[j] = 1.0 / j;
+        any = any || !zero[t + j];
+      }
+      if (!any) continue;
+      gc.dst = lvl + c->lvl_off[t];
+      gc.acc = zero[t] ? 0u : 1u;
+      gc.m = m;
+      gc.L = Lg;
+      gc.B = B;
+      gc.W = grp_width(c, g);
+      gc.nh = (uint64_t)1 << H;
+      gc.ntiles = ((uint64_t)1 << B) / gc.W;
+      BDC_LAUNCH(launch_grp_ins(gc, c->stream));
+      zero[t] = false;
+    }
+  }
+  if (zero[0]) return fail(BDC_ERR_STATE, "chain cross term: no path reached the input length");
+  return BDC_OK;
+}
+
+// blocks of the chain epilogue (each owns a contiguous range of inputs)
+static int chain_epi_blocks(const bdc_ctx *c) {
+  return (int)std::min<uint64_t>(c->part_blocks, std::max<uint64_t>(1, c->nx / 1024));
+}
+
 // q(y) and L(y) = ln q - mu from weights X (p = scal[scale_slot] * X).
 // mirror: use q_j(y) = q_{k-j}(rev y) for j in JR (X reversal-symmetric, n even).
 // rows = N1/2: X holds only the half space x1 < N1/2 of a complement-symmetric
@@ -556,6 +705,11 @@ static int output_law(bdc_ctx *c, int prec, const double *X, int scale_slot, boo
   const int k = c->k, N1 = c->N1, N2 = c->N2;
   const size_t ny = (size_t)1 << k;
   const Opnd none{nullptr, 0, 0, 0};
+  if (c->chain) {
+    BDC_TRY(timer_begin(c, c->t_g1));
+    BDC_TRY(chain_output_law(c, X, scale_slot, qout, Lout));
+    return timer_end(c, c->t_g1);
+  }
   if (c->n1 != c->n2) mirror = false;
   if (rows != N1 && !(mirror && rows == N1 / 2)) return fail(BDC_ERR_STATE, "half-space output law needs the mirror rule");
   // G1: Tbar = Abar[rows]^T X   (KL x N2, K = rows)
@@ -664,6 +818,7 @@ static int output_law(bdc_ctx *c, int prec, const double *X, int scale_slot, boo
 static int cross_operands(bdc_ctx *c, const double *L, bool mirror = false, int rows = 0, double *amax = nullptr,
                           size_t amax_cap = 0, int *namax = nullptr) {
   if (namax) *namax = 0;
+  if (c->chain) return chain_cross(c, L);
   const int k = c->k, N1 = c->N1, N2 = c->N2;
   // V_j = L_j B_{k-j}^T for all j in JL in one launch of factored
   // single-deletion steps (row y1 of L_j, 2^(k-j) entries, to 2^n2 entries)
@@ -759,7 +914,12 @@ static int cross_term(bdc_ctx *c, int prec, int epi, double *Dout, int *nblocks,
   ea.ldx = c->N2;
   ea.exptab = c->exptab.as<double>();
   BDC_TRY(timer_begin(c, c->t_g2));
-  if (prec == BDC_PREC_FP64) {
+  if (c->chain) {
+    const int nb = chain_epi_blocks(c);
+    BDC_LAUNCH(launch_chain_epi(epi, c->lvl.as<double>(), ea.P, ea.Hc, Dout, c->nx, ea.scal, c->inv_c, ea.exptab,
+                                ea.part, nb, c->stream));
+    if (nblocks) *nblocks = nb;
+  } else if (prec == BDC_PREC_FP64) {
     const Opnd A1{c->abar16.p, c->KLp, 1, 1}, B1{c->g2b.p, c->N2, 1, 0};
     const Opnd A2{c->g2a.p, c->KRp, 1, 0}, B2{c->bbar16.p, 1, c->KRp, 1};
     BDC_LAUNCH(gemm64(rows, c->N2, A1, B1, c->KL, A2, B2, c->KR, nullptr, 0, 0, 1, 0, epi, &ea, c->stream, nblocks));
@@ -791,7 +951,7 @@ static int ensure_dvec(bdc_ctx *c) {
 static int ba_rows(bdc_ctx *c, int prec) {
   const char *e = getenv("BDC_HALF_SPACE");
   if (e && e[0] == '0') return c->N1;
-  if (c->n1 != c->n2 || c->N1 < 2) return c->N1;
+  if (c->chain || c->n1 != c->n2 || c->N1 < 2) return c->N1;
   if (prec == BDC_PREC_SPLIT16 && (c->N1 / 2) % 128) return c->N1;
   return c->N1 / 2;
 }
@@ -1290,6 +1450,27 @@ int bdc_kernel_time(bdc_ctx *c, const char *name, double *avg_ms, int *count) {
 int bdc_work_model(bdc_ctx *c, int prec, double *g1, double *g2, double *bytes) {
   if (!c) return fail(BDC_ERR_ARG, "null context");
   if (prec == BDC_PREC_SPLIT16) BDC_TRY(split_prepare(c));
+  if (c->chain) {
+    // chain path: additions of the deletion steps (2l per output of a step from
+    // length l, l per output of a transposed step) are reported as flops; bytes:
+    // the epilogue reads S, p and H and writes p (32 B per input)
+    const int n = c->n, d = c->n - c->k;
+    double fwd = 0.0, bwd = 0.0;
+    for (int g = 0; g < c->ngrp; ++g)
+      for (int t = 0; t < d; ++t) {
+        const int L = c->grpL[g], H = n - t - L - c->grpB[g];
+        if (H < 0) continue;
+        for (int j = 1; j <= std::min(d - t, L); ++j) {
+          const double outs = std::ldexp(1.0, n - t - j);
+          fwd += outs * 2.0 * (L - j + 1);
+          bwd += outs * 2.0 * (L - j + 1);  // the transposed step into length L - j + 1
+        }
+      }
+    if (g1) *g1 = fwd;
+    if (g2) *g2 = bwd;
+    if (bytes) *bytes = 32.0 * (double)c->nx;
+    return BDC_OK;
+  }
   // a BA step covers the rows x1 < ba_rows of P
   const double nx = (double)c->nx * ((double)ba_rows(c, prec) / c->N1);
   // G1 with the mirror rule contracts only Abar (KL columns); G2 has K = KL + KR.
@@ -1303,6 +1484,11 @@ int bdc_work_model(bdc_ctx *c, int prec, double *g1, double *g2, double *bytes) 
 
 int bdc_split_config(bdc_ctx *c, int *g1_limbs, int *g1_bits, int *g2_limbs, int *g2_bits, int *residuals) {
   if (!c) return fail(BDC_ERR_ARG, "null context");
+  if (c->chain) {
+    c->split.ok = true;  // either precision runs the FP64 chain path
+    return fail(BDC_ERR_ARG, "BDC_{%d,%d} runs the FP64 deletion-chain path (k > %d or k = n): no split operands", c->n,
+                c->k, kDenseMaxK);
+  }
   BDC_TRY(split_prepare(c));
   const Split &sp = c->split;
   if (g1_limbs) *g1_limbs = sp.g1_limbs;
@@ -1341,6 +1527,10 @@ static int g1_splits(int rows, int blocks) {
 
 int split_prepare(bdc_ctx *c) {
   if (c->split.ok) return BDC_OK;
+  if (c->chain) {  // the chain path runs in FP64 at either precision setting
+    c->split.ok = true;
+    return BDC_OK;
+  }
   if (c->N1 % 128 || c->N2 % 128)
     return fail(BDC_ERR_ARG, "split-FP16 path needs n >= 14 (128 x 128 tiles of P); use FP64 for BDC_{%d,%d}", c->n,
                 c->k);
@@ -1503,6 +1693,7 @@ int split_output_law_g1(bdc_ctx *c, const double *X, int rows) {
 }
 
 int split_cross_operands(bdc_ctx *c, const double *L, bool mirror, int rows) {
+  if (c->chain) return chain_cross(c, L);
   // V, W in FP64 (stage contractions); the deletion chain also leaves the block
   // maxima of |V| for the limb exponent when W is a permutation of V
   int namax = 0;

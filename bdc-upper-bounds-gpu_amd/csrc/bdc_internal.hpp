@@ -32,6 +32,8 @@ namespace bdc {
 constexpr int kMaxN = 32;       // backend/config.hpp:24 (MAX_N = 33 -> n <= 32)
 constexpr int kMaxHalf = 16;    // half-length caches up to n' = 16
 constexpr int kPartStride = 4;  // doubles per block partial in the fused epilogue
+constexpr int kDenseMaxK = 20;  // largest k of the dense split contractions (2^k outputs per split)
+constexpr int kMaxGroups = 4;   // bit groups of the deletion-chain path (n <= 32: at most 3)
 
 void set_error(const char *fmt, ...);
 int fail(int code, const char *fmt, ...);
@@ -158,6 +160,13 @@ struct bdc_ctx {
   int plimb_rows = 0;
   int n_resid_a = 0, n_resid_b = 0;
   double g1_chunk_sum = 0, g2_row_sum = 0;  // exactness bounds (max count mass per FP32 accumulation)
+
+  // deletion-chain path (k > 20 or k = n): bit groups top to bottom and the
+  // level buffer (slot t: strings after t deletions, 2^(n-t) doubles at lvl_off[t])
+  bool chain = false;
+  int ngrp = 0, grpL[bdc::kMaxGroups] = {}, grpB[bdc::kMaxGroups] = {};
+  bdc::DevBuf lvl;
+  std::vector<uint64_t> lvl_off;
 
   // dual refinement state
   bdc::DevBuf theta, adam_m, adam_v, grad, dvec;

@@ -133,6 +133,35 @@ struct MirrorW {  // for each j in JR: column offset in Wbar, m = k - j, row off
 int launch_w_from_v(const MirrorW &mw, const double *V, int N2, int n2, double *W, int N1, int KR, int KRp,
                     hipStream_t s);
 
+// ---- deletion-chain path for k close to n (kernels_chain.hip) ----
+// One group of bits of a level buffer viewed as [nh][2^L][2^B], processed by
+// blocks of one high row and W low columns (W a power of two <= 16, W <= 2^B).
+constexpr int kMaxGrpSteps = 13;  // group length <= 13 (2^13 doubles of LDS per column)
+struct GrpChain {
+  const double *src;                   // del: R_t
+  double *dst;                         // ins: target R'_t
+  double *lvl[kMaxGrpSteps + 1];       // del: R_{t+j} (targets); ins: R'_{t+j} (sources, null = zero)
+  double scale[kMaxGrpSteps + 2];      // del: 1/j!; ins: 1/j
+  uint32_t acc;                        // del: bit j set = add into R_{t+j}; ins: bit 0 = add into R'_t
+  int L, B, m, W;                      // group bits, low bits, steps, tile width
+  uint64_t nh, ntiles;                 // high rows, 2^B / W
+};
+// output law: (D_g^j R_t) / j! into R_{t+j}, j = 1..m
+int launch_grp_del(const GrpChain &g, hipStream_t s);
+// cross term: R'_t (=|+=) sum_{j=1..m} (D_g^T)^j R'_{t+j} / j!
+int launch_grp_ins(const GrpChain &g, hipStream_t s);
+// q = scal[slot] * inv_c * R, L = ln q (L may be null); sets the centring mu to 0
+int launch_chain_q(const double *R, uint64_t ny, double *scal, int slot, double inv_c, double *q, double *L,
+                   hipStream_t s);
+// D = Hc - mu - S inv_c; EPI_BA: fused update of P with block partials; EPI_D: D stored
+int launch_chain_epi(int epi, const double *S, double *P, const double *Hc, double *Dout, uint64_t nx,
+                     const double *scal, double inv_c, const double *exptab, double *part, int nblocks,
+                     hipStream_t s);
+// H(x) = sum_y c ln c at every representative (ascending orbit list) by a
+// walk of the distinct subsequences; n - k <= 16
+int launch_entropy_dfs(int n, int k, uint64_t norb, const uint32_t *orbits, const double *logtab, double *Hrep,
+                       hipStream_t s);
+
 // ---- split-FP16 contractions (kernels_split.hip) ----
 // exponents of the 128 x 32 tiles of a P-like array (rows x1 < M, N2 columns), row-major by tile
 int launch_tile_exp(const double *P, int M, int N2, int bits, int32_t *texp, hipStream_t s);

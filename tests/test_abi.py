@@ -33,14 +33,19 @@ def test_supported_validation_without_gpu():
     import bdc_gpu
     L = bdc_gpu.lib()
     assert L.bdc_supported(16, 8) == 1
-    assert L.bdc_supported(16, 16) == 0
+    assert L.bdc_supported(16, 16) == 1  # k = n (the identity channel, C = n bits)
     assert L.bdc_supported(16, 0) == 0
-    # inputs up to 2^32 (the scale config BDC_(32,16)), outputs up to 2^20
+    assert L.bdc_supported(16, 17) == 0
+    assert b"k must be in [1, n]" in L.bdc_last_error()
+    # inputs up to 2^32 (the scale config BDC_(32,16)), outputs up to 2^30
     assert L.bdc_supported(32, 16) == 1
     assert L.bdc_supported(29, 14) == 1
+    assert L.bdc_supported(29, 28) == 1
+    assert L.bdc_supported(31, 30) == 1
     assert L.bdc_supported(33, 16) == 0
     assert b"[2, 32]" in L.bdc_last_error()
-    assert L.bdc_supported(32, 21) == 0
+    assert L.bdc_supported(32, 31) == 0
+    assert b"k <= 30" in L.bdc_last_error()
     assert L.bdc_version() >= 1
 
 
@@ -53,8 +58,8 @@ def test_argument_errors_without_gpu():
     assert L.bdc_create(33, 16, None, ctypes.byref(ctx)) == bdc_gpu.ERR_ARG
     assert ctx.value is None  # cleared on failure
     assert b"n must be in [2, 32]" in L.bdc_last_error()
-    assert L.bdc_create(28, 21, None, ctypes.byref(ctx)) == bdc_gpu.ERR_ARG
-    assert b"k <= 20" in L.bdc_last_error()
+    assert L.bdc_create(28, 29, None, ctypes.byref(ctx)) == bdc_gpu.ERR_ARG
+    assert b"k must be in [1, n]" in L.bdc_last_error()
     assert L.bdc_create(28, 14, None, None) == bdc_gpu.ERR_ARG
     assert L.bdc_iterate(None, bdc_gpu.PREC_FP64, 1, 0.0, None, None) == bdc_gpu.ERR_ARG
     assert L.bdc_certify(None, None) == bdc_gpu.ERR_ARG

@@ -27,9 +27,9 @@ def run(tool, args, cwd):
         (["28"], "Usage"),
         (["33", "16"], "n must be in [2, 32]"),
         (["1", "1"], "n must be in [2, 32]"),
-        (["28", "0"], "k must be in [1, n-1]"),
-        (["28", "28"], "k must be in [1, n-1]"),
-        (["28", "21"], "k <= 20"),
+        (["28", "0"], "k must be in [1, n]"),
+        (["28", "29"], "k must be in [1, n]"),
+        (["32", "31"], "k <= 30"),
         (["28", "14", "0", "0.005", "100", "fp32"], "precision must be fp64 or split16"),
     ],
 )
