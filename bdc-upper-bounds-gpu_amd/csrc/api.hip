@@ -246,6 +246,8 @@ static int make_cache_u16(bdc_ctx *c, int len, int m, DevBuf &dst, const char *d
   return BDC_OK;
 }
 
+static int g64_g1_splits(const bdc_ctx *c, int rows);
+
 // The deletion-chain path (no count caches, FP64) serves k close to n: above
 // k = 20 the dense split contractions would hold 2^k outputs per split, and
 // at k = n both halves are consumed whole.  BDC_CHAIN_PATH=1 forces it (tests).
@@ -472,6 +474,14 @@ int bdc_create(int n, int k, const char *cache_dir, bdc_ctx **out) {
     BDC_TRY_C(c->g2b.alloc((size_t)c->KLp * c->N2 * 8));
     if (hipMemsetAsync(c->g2b.p, 0, c->g2b.bytes, c->stream) != hipSuccess) return cleanup(fail(BDC_ERR_HIP, "memset"));
     c->splitk_elems = (size_t)64 * ny;  // split-K partials for the stage GEMMs: up to 64 splits of 2^k outputs
+    {  // split-K slabs of the aligned FP64 output law (full and half space)
+      size_t need = 0;
+      for (int rows : {c->N1, c->N1 / 2}) {
+        const int ks = g64_g1_splits(c, rows);
+        if (ks > 1 && g64_g1_fits(c->KLp, c->N2, rows, ks)) need = std::max(need, (size_t)ks * c->KLp * c->N2 * 8);
+      }
+      if (need) BDC_TRY_C(c->g64part.alloc(need));
+    }
   } else {
     BDC_TRY_C(chain_setup(c));
     c->splitk_elems = 4096;  // the certificate's three sums
@@ -543,7 +553,7 @@ int bdc_destroy(bdc_ctx *c) {
                          &c->entA, &c->rowptrB, &c->entB, &c->blkA, &c->blkB, &c->restiles[0], &c->restiles[1], &c->g1part, &c->repmask, &c->wordbase, &c->orbits, &c->P,
                          &c->Hc, &c->T, &c->U, &c->qpart, &c->qv, &c->g2a, &c->g2b, &c->splitk, &c->part, &c->scal, &c->exptab,
                          &c->hist, &c->ptexp, &c->vlimb, &c->wlimb, &c->theta, &c->adam_m,
-                         &c->adam_v, &c->grad, &c->dvec, &c->ck_p, &c->ck_s, &c->ck_t, &c->lvl})
+                         &c->adam_v, &c->grad, &c->dvec, &c->ck_p, &c->ck_s, &c->ck_t, &c->lvl, &c->g64part})
     b->release();
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -604,6 +614,19 @@ static int count_stage(bdc_ctx *c, int M, int N, int K, const double *A, int64_t
   }
   *done = true;
   return BDC_OK;
+}
+
+// The aligned FP64 product kernels (launch_g64_*) unless BDC_GEMM64=legacy.
+static bool use_g64() {
+  const char *e = getenv("BDC_GEMM64");
+  return !(e && !strcmp(e, "legacy"));
+}
+// split-K of the aligned FP64 output law: enough blocks (>= 512) for 256 CUs
+static int g64_g1_splits(const bdc_ctx *c, int rows) {
+  const int base = (c->N2 / 128) * (c->KLp / 128);
+  int ks = 1;
+  while (ks < 8 && base * ks < 512 && rows % (16 * ks * 2) == 0 && rows / (ks * 2) >= 256) ks *= 2;
+  return ks;
 }
 
 // ---- deletion-chain path (kernels_chain.hip) ----
@@ -714,7 +737,15 @@ static int output_law(bdc_ctx *c, int prec, const double *X, int scale_slot, boo
   if (rows != N1 && !(mirror && rows == N1 / 2)) return fail(BDC_ERR_STATE, "half-space output law needs the mirror rule");
   // G1: Tbar = Abar[rows]^T X   (KL x N2, K = rows)
   BDC_TRY(timer_begin(c, c->t_g1));
-  if (prec == BDC_PREC_FP64) {
+  const int g64ks = g64_g1_splits(c, rows);
+  if (prec == BDC_PREC_FP64 && use_g64() && g64_g1_fits(c->KLp, N2, rows, g64ks) &&
+      (g64ks == 1 || c->g64part.bytes >= (size_t)g64ks * c->KLp * N2 * 8)) {
+    const size_t slab = (size_t)c->KLp * N2;
+    double *dst = g64ks > 1 ? c->g64part.as<double>() : c->T.as<double>();
+    BDC_LAUNCH(launch_g64_g1(c->KLp, N2, rows, c->abar16.as<uint16_t>(), c->KLp, X, N2, dst, N2, g64ks, (int64_t)slab,
+                             c->stream));
+    if (g64ks > 1) BDC_LAUNCH(launch_reduce_splits(dst, g64ks, (int64_t)slab, slab, c->T.as<double>(), c->stream));
+  } else if (prec == BDC_PREC_FP64) {
     const Opnd A{c->abar16.p, 1, c->KLp, 1}, B{X, N2, 1, 0};
     BDC_LAUNCH(gemm64(c->KL, N2, A, B, rows, none, none, 0, c->T.as<double>(), N2, 1, 1, 0, EPI_STORE, nullptr,
                       c->stream, nullptr));
@@ -919,6 +950,10 @@ static int cross_term(bdc_ctx *c, int prec, int epi, double *Dout, int *nblocks,
     BDC_LAUNCH(launch_chain_epi(epi, c->lvl.as<double>(), ea.P, ea.Hc, Dout, c->nx, ea.scal, c->inv_c, ea.exptab,
                                 ea.part, nb, c->stream));
     if (nblocks) *nblocks = nb;
+  } else if (prec == BDC_PREC_FP64 && use_g64() && g64_g2_fits(rows, c->N2, c->KLp, c->KRp)) {
+    BDC_LAUNCH(launch_g64_g2(rows, c->N2, c->abar16.as<uint16_t>(), c->KLp, c->g2b.as<double>(), c->N2, c->KLp,
+                             c->g2a.as<double>(), c->KRp, c->bbar16.as<uint16_t>(), c->KRp, c->KRp, epi, ea, c->stream,
+                             nblocks));
   } else if (prec == BDC_PREC_FP64) {
     const Opnd A1{c->abar16.p, c->KLp, 1, 1}, B1{c->g2b.p, c->N2, 1, 0};
     const Opnd A2{c->g2a.p, c->KRp, 1, 0}, B2{c->bbar16.p, 1, c->KRp, 1};

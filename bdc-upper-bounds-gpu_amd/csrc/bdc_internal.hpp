@@ -133,6 +133,7 @@ struct bdc_ctx {
   bdc::DevBuf g2a;    // Wbar part: N1 x KRp FP64
   bdc::DevBuf g2b;    // Vbar part: KLp x N2 FP64
   bdc::DevBuf splitk; // split-K partials for stage GEMMs
+  bdc::DevBuf g64part; // split-K slabs of the aligned FP64 output law
   size_t splitk_elems = 0;
   bdc::DevBuf part;   // per-block partials of the fused epilogue
   size_t part_blocks = 0;

@@ -56,6 +56,17 @@ struct EpiArgs {
 int gemm64(int M, int N, const Opnd &A1, const Opnd &B1, int K1, const Opnd &A2, const Opnd &B2, int K2, double *C,
            int64_t crs, int64_t ccs, int splits, int64_t split_stride, int epi, const EpiArgs *ea, hipStream_t s,
            int *nblocks_out);
+// The two large products of a BA step on tile-aligned operands (128 x 128
+// tiles, K stages of 16): the output law C[z] = A^T X over the z-th K range
+// (A u16 [K][lda], X f64 [K][ldb]) and the cross term with the BA epilogue
+// (A1 u16 [M][lda1] x B1 f64 [K1][ldb1] + A2 f64 [M][lda2] x B2^T, B2 u16 [N][ldb2]).
+bool g64_g1_fits(int Mpad, int N, int K, int splits);
+int launch_g64_g1(int Mpad, int N, int K, const uint16_t *A, int64_t lda, const double *X, int64_t ldb, double *C,
+                  int64_t ldc, int splits, int64_t split_stride, hipStream_t s);
+bool g64_g2_fits(int M, int N, int K1, int K2);
+int launch_g64_g2(int M, int N, const uint16_t *A1, int64_t lda1, const double *B1, int64_t ldb1, int K1,
+                  const double *A2, int64_t lda2, const uint16_t *B2, int64_t ldb2, int K2, int epi,
+                  const EpiArgs &ea, hipStream_t s, int *nb);
 // C[i] = sum_z part[z * stride + i] for i < len
 int launch_reduce_splits(const double *part, int splits, int64_t stride, uint64_t len, double *C, hipStream_t s);
 

@@ -271,6 +271,241 @@ int gemm64(int M, int N, const Opnd &A1, const Opnd &B1, int K1, const Opnd &A2,
   return BDC_ERR_ARG;
 }
 
+// ---------------------------------------------------------------------------
+// The two large FP64 products of a BA step on tile-aligned operands (every
+// dimension a multiple of the tile once K is padded to the operands' zero
+// columns): 128 x 128 tiles of four waves (64 x 64 each, 4 x 4 MFMA tiles),
+// K stages of 16 through two LDS buffers (one barrier per stage; the next
+// stage's global loads are issued before the current stage's MFMAs), and
+// 16-byte global loads.  Operand layouts:
+//   A code 0: u16 A(m, k) = a[k * lda + m]    (Abar^T of the output law)
+//   A code 1: u16 A(m, k) = a[m * lda + k]    (Abar of the cross term)
+//   A code 2: f64 A(m, k) = a[m * lda + k]    (Wbar of the cross term)
+//   B code 0: f64 B(k, n) = b[k * ldb + n]    (p, Vbar)
+//   B code 1: u16 B(k, n) = b[n * ldb + k]    (Bbar^T of the cross term)
+constexpr int kG64Tile = 128, kG64BK = 16, kG64Row = kG64Tile + 2;
+
+template <int ACODE, int BCODE>
+__device__ inline void g64_load(const void *a, int64_t lda, const void *b, int64_t ldb, int m0, int n0, int k0,
+                                int tid, double (&ra)[8], double (&rb)[8]) {
+  if constexpr (ACODE == 0) {
+    const int k = tid >> 4, m = (tid & 15) * 8;
+    const uint4 v = *reinterpret_cast<const uint4 *>(static_cast<const uint16_t *>(a) + (int64_t)(k0 + k) * lda + m0 + m);
+    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int i = 0; i < 8; ++i) ra[i] = (double)((w[i >> 1] >> (16 * (i & 1))) & 0xffffu);
+  } else if constexpr (ACODE == 1) {
+    const int m = tid >> 1, k = (tid & 1) * 8;
+    const uint4 v = *reinterpret_cast<const uint4 *>(static_cast<const uint16_t *>(a) + (int64_t)(m0 + m) * lda + k0 + k);
+    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int i = 0; i < 8; ++i) ra[i] = (double)((w[i >> 1] >> (16 * (i & 1))) & 0xffffu);
+  } else {
+    const int m = tid >> 1, k = (tid & 1) * 8;
+    const double2 *p = reinterpret_cast<const double2 *>(static_cast<const double *>(a) + (int64_t)(m0 + m) * lda + k0 + k);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const double2 v = p[i];
+      ra[2 * i] = v.x;
+      ra[2 * i + 1] = v.y;
+    }
+  }
+  if constexpr (BCODE == 0) {
+    const int k = tid >> 4, n = (tid & 15) * 8;
+    const double2 *p = reinterpret_cast<const double2 *>(static_cast<const double *>(b) + (int64_t)(k0 + k) * ldb + n0 + n);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const double2 v = p[i];
+      rb[2 * i] = v.x;
+      rb[2 * i + 1] = v.y;
+    }
+  } else {
+    const int n = tid >> 1, k = (tid & 1) * 8;
+    const uint4 v = *reinterpret_cast<const uint4 *>(static_cast<const uint16_t *>(b) + (int64_t)(n0 + n) * ldb + k0 + k);
+    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int i = 0; i < 8; ++i) rb[i] = (double)((w[i >> 1] >> (16 * (i & 1))) & 0xffffu);
+  }
+}
+
+template <int ACODE, int BCODE>
+__device__ inline void g64_store(double (*As)[kG64Row], double (*Bs)[kG64Row], int tid, const double (&ra)[8],
+                                 const double (&rb)[8]) {
+  if constexpr (ACODE == 0) {
+    const int k = tid >> 4, m = (tid & 15) * 8;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) As[k][m + i] = ra[i];
+  } else {
+    const int m = tid >> 1, k = (tid & 1) * 8;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) As[k + i][m] = ra[i];
+  }
+  if constexpr (BCODE == 0) {
+    const int k = tid >> 4, n = (tid & 15) * 8;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) Bs[k][n + i] = rb[i];
+  } else {
+    const int n = tid >> 1, k = (tid & 1) * 8;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) Bs[k + i][n] = rb[i];
+  }
+}
+
+// acc += A[m0.., kb..ke) B[kb..ke), n0..]; leaves both LDS buffers free (ends on a barrier)
+template <int ACODE, int BCODE>
+__device__ inline void g64_phase(const void *a, int64_t lda, const void *b, int64_t ldb, int m0, int n0, int kb,
+                                 int ke, double (*As)[kG64BK][kG64Row], double (*Bs)[kG64BK][kG64Row],
+                                 double4_ (&acc)[4][4]) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int nst = (ke - kb) / kG64BK;
+  if (nst <= 0) return;
+  double ra[8], rb[8];
+  g64_load<ACODE, BCODE>(a, lda, b, ldb, m0, n0, kb, tid, ra, rb);
+  g64_store<ACODE, BCODE>(As[0], Bs[0], tid, ra, rb);
+  __syncthreads();
+  for (int st = 0; st < nst; ++st) {
+    const int cur = st & 1;
+    const bool more = st + 1 < nst;
+    if (more) g64_load<ACODE, BCODE>(a, lda, b, ldb, m0, n0, kb + (st + 1) * kG64BK, tid, ra, rb);
+#pragma unroll
+    for (int ks = 0; ks < kG64BK / 4; ++ks) {
+      const int kr = ks * 4 + (lane >> 4);
+      double av[4], bv[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) av[i] = As[cur][kr][wm * 64 + i * 16 + (lane & 15)];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) bv[j] = Bs[cur][kr][wn * 64 + j * 16 + (lane & 15)];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i], bv[j], acc[i][j], 0, 0, 0);
+    }
+    if (more) g64_store<ACODE, BCODE>(As[cur ^ 1], Bs[cur ^ 1], tid, ra, rb);
+    __syncthreads();
+  }
+}
+
+// Output law: C[z] = Abar^T X over the K range of split z (C + z * split_stride)
+__global__ void __launch_bounds__(256) k_g64_g1(const uint16_t *__restrict__ A, int64_t lda,
+                                                const double *__restrict__ X, int64_t ldb, int kchunk,
+                                                double *__restrict__ C, int64_t ldc, int64_t split_stride) {
+  __shared__ double As[2][kG64BK][kG64Row];
+  __shared__ double Bs[2][kG64BK][kG64Row];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, wm = wave >> 1, wn = wave & 1;
+  const int m0 = blockIdx.y * kG64Tile, n0 = blockIdx.x * kG64Tile;
+  double4_ acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = (double4_){0.0, 0.0, 0.0, 0.0};
+  const int kb = blockIdx.z * kchunk;
+  g64_phase<0, 0>(A, lda, X, ldb, m0, n0, kb, kb + kchunk, As, Bs, acc);
+  double *Cz = C + (int64_t)blockIdx.z * split_stride;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = m0 + wm * 64 + i * 16 + mfma64_row(lane, r);
+        const int col = n0 + wn * 64 + j * 16 + (lane & 15);
+        Cz[(int64_t)row * ldc + col] = acc[i][j][r];
+      }
+}
+
+// Cross term with the fused BA update (EPI_BA) or the D pass (EPI_D):
+// S = Abar[rows] Vbar + Wbar[rows] Bbar^T, then the epilogue of k_gemm64
+template <int EPI>
+__global__ void __launch_bounds__(256) k_g64_g2(const uint16_t *__restrict__ A1, int64_t lda1,
+                                                const double *__restrict__ B1, int64_t ldb1, int K1,
+                                                const double *__restrict__ A2, int64_t lda2,
+                                                const uint16_t *__restrict__ B2, int64_t ldb2, int K2, EpiArgs ea) {
+  __shared__ double As[2][kG64BK][kG64Row];
+  __shared__ double Bs[2][kG64BK][kG64Row];
+  __shared__ double red[48];
+  __shared__ double etab[32];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, wm = wave >> 1, wn = wave & 1;
+  if (EPI == EPI_BA && tid < 32) etab[tid] = ea.exptab[tid];
+  const int m0 = blockIdx.y * kG64Tile, n0 = blockIdx.x * kG64Tile;
+  double4_ acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = (double4_){0.0, 0.0, 0.0, 0.0};
+  g64_phase<1, 0>(A1, lda1, B1, ldb1, m0, n0, 0, K1, As, Bs, acc);
+  g64_phase<2, 1>(A2, lda2, B2, ldb2, m0, n0, 0, K2, As, Bs, acc);
+  __syncthreads();  // etab visible even when no K stage ran
+  const double mu = ea.scal[kScalMu], dref = ea.scal[kScalDRef];
+  double s0 = 0.0, m1 = -INFINITY, s2 = 0.0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    double hv[4][4], pv[4][4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t x = (int64_t)(m0 + wm * 64 + i * 16 + mfma64_row(lane, r)) * ea.ldx + n0 + wn * 64 + j * 16 +
+                          (lane & 15);
+        hv[j][r] = ea.Hc[x];
+        if constexpr (EPI == EPI_BA) pv[j][r] = ea.P[x];
+      }
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t x = (int64_t)(m0 + wm * 64 + i * 16 + mfma64_row(lane, r)) * ea.ldx + n0 + wn * 64 + j * 16 +
+                          (lane & 15);
+        const double D = hv[j][r] - mu - acc[i][j][r] * ea.inv_c;
+        m1 = fmax(m1, D);
+        if constexpr (EPI == EPI_BA) {
+          const double p = pv[j][r];
+          const double w = p * exp_tang(D - dref, etab);
+          ea.P[x] = w;
+          s0 += w;
+          s2 += p * D;
+        } else {
+          ea.D[x] = D;
+        }
+      }
+  }
+  block_reduce3(s0, m1, s2, red);
+  if (threadIdx.x == 0) {
+    double *pp = ea.part + (int64_t)(blockIdx.y * gridDim.x + blockIdx.x) * kPartStrideDev;
+    pp[0] = s0;
+    pp[1] = m1;
+    pp[2] = s2;
+  }
+}
+
+bool g64_g1_fits(int Mpad, int N, int K, int splits) {
+  return Mpad % kG64Tile == 0 && N % kG64Tile == 0 && splits >= 1 && K % (kG64BK * splits) == 0;
+}
+int launch_g64_g1(int Mpad, int N, int K, const uint16_t *A, int64_t lda, const double *X, int64_t ldb, double *C,
+                  int64_t ldc, int splits, int64_t split_stride, hipStream_t s) {
+  if (!g64_g1_fits(Mpad, N, K, splits) || lda % 8 || ldb % 2) return BDC_ERR_ARG;
+  dim3 grid(N / kG64Tile, Mpad / kG64Tile, splits);
+  hipLaunchKernelGGL(k_g64_g1, grid, dim3(256), 0, s, A, lda, X, ldb, K / splits, C, ldc, split_stride);
+  return hipGetLastError() == hipSuccess ? BDC_OK : BDC_ERR_HIP;
+}
+bool g64_g2_fits(int M, int N, int K1, int K2) {
+  return M % kG64Tile == 0 && N % kG64Tile == 0 && K1 % kG64BK == 0 && K2 % kG64BK == 0;
+}
+int launch_g64_g2(int M, int N, const uint16_t *A1, int64_t lda1, const double *B1, int64_t ldb1, int K1,
+                  const double *A2, int64_t lda2, const uint16_t *B2, int64_t ldb2, int K2, int epi,
+                  const EpiArgs &ea, hipStream_t s, int *nb) {
+  if (!g64_g2_fits(M, N, K1, K2) || lda1 % 8 || ldb1 % 2 || lda2 % 2 || ldb2 % 8) return BDC_ERR_ARG;
+  dim3 grid(N / kG64Tile, M / kG64Tile);
+  if (nb) *nb = (int)(grid.x * grid.y);
+  if (epi == EPI_BA)
+    hipLaunchKernelGGL(k_g64_g2<EPI_BA>, grid, dim3(256), 0, s, A1, lda1, B1, ldb1, K1, A2, lda2, B2, ldb2, K2, ea);
+  else if (epi == EPI_D)
+    hipLaunchKernelGGL(k_g64_g2<EPI_D>, grid, dim3(256), 0, s, A1, lda1, B1, ldb1, K1, A2, lda2, B2, ldb2, K2, ea);
+  else
+    return BDC_ERR_ARG;
+  return hipGetLastError() == hipSuccess ? BDC_OK : BDC_ERR_HIP;
+}
+
 // C[i] = sum_z part[z * stride + i]: four lanes per output (each a quarter of
 // the splits, combined in a fixed order), so small outputs still fill the GPU
 __global__ void k_reduce_splits(const double *__restrict__ part, int splits, int64_t stride, uint64_t len,

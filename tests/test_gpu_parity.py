@@ -493,3 +493,25 @@ def test_runs_are_bitwise_deterministic(bdc, n, k):
             (s0, p0, c0), (s1, p1, c1) = runs
             assert np.array_equal(s0, s1) and np.array_equal(p0, p1), prec
             assert c0 == c1, prec
+
+
+@pytest.mark.parametrize("n,k", [(24, 12), (20, 10), (22, 14)])
+def test_fp64_aligned_kernels_match_generic(bdc, monkeypatch, n, k):
+    """The aligned FP64 product kernels (16-byte loads, two LDS stages) against
+    the generic strided FP64 kernel (BDC_GEMM64=legacy): the same BA steps,
+    certificate and output law to rounding (the sums run in another order)."""
+    say(f"fp64 aligned vs generic n={n} k={k}")
+    rng = np.random.default_rng(n + k)
+    w = rng.random(1 << n)
+    out = {}
+    for mode in ("aligned", "legacy"):
+        monkeypatch.setenv("BDC_GEMM64", mode)
+        with bdc.Engine(n, k) as e:
+            e.set_p_dirichlet(1.0, 7)
+            st = e.iterate(6, bdc.PREC_FP64)
+            out[mode] = (st, e.certify(), e.output_law(w, bdc.PREC_FP64))
+    (sa, ca, qa), (sl, cl, ql) = out["aligned"], out["legacy"]
+    np.testing.assert_allclose(sa, sl, rtol=1e-13, atol=1e-14)
+    assert abs(ca["upper_cert"] - cl["upper_cert"]) < 1e-13
+    assert abs(ca["lower"] - cl["lower"]) < 1e-13
+    np.testing.assert_allclose(qa, ql, rtol=1e-13)
