@@ -248,6 +248,17 @@ static int make_cache_u16(bdc_ctx *c, int len, int m, DevBuf &dst, const char *d
 
 static int g64_g1_splits(const bdc_ctx *c, int rows);
 
+// BDC_QUARTER: unset or 1 = bit-reversed column storage and quarter-space BA
+// steps; "cols" = the column order only; 0 = natural order, half-space steps
+static bool column_order_enabled() {
+  const char *e = getenv("BDC_QUARTER");
+  return !(e && e[0] == '0');
+}
+static bool quarter_enabled() {
+  const char *e = getenv("BDC_QUARTER");
+  return !(e && (e[0] == '0' || !strcmp(e, "cols")));
+}
+
 // The deletion-chain path (no count caches, FP64) serves k close to n: above
 // k = 20 the dense split contractions would hold 2^k outputs per split, and
 // at k = n both halves are consumed whole.  BDC_CHAIN_PATH=1 forces it (tests).
@@ -428,6 +439,17 @@ int bdc_create(int n, int k, const char *cache_dir, bdc_ctx **out) {
                                                  c->stream), "stack B"));
       }
     }
+    // stored column order: bit-reversed columns for even n with half length 8..14
+    if (c->n1 == c->n2 && c->n1 >= 8 && c->n1 <= 14 && column_order_enabled()) c->cr = c->n2;
+    if (c->cr && c->KR) {  // Bbar rows follow the stored columns of P
+      DevBuf tmp;
+      BDC_TRY_C(tmp.alloc(c->bbar16.bytes));
+      BDC_TRY_C(check_launch(launch_permute_rows_u16(c->bbar16.as<uint16_t>(), tmp.as<uint16_t>(), c->cr, c->KRp,
+                                                     c->stream), "permute Bbar"));
+      if (hipStreamSynchronize(c->stream) != hipSuccess) return cleanup(fail(BDC_ERR_HIP, "permute Bbar"));
+      std::swap(c->bbar16.p, tmp.p);  // tmp releases the natural-order copy
+      std::swap(c->bbar16.bytes, tmp.bytes);
+    }
   }
 
   // orbits: representative mask, word ranks, ascending list (main_BAA.cu:46-50)
@@ -473,6 +495,11 @@ int bdc_create(int n, int k, const char *cache_dir, bdc_ctx **out) {
     }
     BDC_TRY_C(c->g2b.alloc((size_t)c->KLp * c->N2 * 8));
     if (hipMemsetAsync(c->g2b.p, 0, c->g2b.bytes, c->stream) != hipSuccess) return cleanup(fail(BDC_ERR_HIP, "memset"));
+    if (c->cr) {
+      BDC_TRY_C(c->T2.alloc((size_t)c->KLp * c->N2 * 8));
+      BDC_TRY_C(c->V2.alloc((size_t)c->KLp * c->N2 * 8));
+      if (hipMemsetAsync(c->V2.p, 0, c->V2.bytes, c->stream) != hipSuccess) return cleanup(fail(BDC_ERR_HIP, "memset"));
+    }
     c->splitk_elems = (size_t)64 * ny;  // split-K partials for the stage GEMMs: up to 64 splits of 2^k outputs
     {  // split-K slabs of the aligned FP64 output law (full and half space)
       size_t need = 0;
@@ -529,7 +556,7 @@ int bdc_create(int n, int k, const char *cache_dir, bdc_ctx **out) {
                                                  c->wordbase.as<uint32_t>(), ltab.as<double>(), hrep.as<double>(),
                                                  c->stream), "entropy"));
     BDC_TRY_C(check_launch(launch_expand_orbits(n, hrep.as<double>(), c->repmask.as<uint64_t>(), c->wordbase.as<uint32_t>(),
-                                                c->inv_c, -c->log_c, c->Hc.as<double>(), c->stream), "expand H"));
+                                                c->inv_c, -c->log_c, c->cr, c->Hc.as<double>(), c->stream), "expand H"));
     if (hipStreamSynchronize(c->stream) != hipSuccess) return cleanup(fail(BDC_ERR_HIP, "entropy sync"));
     hrep.release();
     ltab.release();
@@ -553,7 +580,8 @@ int bdc_destroy(bdc_ctx *c) {
                          &c->entA, &c->rowptrB, &c->entB, &c->blkA, &c->blkB, &c->restiles[0], &c->restiles[1], &c->g1part, &c->repmask, &c->wordbase, &c->orbits, &c->P,
                          &c->Hc, &c->T, &c->U, &c->qpart, &c->qv, &c->g2a, &c->g2b, &c->splitk, &c->part, &c->scal, &c->exptab,
                          &c->hist, &c->ptexp, &c->vlimb, &c->wlimb, &c->theta, &c->adam_m,
-                         &c->adam_v, &c->grad, &c->dvec, &c->ck_p, &c->ck_s, &c->ck_t, &c->lvl, &c->g64part})
+                         &c->adam_v, &c->grad, &c->dvec, &c->ck_p, &c->ck_s, &c->ck_t, &c->lvl, &c->g64part,
+                         &c->T2, &c->V2, &c->qtiles[0], &c->qtiles[1]})
     b->release();
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -753,6 +781,12 @@ static int output_law(bdc_ctx *c, int prec, const double *X, int scale_slot, boo
     BDC_TRY(split_output_law_g1(c, X, rows));
   }
   BDC_TRY(timer_end(c, c->t_g1));
+  // Tbar's columns follow the stored order of X; the stage contractions read them in natural order
+  double *Tn = c->T.as<double>();
+  if (c->cr) {
+    BDC_LAUNCH(launch_permute_cols(c->T.as<double>(), c->T2.as<double>(), (uint64_t)c->KLp, c->cr, c->stream));
+    Tn = c->T2.as<double>();
+  }
   if (!mirror && c->KR) {
     // Ubar = Bbar^T X^T  (KR x N1, K = N2)
     const Opnd A{c->bbar16.p, 1, c->KRp, 1}, B{X, 1, N2, 0};
@@ -780,14 +814,14 @@ static int output_law(bdc_ctx *c, int prec, const double *X, int scale_slot, boo
     }
     if (jb.count <= kMaxChainJobs) {
       jb.row0[jb.count] = rows;
-      const int rc = launch_chain_ins(c->T.as<double>(), qL, c->n2, jb, c->stream);
+      const int rc = launch_chain_ins(Tn, qL, c->n2, jb, c->stream);
       if (rc == BDC_ERR_HIP) return fail(rc, "chain_ins launch failed");
       staged = rc == BDC_OK;
     }
   }
   for (size_t i = 0; i < c->JL.size() && !staged; ++i) {
     const int j = c->JL[i], m = k - j;
-    const double *Tj = c->T.as<double>() + (size_t)c->offL[i] * N2;
+    const double *Tj = Tn + (size_t)c->offL[i] * N2;
     double *dst = qL + i * ny;
     const int d = c->n2 - m;  // symbols B_m deletes
     if (d == 0) {
@@ -846,8 +880,19 @@ static int output_law(bdc_ctx *c, int prec, const double *X, int scale_slot, boo
 // receives one maximum of |Vbar| per row block of the deletion-chain launch,
 // and *namax their number, when that launch formed Vbar and Wbar is its
 // permutation (else *namax = 0).
+static int cross_operands_nat(bdc_ctx *c, const double *L, bool mirror, int rows, double *amax, size_t amax_cap,
+                              int *namax);
+// Vbar in the stored column order of P (V2) when the columns are bit-reversed
+static double *vbar(bdc_ctx *c) { return c->cr ? c->V2.as<double>() : c->g2b.as<double>(); }
 static int cross_operands(bdc_ctx *c, const double *L, bool mirror = false, int rows = 0, double *amax = nullptr,
                           size_t amax_cap = 0, int *namax = nullptr) {
+  BDC_TRY(cross_operands_nat(c, L, mirror, rows, amax, amax_cap, namax));
+  if (c->cr && !c->chain)
+    BDC_LAUNCH(launch_permute_cols(c->g2b.as<double>(), c->V2.as<double>(), (uint64_t)c->KLp, c->cr, c->stream));
+  return BDC_OK;
+}
+static int cross_operands_nat(bdc_ctx *c, const double *L, bool mirror, int rows, double *amax, size_t amax_cap,
+                              int *namax) {
   if (namax) *namax = 0;
   if (c->chain) return chain_cross(c, L);
   const int k = c->k, N1 = c->N1, N2 = c->N2;
@@ -951,11 +996,11 @@ static int cross_term(bdc_ctx *c, int prec, int epi, double *Dout, int *nblocks,
                                 ea.part, nb, c->stream));
     if (nblocks) *nblocks = nb;
   } else if (prec == BDC_PREC_FP64 && use_g64() && g64_g2_fits(rows, c->N2, c->KLp, c->KRp)) {
-    BDC_LAUNCH(launch_g64_g2(rows, c->N2, c->abar16.as<uint16_t>(), c->KLp, c->g2b.as<double>(), c->N2, c->KLp,
+    BDC_LAUNCH(launch_g64_g2(rows, c->N2, c->abar16.as<uint16_t>(), c->KLp, vbar(c), c->N2, c->KLp,
                              c->g2a.as<double>(), c->KRp, c->bbar16.as<uint16_t>(), c->KRp, c->KRp, epi, ea, c->stream,
                              nblocks));
   } else if (prec == BDC_PREC_FP64) {
-    const Opnd A1{c->abar16.p, c->KLp, 1, 1}, B1{c->g2b.p, c->N2, 1, 0};
+    const Opnd A1{c->abar16.p, c->KLp, 1, 1}, B1{vbar(c), c->N2, 1, 0};
     const Opnd A2{c->g2a.p, c->KRp, 1, 0}, B2{c->bbar16.p, 1, c->KRp, 1};
     BDC_LAUNCH(gemm64(rows, c->N2, A1, B1, c->KL, A2, B2, c->KR, nullptr, 0, 0, 1, 0, epi, &ea, c->stream, nblocks));
   } else {
@@ -1017,6 +1062,21 @@ static int reset_scalars(bdc_ctx *c, double pscale) {
   return BDC_OK;
 }
 
+// an x-indexed device array (stored column order) to the host in natural order
+static int copy_x_to_host(bdc_ctx *c, const double *src, double *dst) {
+  if (c->cr) {
+    DevBuf tmp;
+    BDC_TRY(tmp.alloc(c->nx * 8));
+    BDC_LAUNCH(launch_permute_cols(src, tmp.as<double>(), (uint64_t)c->N1, c->cr, c->stream));
+    BDC_HIP(hipMemcpyAsync(dst, tmp.p, c->nx * 8, hipMemcpyDeviceToHost, c->stream));
+    BDC_HIP(hipStreamSynchronize(c->stream));
+    return BDC_OK;
+  }
+  BDC_HIP(hipMemcpyAsync(dst, src, c->nx * 8, hipMemcpyDeviceToHost, c->stream));
+  BDC_HIP(hipStreamSynchronize(c->stream));
+  return BDC_OK;
+}
+
 }  // namespace bdc
 
 extern "C" {
@@ -1059,7 +1119,7 @@ int bdc_set_p_orbits(bdc_ctx *c, const double *q, uint64_t len) {
   BDC_TRY(part.alloc(1025 * 8));
   BDC_HIP(hipMemcpyAsync(d.p, q, len * 8, hipMemcpyHostToDevice, c->stream));
   BDC_LAUNCH(launch_expand_orbits(c->n, d.as<double>(), c->repmask.as<uint64_t>(), c->wordbase.as<uint32_t>(), 1.0, 0.0,
-                                  c->P.as<double>(), c->stream));
+                                  c->cr, c->P.as<double>(), c->stream));
   // sum over all strings (orbit value x orbit size) must be 1
   BDC_LAUNCH(launch_sum(c->P.as<double>(), c->nx, part.as<double>(), 1024, part.as<double>() + 1024, c->stream));
   double total = 0.0;
@@ -1083,8 +1143,8 @@ int bdc_get_p_orbits(bdc_ctx *c, double *q, uint64_t len) {
   if (!c->have_p) return fail(BDC_ERR_STATE, "p is not set");
   DevBuf d;
   BDC_TRY(d.alloc(len * 8));
-  BDC_LAUNCH(launch_gather_orbits(len, c->orbits.as<uint32_t>(), c->P.as<double>(), c->scal.as<double>(), d.as<double>(),
-                                  c->stream));
+  BDC_LAUNCH(launch_gather_orbits(len, c->orbits.as<uint32_t>(), c->P.as<double>(), c->scal.as<double>(), c->cr,
+                                  d.as<double>(), c->stream));
   BDC_HIP(hipMemcpyAsync(q, d.p, len * 8, hipMemcpyDeviceToHost, c->stream));
   BDC_HIP(hipStreamSynchronize(c->stream));
   d.release();
@@ -1106,7 +1166,7 @@ int bdc_set_p_dirichlet(bdc_ctx *c, double alpha, uint64_t seed) {
   BDC_HIP(hipStreamSynchronize(c->stream));
   if (!(total > 0.0)) return fail(BDC_ERR_NUMERIC, "Dirichlet masses sum to %g", total);
   BDC_LAUNCH(launch_expand_orbits(c->n, Q.as<double>(), c->repmask.as<uint64_t>(), c->wordbase.as<uint32_t>(), 1.0 / total,
-                                  0.0, c->P.as<double>(), c->stream));
+                                  0.0, c->cr, c->P.as<double>(), c->stream));
   BDC_HIP(hipStreamSynchronize(c->stream));
   BDC_TRY(reset_scalars(c, 1.0));
   c->have_p = true;
@@ -1239,7 +1299,15 @@ int bdc_output_law(bdc_ctx *c, int prec, const double *w, double *qo) {
   if (!c || !w || !qo) return fail(BDC_ERR_ARG, "null argument");
   if (prec == BDC_PREC_SPLIT16) BDC_TRY(split_prepare(c));
   BDC_TRY(ensure_dvec(c));
-  BDC_HIP(hipMemcpyAsync(c->dvec.p, w, c->nx * 8, hipMemcpyHostToDevice, c->stream));
+  if (c->cr) {  // natural order in, stored order on the device
+    DevBuf tmp;
+    BDC_TRY(tmp.alloc(c->nx * 8));
+    BDC_HIP(hipMemcpyAsync(tmp.p, w, c->nx * 8, hipMemcpyHostToDevice, c->stream));
+    BDC_LAUNCH(launch_permute_cols(tmp.as<double>(), c->dvec.as<double>(), (uint64_t)c->N1, c->cr, c->stream));
+    BDC_HIP(hipStreamSynchronize(c->stream));
+  } else {
+    BDC_HIP(hipMemcpyAsync(c->dvec.p, w, c->nx * 8, hipMemcpyHostToDevice, c->stream));
+  }
   double *q = c->qv.as<double>() + 2 * ((size_t)1 << c->k), *L = c->qv.as<double>() + 3 * ((size_t)1 << c->k);
   BDC_TRY(output_law(c, prec, c->dvec.as<double>(), kScalOne, false, c->N1, q, L));
   BDC_HIP(hipMemcpyAsync(qo, q, ((size_t)1 << c->k) * 8, hipMemcpyDeviceToHost, c->stream));
@@ -1267,16 +1335,12 @@ int bdc_kl(bdc_ctx *c, int prec, const double *log_q, double *d) {
     BDC_TRY(split_cross_operands(c, L, false, c->N1));
   int nb = 0;
   BDC_TRY(cross_term(c, prec, EPI_D, c->dvec.as<double>(), &nb));
-  BDC_HIP(hipMemcpyAsync(d, c->dvec.p, c->nx * 8, hipMemcpyDeviceToHost, c->stream));
-  BDC_HIP(hipStreamSynchronize(c->stream));
-  return BDC_OK;
+  return copy_x_to_host(c, c->dvec.as<double>(), d);
 }
 
 int bdc_entropy_term(bdc_ctx *c, double *h) {
   if (!c || !h) return fail(BDC_ERR_ARG, "null argument");
-  BDC_HIP(hipMemcpyAsync(h, c->Hc.p, c->nx * 8, hipMemcpyDeviceToHost, c->stream));
-  BDC_HIP(hipStreamSynchronize(c->stream));
-  return BDC_OK;
+  return copy_x_to_host(c, c->Hc.as<double>(), h);
 }
 
 int bdc_sample_inputs(bdc_ctx *c, int what, const uint32_t *x, uint64_t len, double *out) {
@@ -1303,7 +1367,8 @@ int bdc_sample_inputs(bdc_ctx *c, int what, const uint32_t *x, uint64_t len, dou
   BDC_TRY(dx.alloc(len * 4));
   BDC_TRY(dout.alloc(len * 8));
   BDC_HIP(hipMemcpyAsync(dx.p, x, len * 4, hipMemcpyHostToDevice, c->stream));
-  BDC_LAUNCH(launch_gather_at(src, dx.as<uint32_t>(), len, c->scal.as<double>(), slot, dout.as<double>(), c->stream));
+  BDC_LAUNCH(launch_gather_at(src, dx.as<uint32_t>(), len, c->scal.as<double>(), slot, c->cr, dout.as<double>(),
+                              c->stream));
   BDC_HIP(hipMemcpyAsync(out, dout.p, len * 8, hipMemcpyDeviceToHost, c->stream));
   BDC_HIP(hipStreamSynchronize(c->stream));
   return BDC_OK;
@@ -1664,6 +1729,48 @@ int split_prepare(bdc_ctx *c) {
     BDC_TRY(c->restiles[h].alloc(t.size() * 4));
     BDC_HIP(hipMemcpyAsync(c->restiles[h].p, t.data(), t.size() * 4, hipMemcpyHostToDevice, c->stream));
   }
+  if (c->cr && quarter_enabled()) {
+    // quarter mode of the BA step: the 128 x 32 tiles of the half space holding a
+    // representative (canonical r <= c left of the middle column, r + c <= N - 1
+    // right of it), without and with residuals, in 8 x 8 patches dealt to the
+    // eight XCDs in turn (block b runs on XCD b mod 8)
+    const int N = (int)N2, TX = N / 32, TY = N / 2 / 128;
+    std::vector<std::vector<int32_t>> lists(2);
+    for (int h = 0; h < 2; ++h) {
+      std::vector<std::vector<int32_t>> queues(8);
+      int patch = 0;
+      for (int py = 0; py < (TY + 7) / 8; ++py)
+        for (int px = 0; px < (TX + 7) / 8; ++px) {
+          std::vector<int32_t> &q = queues[patch % 8];
+          bool any = false;
+          for (int ty = py * 8; ty < std::min(TY, py * 8 + 8); ++ty)
+            for (int tx = px * 8; tx < std::min(TX, px * 8 + 8); ++tx) {
+              const int m0 = ty * 128, n0 = tx * 32;
+              const bool need = n0 < N / 2 ? m0 <= n0 + 31 : m0 + n0 <= N - 1;
+              const bool res = fA[ty] || fB[tx / 2];
+              if (need && res == (h == 1)) {
+                q.push_back(ty * TX + tx);
+                any = true;
+              }
+            }
+          if (any) ++patch;
+        }
+      for (size_t i = 0;; ++i) {
+        bool more = false;
+        for (auto &q : queues)
+          if (i < q.size()) {
+            lists[h].push_back(q[i]);
+            more = true;
+          }
+        if (!more) break;
+      }
+      c->nqtiles[h] = (int)lists[h].size();
+      if (lists[h].empty()) lists[h].push_back(0);
+      BDC_TRY(c->qtiles[h].alloc(lists[h].size() * 4));
+      BDC_HIP(hipMemcpyAsync(c->qtiles[h].p, lists[h].data(), lists[h].size() * 4, hipMemcpyHostToDevice, c->stream));
+    }
+    BDC_HIP(hipStreamSynchronize(c->stream));
+  }
   BDC_TRY(c->blkA.alloc(fA.size()));
   BDC_TRY(c->blkB.alloc(fB.size()));
   BDC_HIP(hipMemcpyAsync(c->blkA.p, fA.data(), fA.size(), hipMemcpyHostToDevice, c->stream));
@@ -1734,7 +1841,7 @@ int split_cross_operands(bdc_ctx *c, const double *L, bool mirror, int rows) {
   int namax = 0;
   BDC_TRY(cross_operands(c, L, mirror, rows, c->part.as<double>(), c->part.bytes / 8, &namax));
   const Split &sp = c->split;
-  BDC_LAUNCH(launch_split_vw(c->g2b.as<double>(), c->KL, c->KLp, c->N2, c->g2a.as<double>(), rows, c->KR, c->KRp,
+  BDC_LAUNCH(launch_split_vw(vbar(c), c->KL, c->KLp, c->N2, c->g2a.as<double>(), rows, c->KR, c->KRp,
                              sp.g2_limbs, sp.g2_bits, c->vlimb.as<_Float16>(), c->wlimb.as<_Float16>(),
                              c->part.as<double>(), namax, c->scal.as<double>(), c->stream));
   return BDC_OK;
@@ -1765,7 +1872,7 @@ int split_cross_term(bdc_ctx *c, int epi, const EpiArgs &ea, int *nb, int rows) 
   a.entA = c->entA.as<int32_t>();
   a.rowptrB = c->rowptrB.as<int32_t>();
   a.entB = c->entB.as<int32_t>();
-  a.V64 = c->g2b.as<double>();
+  a.V64 = vbar(c);
   a.W64 = c->g2a.as<double>();
   a.texp_next = fuse ? c->ptexp.as<int32_t>() : nullptr;
   a.bits_p = sp.g1_bits;
@@ -1773,6 +1880,12 @@ int split_cross_term(bdc_ctx *c, int epi, const EpiArgs &ea, int *nb, int rows) 
   const int h = rows == c->N1 ? 0 : 1;
   a.res_tiles = c->restiles[h].as<int32_t>();
   a.nres_tiles = c->nrestiles[h];
+  // the half-space BA step with bit-reversed columns runs on the quarter's tiles
+  a.quarter = fuse && c->cr && rows == c->N1 / 2 && c->qtiles[0].p ? 1 : 0;
+  a.q_tiles = c->qtiles[0].as<int32_t>();
+  a.qres_tiles = c->qtiles[1].as<int32_t>();
+  a.nq_tiles = c->nqtiles[0];
+  a.nqres_tiles = c->nqtiles[1];
   BDC_LAUNCH(launch_split_g2(a, ea, c->stream, nb));
   if (epi == EPI_BA) {
     c->plimb_valid = fuse;

@@ -134,6 +134,15 @@ struct bdc_ctx {
   bdc::DevBuf g2b;    // Vbar part: KLp x N2 FP64
   bdc::DevBuf splitk; // split-K partials for stage GEMMs
   bdc::DevBuf g64part; // split-K slabs of the aligned FP64 output law
+  // Column order of the x-indexed arrays (P, Hc, D): with cr > 0 the low cr
+  // bits of x (the column x2) are stored bit-reversed (n1 = n2 = cr), so the
+  // reversal x -> rev x is the transpose of the N1 x N2 matrix and the
+  // complement a point reflection.  T2 / V2: Tbar with natural columns (for the
+  // stage contractions) and Vbar with stored columns (for the cross term).
+  int cr = 0;
+  bdc::DevBuf T2, V2;
+  bdc::DevBuf qtiles[2];     // quarter-mode tile lists of the BA step: without / with residuals
+  int nqtiles[2] = {0, 0};
   size_t splitk_elems = 0;
   bdc::DevBuf part;   // per-block partials of the fused epilogue
   size_t part_blocks = 0;

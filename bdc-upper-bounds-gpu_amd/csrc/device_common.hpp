@@ -33,6 +33,15 @@ __host__ __device__ inline int orbit_size_of(uint32_t x, int n) {
   return (x == r || x == rc) ? 2 : 4;
 }
 
+// Storage index of input x when the columns of P are kept in bit-reversed
+// order (the low cr bits of x reversed; cr = 0: natural order).  An
+// involution: it also maps a storage index back to its input.
+__host__ __device__ inline uint64_t store_index(uint64_t x, int cr) {
+  if (cr == 0) return x;
+  const uint64_t m = (1ull << cr) - 1ull;
+  return (x & ~m) | rev_bits((uint32_t)(x & m), cr);
+}
+
 // rank of a representative r among all representatives (ascending), from the
 // representative bitmask and its per-word exclusive popcount prefix.
 __device__ inline uint32_t rank_of_rep(uint32_t r, const uint64_t *mask, const uint32_t *wordbase) {

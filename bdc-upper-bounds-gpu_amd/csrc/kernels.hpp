@@ -18,16 +18,21 @@ int launch_exclusive_scan(const uint32_t *in, uint64_t len, uint32_t *out, uint3
                           hipStream_t s);
 int launch_orbit_list(uint64_t nwords, const uint64_t *mask, const uint32_t *wordbase, uint32_t *orbits, hipStream_t s);
 int launch_fill(double *P, uint64_t len, double val, hipStream_t s);
+// cr: bits of the column index stored reversed (0: natural order), see store_index
 int launch_expand_orbits(int n, const double *Q, const uint64_t *mask, const uint32_t *wordbase, double scale,
-                         double add, double *P, hipStream_t s);
-int launch_gather_orbits(uint64_t norb, const uint32_t *orbits, const double *P, const double *scal, double *Q,
+                         double add, int cr, double *P, hipStream_t s);
+int launch_gather_orbits(uint64_t norb, const uint32_t *orbits, const double *P, const double *scal, int cr, double *Q,
                          hipStream_t s);
+// out[r][c] = in[r][rev(c)] for rows of 2^h doubles (h <= 14)
+int launch_permute_cols(const double *in, double *out, uint64_t rows, int h, hipStream_t s);
+// out[r][:] = in[rev(r)][:] for 2^h rows of width u16
+int launch_permute_rows_u16(const uint16_t *in, uint16_t *out, int h, int width, hipStream_t s);
 int launch_dirichlet_orbits(int n, uint64_t norb, const uint32_t *orbits, double alpha, uint64_t seed, double *Q,
                             double *mass, hipStream_t s);
 int launch_sum(const double *v, uint64_t len, double *part, int np, double *out, hipStream_t s);
 // out[i] = scal[slot] * src[idx[i]] (scale 1 when slot < 0), len <= 2^28
-int launch_gather_at(const double *src, const uint32_t *idx, uint64_t len, const double *scal, int slot, double *out,
-                     hipStream_t s);
+int launch_gather_at(const double *src, const uint32_t *idx, uint64_t len, const double *scal, int slot, int cr,
+                     double *out, hipStream_t s);
 // logtab: ln(1 + j/64) then 1/(1 + j/64), j = 0..64 (130 doubles)
 int launch_entropy_reps(int n, int k, int n1, int n2, const uint64_t *mask, const uint32_t *wordbase,
                         const double *logtab, double *Hrep, hipStream_t s);
@@ -198,6 +203,11 @@ struct SplitG2Args {
   const double *exptab;
   const int32_t *res_tiles;  // 128 x 32 tiles (ty * N2/32 + tx) holding a residual, run by a second launch
   int nres_tiles;
+  // quarter mode (BA step, columns stored bit-reversed, M = N1/2 = N2/2): the
+  // tiles holding representatives, without and with residuals
+  int quarter;
+  const int32_t *q_tiles, *qres_tiles;
+  int nq_tiles, nqres_tiles;
 };
 int launch_split_g2(const SplitG2Args &a, const EpiArgs &ea, hipStream_t s, int *nb);
 

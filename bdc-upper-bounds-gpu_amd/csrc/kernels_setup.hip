@@ -205,44 +205,77 @@ int launch_fill(double *P, uint64_t len, double val, hipStream_t s) {
   return hipGetLastError() == hipSuccess ? BDC_OK : BDC_ERR_HIP;
 }
 
-// full-space expansion of an orbit-indexed array: P[x] = scale * Q[rank(rep(x))] + add
+// full-space expansion of an orbit-indexed array: P[s] = scale * Q[rank(rep(x))] + add
+// for the input x stored at s (store_index with cr column bits reversed)
 __global__ void k_expand_orbits(int n, const double *__restrict__ Q, const uint64_t *__restrict__ mask,
-                                const uint32_t *__restrict__ wordbase, double scale, double add,
+                                const uint32_t *__restrict__ wordbase, double scale, double add, int cr,
                                 double *__restrict__ P) {
   const uint64_t nx = 1ull << n;
-  for (uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; x < nx; x += (uint64_t)gridDim.x * blockDim.x) {
-    const uint32_t r = rep_of((uint32_t)x, n);
-    P[x] = scale * Q[rank_of_rep(r, mask, wordbase)] + add;
+  for (uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; s < nx; s += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t r = rep_of((uint32_t)store_index(s, cr), n);
+    P[s] = scale * Q[rank_of_rep(r, mask, wordbase)] + add;
   }
 }
 int launch_expand_orbits(int n, const double *Q, const uint64_t *mask, const uint32_t *wordbase, double scale,
-                         double add, double *P, hipStream_t s) {
-  hipLaunchKernelGGL(k_expand_orbits, dim3(grid_for(1ull << n)), dim3(256), 0, s, n, Q, mask, wordbase, scale, add, P);
+                         double add, int cr, double *P, hipStream_t s) {
+  hipLaunchKernelGGL(k_expand_orbits, dim3(grid_for(1ull << n)), dim3(256), 0, s, n, Q, mask, wordbase, scale, add, cr,
+                     P);
   return hipGetLastError() == hipSuccess ? BDC_OK : BDC_ERR_HIP;
 }
 
-// Q[i] = scale * P[orbits[i]]  (scale read from a device scalar)
+// Q[i] = scale * P[store(orbits[i])]  (scale read from a device scalar)
 __global__ void k_gather_orbits(uint64_t norb, const uint32_t *__restrict__ orbits, const double *__restrict__ P,
-                                const double *__restrict__ scal, double *__restrict__ Q) {
+                                const double *__restrict__ scal, int cr, double *__restrict__ Q) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < norb) Q[i] = scal[kScalPScale] * P[orbits[i]];
+  if (i < norb) Q[i] = scal[kScalPScale] * P[store_index(orbits[i], cr)];
 }
-int launch_gather_orbits(uint64_t norb, const uint32_t *orbits, const double *P, const double *scal, double *Q,
+int launch_gather_orbits(uint64_t norb, const uint32_t *orbits, const double *P, const double *scal, int cr, double *Q,
                          hipStream_t s) {
   hipLaunchKernelGGL(k_gather_orbits, dim3((unsigned)((norb + 255) / 256)), dim3(256), 0, s, norb, orbits, P, scal,
-                     Q);
+                     cr, Q);
   return hipGetLastError() == hipSuccess ? BDC_OK : BDC_ERR_HIP;
 }
 
-// out[i] = scale * src[idx[i]], scale = scal[slot] (1 when slot < 0)
+// out[i] = scale * src[store(idx[i])], scale = scal[slot] (1 when slot < 0)
 __global__ void k_gather_at(const double *__restrict__ src, const uint32_t *__restrict__ idx, uint64_t len,
-                            const double *__restrict__ scal, int slot, double *__restrict__ out) {
+                            const double *__restrict__ scal, int slot, int cr, double *__restrict__ out) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < len) out[i] = (slot >= 0 ? scal[slot] : 1.0) * src[idx[i]];
+  if (i < len) out[i] = (slot >= 0 ? scal[slot] : 1.0) * src[store_index(idx[i], cr)];
 }
-int launch_gather_at(const double *src, const uint32_t *idx, uint64_t len, const double *scal, int slot, double *out,
-                     hipStream_t s) {
-  hipLaunchKernelGGL(k_gather_at, dim3(grid_for(len)), dim3(256), 0, s, src, idx, len, scal, slot, out);
+int launch_gather_at(const double *src, const uint32_t *idx, uint64_t len, const double *scal, int slot, int cr,
+                     double *out, hipStream_t s) {
+  hipLaunchKernelGGL(k_gather_at, dim3(grid_for(len)), dim3(256), 0, s, src, idx, len, scal, slot, cr, out);
+  return hipGetLastError() == hipSuccess ? BDC_OK : BDC_ERR_HIP;
+}
+
+// out[r][c] = in[r][rev(c)] over rows of 2^h doubles (h <= 14): one row per
+// block, staged in LDS so both the reads and the writes are contiguous.  With
+// rev an involution this both applies and undoes the column order.
+__global__ void __launch_bounds__(1024) k_permute_cols(const double *__restrict__ in, double *__restrict__ out, int h) {
+  extern __shared__ double row[];
+  const uint32_t w = 1u << h;
+  const double *src = in + (uint64_t)blockIdx.x * w;
+  double *dst = out + (uint64_t)blockIdx.x * w;
+  for (uint32_t c = threadIdx.x; c < w; c += blockDim.x) row[c] = src[c];
+  __syncthreads();
+  for (uint32_t c = threadIdx.x; c < w; c += blockDim.x) dst[c] = row[rev_bits(c, h)];
+}
+int launch_permute_cols(const double *in, double *out, uint64_t rows, int h, hipStream_t s) {
+  if (h < 1 || h > 14 || rows == 0 || rows > 0x7fffffffull) return BDC_ERR_ARG;
+  hipLaunchKernelGGL(k_permute_cols, dim3((unsigned)rows), dim3(1024), (size_t)8 << h, s, in, out, h);
+  return hipGetLastError() == hipSuccess ? BDC_OK : BDC_ERR_HIP;
+}
+
+// out[r][:] = in[rev(r)][:] for 2^h rows of `width` u16
+__global__ void k_permute_rows_u16(const uint16_t *__restrict__ in, uint16_t *__restrict__ out, int h, int width) {
+  const uint64_t tot = ((uint64_t)1 << h) * width;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < tot; i += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t r = (uint32_t)(i / width), c = (uint32_t)(i % width);
+    out[i] = in[(uint64_t)rev_bits(r, h) * width + c];
+  }
+}
+int launch_permute_rows_u16(const uint16_t *in, uint16_t *out, int h, int width, hipStream_t s) {
+  hipLaunchKernelGGL(k_permute_rows_u16, dim3(grid_for(((uint64_t)1 << h) * width)), dim3(256), 0, s, in, out, h, width);
   return hipGetLastError() == hipSuccess ? BDC_OK : BDC_ERR_HIP;
 }
 

@@ -372,6 +372,7 @@ struct G2Out {
   int patches;  // 1: XCD patch mapping of tiles
   int M;        // rows x1 computed (N1/2 in the half-space BA step, else N1)
   const double *exptab;
+  int part_off;  // quarter mode: partials at part[part_off + blockIdx.x]
 };
 
 // S[x1][x2] = sum_kappa A16[x1][kappa] V[kappa][x2] + sum_kappa' W[x1][kappa'] B16[x2][kappa']
@@ -401,8 +402,19 @@ struct G2Rows {
   static constexpr int LD = 72;  // halves per LDS row (64 + 8 pad)
 };
 
-template <int L, int EPI, bool RES>
-__global__ void __launch_bounds__(256, 3) k_split_g2(int N2, int KLp, int KRp, const _Float16 *__restrict__ A16,
+// Quarter mode (QTR, the BA step with the columns of P stored bit-reversed, so
+// that P[r][c] = P[c][r] and P[r][c] = P[~r][~c]): only the tiles listed hold
+// representatives.  An element is canonical when r <= c (left half of the
+// columns) or r + c <= N - 1 (right half); a canonical element is written to
+// its place and to its mirror ((c, r) left, (~c, ~r) right, staged through LDS
+// so the mirror rows are written contiguously) and counts twice in the sums,
+// once when it is its own mirror; other elements of a listed tile are written
+// by their canonical partner.  Tile exponents of the new P are combined with
+// atomic maxima (the array is reset to INT_MIN before the launch).  The
+// quarter variant runs at two waves per SIMD: its epilogue needs ~180 VGPRs,
+// and at three waves the four-limb kernel spills in the main loop.
+template <int L, int EPI, bool RES, bool QTR>
+__global__ void __launch_bounds__(256, QTR ? 2 : 3) k_split_g2(int N2, int KLp, int KRp, const _Float16 *__restrict__ A16,
                                                     const _Float16 *__restrict__ vT, const _Float16 *__restrict__ wl,
                                                     const _Float16 *__restrict__ B16, int bits, EpiArgs ea,
                                                     G2Resid rs, G2Out go, const int32_t *__restrict__ tiles) {
@@ -417,7 +429,7 @@ __global__ void __launch_bounds__(256, 3) k_split_g2(int N2, int KLp, int KRp, c
   const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int TX = N2 / BN;
   int ty, tx;
-  if constexpr (RES) {  // tiles with residuals, from a list
+  if constexpr (RES || QTR) {  // tiles from a list (residual tiles, or the quarter's tiles)
     ty = tiles[blockIdx.x] / TX;
     tx = tiles[blockIdx.x] % TX;
   } else if (go.patches) {
@@ -431,7 +443,7 @@ __global__ void __launch_bounds__(256, 3) k_split_g2(int N2, int KLp, int KRp, c
   }
   const int m0 = ty * BM, n0 = tx * BN;
   // tiles touched by a residual of the rounded counts run in the second launch
-  if (!RES && (rs.blkA[m0 / 128] || rs.blkB[n0 / 64])) return;
+  if (!RES && !QTR && (rs.blkA[m0 / 128] || rs.blkB[n0 / 64])) return;
   if (EPI == EPI_BA && tid < 32) etab[tid] = go.exptab[tid];
   constexpr int NP2 = G::NP2;
   const int nst1 = KLp / BK, nst2 = KRp / BK, S = nst1 + nst2 * NP2;  // sub-stages
@@ -613,25 +625,48 @@ __global__ void __launch_bounds__(256, 3) k_split_g2(int N2, int KLp, int KRp, c
   }
   // the accumulators and H are dead now: room for the P tile
   p_loads();
+  const int Nq = N2, halfq = N2 / 2;  // quarter mode: N1 = N2
+  // quarter mode: the tile's mirror values (-1: none) go to LDS rows of 33
+  // doubles; the operand buffer is free since the barrier after the last stage
+  double *tl = reinterpret_cast<double *>(sm);
+  // canonical: row <= lim, its own mirror: row == lim (lim = c left, N - 1 - c right)
+  const int qlim = (n0 + lr < halfq) ? n0 + lr : Nq - 1 - (n0 + lr);
+  const int qrow0 = m0 + wave * 32 + (lane >> 5) * 4;
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
     const int64_t off = (int64_t)((r >> 2) * 8 + (r & 3)) * ldx;
     const double D = dv[r];
-    m1 = fmax(m1, D);
-    if constexpr (EPI == EPI_BA) {
+    if constexpr (QTR) {
+      const int row = qrow0 + (r >> 2) * 8 + (r & 3);
+      const bool canon = row <= qlim, self = row == qlim;
+      const double wt = canon ? (self ? 1.0 : 2.0) : 0.0;
       const double p = pv[r];
       const double w = p * exp_tang(D - dref, etab);
-      __builtin_nontemporal_store(w, Pu + off + lo);
-      s0 += w;
-      s2 = fma(p, D, s2);
-      wmax = fmax(wmax, w);
+      if (canon) {
+        __builtin_nontemporal_store(w, Pu + off + lo);
+        m1 = fmax(m1, D);
+        wmax = fmax(wmax, w);
+      }
+      s0 += wt * w;
+      s2 = fma(wt * p, D, s2);
+      tl[(wave * 32 + mfma32_row(lane, r)) * 33 + lr] = canon && !self ? w : -1.0;
     } else {
-      Pu[off + lo] = D;
+      m1 = fmax(m1, D);
+      if constexpr (EPI == EPI_BA) {
+        const double p = pv[r];
+        const double w = p * exp_tang(D - dref, etab);
+        __builtin_nontemporal_store(w, Pu + off + lo);
+        s0 += w;
+        s2 = fma(p, D, s2);
+        wmax = fmax(wmax, w);
+      } else {
+        Pu[off + lo] = D;
+      }
     }
   }
   block_reduce3(s0, m1, s2, red);
   if (tid == 0) {
-    double *pp = ea.part + (int64_t)(ty * TX + tx) * kPartStrideDev;
+    double *pp = ea.part + (int64_t)(QTR ? go.part_off + (int)blockIdx.x : ty * TX + tx) * kPartStrideDev;
     pp[0] = s0;
     pp[1] = m1;
     pp[2] = s2;
@@ -640,8 +675,59 @@ __global__ void __launch_bounds__(256, 3) k_split_g2(int N2, int KLp, int KRp, c
   // contraction splits P into limbs as it reads it)
   if constexpr (EPI == EPI_BA) {
     wmax = block_max(wmax, red + 16);
-    if (tid == 0) go.texp[(m0 / 128) * go.tcols + n0 / 32] = tile_exponent(wmax, go.bits_p);
+    if constexpr (QTR) {
+      if (tid == 0) atomicMax(&go.texp[(m0 / 128) * go.tcols + n0 / 32], tile_exponent(wmax, go.bits_p));
+    } else {
+      if (tid == 0) go.texp[(m0 / 128) * go.tcols + n0 / 32] = tile_exponent(wmax, go.bits_p);
+    }
   }
+  if constexpr (QTR) {
+    // mirror image of the tile (canonical, non-self elements) from LDS: wave w
+    // writes mirror rows 8w..8w+7 (tile columns), each in two passes of 64
+    // consecutive entries (tile rows), so a store instruction covers 512
+    // contiguous bytes (descending addresses right of the middle column)
+    __syncthreads();
+    const bool left = n0 < halfq;
+    double vlo = -1.0, vhi = -1.0;  // maxima over this lane's entries in tile rows < 64 / >= 64
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const int mr = wave * 8 + (i >> 1), mc = (i & 1) * 64 + lane;
+      const double v = tl[mc * 33 + mr];
+      if (v >= 0.0) {
+        const int64_t grow = left ? n0 + mr : Nq - 1 - (n0 + mr);
+        const int64_t gcol = left ? m0 + mc : Nq - 1 - (m0 + mc);
+        __builtin_nontemporal_store(v, ea.P + grow * ldx + gcol);
+        if (i & 1)
+          vhi = fmax(vhi, v);
+        else
+          vlo = fmax(vlo, v);
+      }
+    }
+    // exponent tiles (32 mirror columns each): tile rows mc in [32g, 32g + 32)
+    // are lanes 32(g & 1).. of the pass g >> 1; reduce within 32-lane halves
+#pragma unroll
+    for (int o = 1; o < 32; o <<= 1) {
+      vlo = fmax(vlo, __shfl_xor(vlo, o, 64));
+      vhi = fmax(vhi, __shfl_xor(vhi, o, 64));
+    }
+    if ((lane & 31) == 0) {
+      const int64_t grow = left ? n0 + wave * 8 : Nq - 1 - (n0 + wave * 8);
+      const int trow = (int)(grow / 128);
+      for (int pass = 0; pass < 2; ++pass) {
+        const double vm = pass ? vhi : vlo;
+        if (vm < 0.0) continue;
+        const int g = pass * 2 + (lane >> 5);
+        const int tcol = left ? (m0 + 32 * g) / 32 : (Nq - m0 - 32 * g - 32) / 32;
+        atomicMax(&go.texp[trow * go.tcols + tcol], tile_exponent(vm, go.bits_p));
+      }
+    }
+  }
+}
+
+// reset of the P tile exponents before a quarter-mode step
+__global__ void k_fill_i32(int32_t *__restrict__ a, int len, int32_t v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < len) a[i] = v;
 }
 
 // ----------------------------------------------------------------- launchers
@@ -718,14 +804,34 @@ int launch_split_g2(const SplitG2Args &a, const EpiArgs &ea, hipStream_t s, int 
   go.exptab = a.exptab;
   if (a.epi == EPI_BA && !a.texp_next) return BDC_ERR_ARG;
   const int blocks = TX * TY;
-  if (nb) *nb = blocks;
+  if (nb && !a.quarter) *nb = blocks;
+  if (a.quarter) {  // the BA step over the quarter's tiles (lists), partials by list position
+    if (a.epi != EPI_BA || a.N1 != a.N2 || a.M != a.N2 / 2) return BDC_ERR_ARG;
+    const int nt = (a.M / 128) * (a.N2 / 32);
+    hipLaunchKernelGGL(k_fill_i32, dim3((nt + 255) / 256), dim3(256), 0, s, a.texp_next, nt, (int32_t)INT32_MIN);
+    if (nb) *nb = a.nq_tiles + a.nqres_tiles;
+#define LQ(LL)                                                                                                    \
+  do {                                                                                                            \
+    go.part_off = 0;                                                                                              \
+    if (a.nq_tiles)                                                                                               \
+      hipLaunchKernelGGL((k_split_g2<LL, EPI_BA, false, true>), dim3(a.nq_tiles), dim3(256), 0, s, a.N2, a.KLp,   \
+                         a.KRp, a.A16, a.vT, a.wl, a.B16, a.bits, ea, rs, go, a.q_tiles);                          \
+    go.part_off = a.nq_tiles;                                                                                     \
+    if (a.nqres_tiles)                                                                                            \
+      hipLaunchKernelGGL((k_split_g2<LL, EPI_BA, true, true>), dim3(a.nqres_tiles), dim3(256), 0, s, a.N2, a.KLp, \
+                         a.KRp, a.A16, a.vT, a.wl, a.B16, a.bits, ea, rs, go, a.qres_tiles);                       \
+  } while (0)
+    BDC_SPLIT_L(a.limbs, LQ)
+#undef LQ
+    return hipGetLastError() == hipSuccess ? BDC_OK : BDC_ERR_HIP;
+  }
 #define LG2(LL, EE)                                                                                               \
   do {                                                                                                            \
-    hipLaunchKernelGGL((k_split_g2<LL, EE, false>), dim3(blocks), dim3(256), 0, s, a.N2, a.KLp, a.KRp, a.A16, a.vT, \
-                       a.wl, a.B16, a.bits, ea, rs, go, nullptr);                                                 \
+    hipLaunchKernelGGL((k_split_g2<LL, EE, false, false>), dim3(blocks), dim3(256), 0, s, a.N2, a.KLp, a.KRp,     \
+                       a.A16, a.vT, a.wl, a.B16, a.bits, ea, rs, go, nullptr);                                    \
     if (a.nres_tiles)                                                                                             \
-      hipLaunchKernelGGL((k_split_g2<LL, EE, true>), dim3(a.nres_tiles), dim3(256), 0, s, a.N2, a.KLp, a.KRp,     \
-                         a.A16, a.vT, a.wl, a.B16, a.bits, ea, rs, go, a.res_tiles);                               \
+      hipLaunchKernelGGL((k_split_g2<LL, EE, true, false>), dim3(a.nres_tiles), dim3(256), 0, s, a.N2, a.KLp,     \
+                         a.KRp, a.A16, a.vT, a.wl, a.B16, a.bits, ea, rs, go, a.res_tiles);                        \
   } while (0)
 #define LG2_E(EE)                \
   switch (a.limbs) {             \

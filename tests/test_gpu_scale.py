@@ -27,6 +27,7 @@ def say(msg):
     sys.stdout.flush()
 
 
+@pytest.mark.timeout(1200)  # setup alone takes about six minutes
 def test_scale_bdc32_16(oracle, capfd):
     import bdc_gpu as bdc
     n, k = 32, 16
