@@ -1572,13 +1572,21 @@ int bdc_work_model(bdc_ctx *c, int prec, double *g1, double *g2, double *bytes) 
     return BDC_OK;
   }
   // a BA step covers the rows x1 < ba_rows of P
-  const double nx = (double)c->nx * ((double)ba_rows(c, prec) / c->N1);
+  const int rows = ba_rows(c, prec);
+  const double nx = (double)c->nx * ((double)rows / c->N1);
   // G1 with the mirror rule contracts only Abar (KL columns); G2 has K = KL + KR.
   if (g1) *g1 = 2.0 * c->KL * nx;
   if (g2) *g2 = 2.0 * (c->KL + c->KR) * nx;
   // G2 epilogue: p read and written, H read (24 B/x) at both precisions (the
   // split16 output law forms p's limbs itself from the FP64 p)
   if (bytes) *bytes = nx * 24.0;
+  if (prec == BDC_PREC_SPLIT16 && c->cr && rows == c->N1 / 2 && c->qtiles[0].p) {
+    // quarter mode: the listed 128 x 32 tiles are contracted and read (p and
+    // H), and the whole half space is written (each element or its mirror)
+    const double tiled = 4096.0 * (c->nqtiles[0] + c->nqtiles[1]);
+    if (g2) *g2 = 2.0 * (c->KL + c->KR) * tiled;
+    if (bytes) *bytes = 16.0 * tiled + 8.0 * nx;
+  }
   return BDC_OK;
 }
 

@@ -486,12 +486,15 @@ def main():
     r = res[head]
     prec = bdc.PREC_SPLIT16 if head == "split" else bdc.PREC_FP64
     g1f, g2f, g2bytes = e.work_model(prec)
+    quarter = head == "split" and g2bytes < 24.0 * (1 << n) / 2 * 0.75
     # Dominant kernel: the cross-term contraction G2 with the fused BA update.
     # Algorithmic bytes are orbit-reduced: per complement/reversal orbit the step
     # reads p and H(x) and writes p once (24 B), as the reference's orbit list
-    # does (main_BAA.cu:46-50, BAA.cu:230-241).  The kernel's own layout moves
-    # 24 B per input of the half space it updates (both members of every reversal
-    # pair), reported beside them as layout bytes.
+    # does (main_BAA.cu:46-50, BAA.cu:230-241).  The kernel's own layout bytes
+    # (from the work model) are reported beside them: 24 B per input of the half
+    # space, or in the split step's quarter mode 16 B per input of the
+    # contracted tiles (about a quarter of the space) plus 8 B written per input
+    # of the half space.
     orbit_bytes = 24.0 * e.num_orbits
     g2s = r["g2_ms"] * 1e-3
     traffic = measured_traffic(n, k, "g2_split" if head == "split" else "g2_fp64")
@@ -527,7 +530,9 @@ def main():
         "dtype": "fp16 limbs x fp32 accumulate + fp64" if head == "split" else "f64",
         "data": "synthetic",
         "config": {"workload": f"BDC_({n},{k}) Blahut-Arimoto step from the uniform p: 2^{n} inputs, "
-                               f"{e.num_orbits} complement/reversal orbits, updated on the half space x1 < 2^{n // 2 - 1}",
+                               f"{e.num_orbits} complement/reversal orbits, updated on the half space x1 < 2^{n // 2 - 1}"
+                               + (" (split step: the reversal-canonical quarter, mirrored on write)"
+                                  if quarter else ""),
                    "n": n, "k": k, "iterations": args.steps, "precision": head},
         "roofline": roof,
         "setup_s": setup_s,

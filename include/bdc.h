@@ -170,7 +170,9 @@ int bdc_reset_kernel_timers(bdc_ctx *ctx, int enable);
 /* algorithmic work of one BA step at the given precision: flops of the two big
  * contractions (counted as FP64-equivalent multiply-adds x 2, whatever the limb
  * count), and the HBM bytes of one launch of the cross-term kernel (G2): p read
- * and written and the entropy term read (24 B per input of the step) */
+ * and written and the entropy term read (24 B per input of the step; in the
+ * quarter mode of the split step, 16 B per input of the contracted tiles plus
+ * 8 B written per input of the half space) */
 int bdc_work_model(bdc_ctx *ctx, int precision, double *g1_flops, double *g2_flops, double *g2_bytes);
 /* split16 configuration (fixed on first split16 use, prepared here if needed):
  * limbs and bits per limb of p in the output-law contraction (g1) and of the

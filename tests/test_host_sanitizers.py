@@ -19,9 +19,10 @@ def build_and_run(cmd, exe, args, tmp_path):
     if shutil.which(cmd[0]) is None:
         pytest.skip(f"{cmd[0]} not available")
     subprocess.run(cmd + ["-o", str(exe)], check=True, capture_output=True, text=True)
-    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=1:abort_on_error=0", UBSAN_OPTIONS="print_stacktrace=1",
-               OMP_NUM_THREADS="4")
-    env.pop("LD_PRELOAD", None)  # the sanitizer runtime must come first
+    # verify_asan_link_order=0: the environment may preload a library ahead of
+    # the sanitizer runtime, which is left in place
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=1:abort_on_error=0:verify_asan_link_order=0",
+               UBSAN_OPTIONS="print_stacktrace=1", OMP_NUM_THREADS="4")
     return subprocess.run([str(exe), *args], capture_output=True, text=True, timeout=300, env=env, cwd=tmp_path)
 
 
