@@ -781,12 +781,10 @@ static int output_law(bdc_ctx *c, int prec, const double *X, int scale_slot, boo
     BDC_TRY(split_output_law_g1(c, X, rows));
   }
   BDC_TRY(timer_end(c, c->t_g1));
-  // Tbar's columns follow the stored order of X; the stage contractions read them in natural order
+  // Tbar's columns follow the stored order of X: the factored stage runs on
+  // them as they are (reversed coordinates, outputs written in natural order);
+  // the other stage forms read a copy in natural order
   double *Tn = c->T.as<double>();
-  if (c->cr) {
-    BDC_LAUNCH(launch_permute_cols(c->T.as<double>(), c->T2.as<double>(), (uint64_t)c->KLp, c->cr, c->stream));
-    Tn = c->T2.as<double>();
-  }
   if (!mirror && c->KR) {
     // Ubar = Bbar^T X^T  (KR x N1, K = N2)
     const Opnd A{c->bbar16.p, 1, c->KRp, 1}, B{X, 1, N2, 0};
@@ -814,10 +812,15 @@ static int output_law(bdc_ctx *c, int prec, const double *X, int scale_slot, boo
     }
     if (jb.count <= kMaxChainJobs) {
       jb.row0[jb.count] = rows;
+      jb.rev = c->cr ? 1 : 0;
       const int rc = launch_chain_ins(Tn, qL, c->n2, jb, c->stream);
       if (rc == BDC_ERR_HIP) return fail(rc, "chain_ins launch failed");
       staged = rc == BDC_OK;
     }
+  }
+  if (!staged && c->cr) {
+    BDC_LAUNCH(launch_permute_cols(c->T.as<double>(), c->T2.as<double>(), (uint64_t)c->KLp, c->cr, c->stream));
+    Tn = c->T2.as<double>();
   }
   for (size_t i = 0; i < c->JL.size() && !staged; ++i) {
     const int j = c->JL[i], m = k - j;
@@ -882,18 +885,21 @@ static int output_law(bdc_ctx *c, int prec, const double *X, int scale_slot, boo
 // permutation (else *namax = 0).
 static int cross_operands_nat(bdc_ctx *c, const double *L, bool mirror, int rows, double *amax, size_t amax_cap,
                               int *namax);
-// Vbar in the stored column order of P (V2) when the columns are bit-reversed
-static double *vbar(bdc_ctx *c) { return c->cr ? c->V2.as<double>() : c->g2b.as<double>(); }
+// Vbar in the stored column order of P when the columns are bit-reversed: the
+// factored stage writes it so (g2b_stored), the other stage forms write natural
+// order into g2b and it is permuted into V2
+static double *vbar(bdc_ctx *c) { return c->cr && !c->g2b_stored ? c->V2.as<double>() : c->g2b.as<double>(); }
 static int cross_operands(bdc_ctx *c, const double *L, bool mirror = false, int rows = 0, double *amax = nullptr,
                           size_t amax_cap = 0, int *namax = nullptr) {
   BDC_TRY(cross_operands_nat(c, L, mirror, rows, amax, amax_cap, namax));
-  if (c->cr && !c->chain)
+  if (c->cr && !c->chain && !c->g2b_stored)
     BDC_LAUNCH(launch_permute_cols(c->g2b.as<double>(), c->V2.as<double>(), (uint64_t)c->KLp, c->cr, c->stream));
   return BDC_OK;
 }
 static int cross_operands_nat(bdc_ctx *c, const double *L, bool mirror, int rows, double *amax, size_t amax_cap,
                               int *namax) {
   if (namax) *namax = 0;
+  c->g2b_stored = false;
   if (c->chain) return chain_cross(c, L);
   const int k = c->k, N1 = c->N1, N2 = c->N2;
   // V_j = L_j B_{k-j}^T for all j in JL in one launch of factored
@@ -915,11 +921,13 @@ static int cross_operands_nat(bdc_ctx *c, const double *L, bool mirror, int rows
     }
     if (jb.count <= kMaxChainJobs) {
       jb.row0[jb.count] = nrows;
+      jb.rev = c->cr ? 1 : 0;  // Vbar straight into the stored column order
       const bool fuse_max = amax && mirror && c->n1 == c->n2 && (size_t)nrows <= amax_cap;
       const int rc = launch_chain_del(L, c->g2b.as<double>(), c->n2, jb, fuse_max ? amax : nullptr, c->stream);
       if (rc == BDC_ERR_HIP) return fail(rc, "chain_del launch failed");
       staged = rc == BDC_OK;
       if (staged && fuse_max && namax) *namax = nrows;
+      c->g2b_stored = staged && c->cr;
     }
   }
   for (size_t i = 0; i < c->JL.size() && !staged; ++i) {
@@ -953,6 +961,7 @@ static int cross_operands_nat(bdc_ctx *c, const double *L, bool mirror, int rows
         if (c->JL[q] == m) mw.offL[i] = c->offL[q];
       if (mw.offL[i] < 0) return fail(BDC_ERR_STATE, "mirror slice for j=%d missing", c->JR[i]);
     }
+    mw.stored = c->g2b_stored ? 1 : 0;
     BDC_LAUNCH(launch_w_from_v(mw, c->g2b.as<double>(), N2, c->n2, c->g2a.as<double>(), rows > 0 ? rows : N1, c->KR,
                                c->KRp, c->stream));
     return BDC_OK;

@@ -137,9 +137,11 @@ struct bdc_ctx {
   // Column order of the x-indexed arrays (P, Hc, D): with cr > 0 the low cr
   // bits of x (the column x2) are stored bit-reversed (n1 = n2 = cr), so the
   // reversal x -> rev x is the transpose of the N1 x N2 matrix and the
-  // complement a point reflection.  T2 / V2: Tbar with natural columns (for the
-  // stage contractions) and Vbar with stored columns (for the cross term).
+  // complement a point reflection.  The factored stage contractions read Tbar
+  // and write Vbar in that order themselves (g2b_stored); the other stage forms
+  // go through T2 / V2, Tbar with natural columns and Vbar with stored columns.
   int cr = 0;
+  bool g2b_stored = false;
   bdc::DevBuf T2, V2;
   bdc::DevBuf qtiles[2];     // quarter-mode tile lists of the BA step: without / with residuals
   int nqtiles[2] = {0, 0};

@@ -138,6 +138,10 @@ struct ChainJobs {
   int row0[kMaxChainJobs + 1];  // first block of each job (prefix sum of the row counts)
   int d[kMaxChainJobs];         // deletions L - m
   int64_t src_off[kMaxChainJobs], src_ld[kMaxChainJobs], dst_off[kMaxChainJobs], dst_ld[kMaxChainJobs];
+  // 1: the long strings (L bits) are in bit-reversed order, the short ones in
+  // natural order (insertion form: source rows reversed, outputs natural;
+  // deletion form: sources natural, outputs reversed)
+  int rev;
 };
 int launch_chain_ins(const double *src, double *dst, int L, const ChainJobs &jobs, hipStream_t s);
 // amax (may be null): one double per block (row), the largest |entry| written
@@ -145,6 +149,7 @@ int launch_chain_del(const double *src, double *dst, int L, const ChainJobs &job
 struct MirrorW {  // for each j in JR: column offset in Wbar, m = k - j, row offset of V_{k-j} in Vbar
   int count;
   int offR[18], m[18], offL[18];
+  int stored;  // 1: the columns of Vbar are in bit-reversed order
 };
 int launch_w_from_v(const MirrorW &mw, const double *V, int N2, int n2, double *W, int N1, int KR, int KRp,
                     hipStream_t s);

@@ -168,14 +168,16 @@ __global__ void k_w_from_v(MirrorW mw, const double *__restrict__ V, int N2, int
   int s = 0;
   while (s + 1 < mw.count && kap >= mw.offR[s + 1]) ++s;
   const int z = kap - mw.offR[s], m = mw.m[s];
-  const uint32_t rz = rev_bits((uint32_t)z, m), rx = rev_bits((uint32_t)x1, n2);
+  // column rev x1 of Vbar, stored at x1 when Vbar's columns are bit-reversed
+  const uint32_t rz = rev_bits((uint32_t)z, m), rx = mw.stored ? (uint32_t)x1 : rev_bits((uint32_t)x1, n2);
   W[(int64_t)x1 * KRp + kap] = V[(int64_t)(mw.offL[s] + rz) * N2 + rx];
 }
 // The same copy tiled: a block covers 64 columns kappa' of Wbar and 32 of its
 // rows, taken as x1 = rev(c << sh) for 32 consecutive c (rows = 2^(n2 - sh)), so
 // that rev x1 = c << sh: the Vbar reads are 32 entries of one row (stride 2^sh)
-// and the Wbar writes 64-entry row segments, transposed through LDS.  (The
-// one-thread-per-entry form reads one Vbar row per entry.)
+// and the Wbar writes 64-entry row segments, transposed through LDS.  With
+// Vbar's columns bit-reversed the rows are x1 = c themselves and the reads
+// contiguous.  (The one-thread-per-entry form reads one Vbar row per entry.)
 __global__ void __launch_bounds__(256) k_w_from_v_tiled(MirrorW mw, const double *__restrict__ V, int N2, int n2,
                                                         double *__restrict__ W, int KR, int KRp, int sh) {
   __shared__ double t[64][33];
@@ -187,7 +189,10 @@ __global__ void __launch_bounds__(256) k_w_from_v_tiled(MirrorW mw, const double
       int s = 0;
       while (s + 1 < mw.count && kap >= mw.offR[s + 1]) ++s;
       const uint32_t rz = rev_bits((uint32_t)(kap - mw.offR[s]), mw.m[s]);
-      t[kl][cl] = V[(int64_t)(mw.offL[s] + rz) * N2 + ((uint32_t)(c0 + cl) << sh)];
+      // (Vbar with bit-reversed columns holds the entry for row x1 at column x1:
+      // the block then takes the rows x1 = c0 .. c0 + 31 themselves)
+      const uint32_t col = mw.stored ? (uint32_t)(c0 + cl) : (uint32_t)(c0 + cl) << sh;
+      t[kl][cl] = V[(int64_t)(mw.offL[s] + rz) * N2 + col];
     }
   }
   __syncthreads();
@@ -195,7 +200,7 @@ __global__ void __launch_bounds__(256) k_w_from_v_tiled(MirrorW mw, const double
   if (kap < KR)
     for (int i = 0; i < 8; ++i) {
       const int r = (tid >> 6) + 4 * i;
-      const uint32_t x1 = rev_bits((uint32_t)(c0 + r) << sh, n2);
+      const uint32_t x1 = mw.stored ? (uint32_t)(c0 + r) : rev_bits((uint32_t)(c0 + r) << sh, n2);
       W[(int64_t)x1 * KRp + kap] = t[kl][r];
     }
 }
@@ -258,8 +263,12 @@ __global__ void __launch_bounds__(1024) k_chain_ins(const double *__restrict__ s
   const double *in = src + jb.src_off[i] + (int64_t)y * jb.src_ld[i];
   double *out = dst + jb.dst_off[i] + (int64_t)y * jb.dst_ld[i];
   const uint32_t tid = threadIdx.x, bd = blockDim.x;
+  // reversed source rows: the steps run on them as they are (c(rev u, rev v) =
+  // c(u, v)), and output v' lands at its natural index rev v'
+  const int m = L - d, rv = jb.rev;
+  auto dsti = [&](uint32_t v) { return rv ? rev_bits(v, m) : v; };
   if (d == 0) {
-    for (uint32_t t = tid; t < (1u << L); t += bd) out[t] = in[t];
+    for (uint32_t t = tid; t < (1u << L); t += bd) out[dsti(t)] = in[t];
     return;
   }
   const double f = factorial(d);
@@ -288,7 +297,7 @@ __global__ void __launch_bounds__(1024) k_chain_ins(const double *__restrict__ s
   }
   __syncthreads();
   if (d == 1) {
-    for (uint32_t v = tid; v < H; v += bd) out[v] = acc[v];
+    for (uint32_t v = tid; v < H; v += bd) out[dsti(v)] = acc[v];
     return;
   }
   double *cur = acc, *nxt = half;
@@ -303,7 +312,7 @@ __global__ void __launch_bounds__(1024) k_chain_ins(const double *__restrict__ s
         a += cur[ins_bit(v, p, 1u)];
       }
       if (s == d)
-        out[v] = a / f;
+        out[dsti(v)] = a / f;
       else
         nxt[v] = a;
     }
@@ -324,15 +333,20 @@ __global__ void __launch_bounds__(1024) k_chain_del(const double *__restrict__ s
   double *out = dst + jb.dst_off[i] + (int64_t)y * jb.dst_ld[i];
   const uint32_t tid = threadIdx.x, bd = blockDim.x;
   double mx = 0.0;
+  // reversed outputs: the steps run on the source read at reversed indices
+  // (c(rev u, rev v) = c(u, v)), so output u' is the natural output rev u'
+  const int rv = jb.rev;
+  auto srci = [&](uint32_t t) { return rv ? rev_bits(t, m) : t; };
   if (d == 0) {
     for (uint32_t t = tid; t < (1u << L); t += bd) {
-      out[t] = in[t];
-      mx = fmax(mx, fabs(in[t]));
+      const double v = in[srci(t)];
+      out[t] = v;
+      mx = fmax(mx, fabs(v));
     }
   } else {
     const double f = factorial(d);
     double *cur = lds, *nxt = lds + (1u << (L - 1));  // each region holds up to 2^(L-1) entries
-    for (uint32_t t = tid; t < (1u << m); t += bd) cur[t] = in[t];
+    for (uint32_t t = tid; t < (1u << m); t += bd) cur[t] = in[srci(t)];
     __syncthreads();
     int l = m;  // bits of cur
     for (int s = 1; s <= d; ++s, ++l) {

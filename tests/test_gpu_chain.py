@@ -7,9 +7,9 @@ against the oracle's full-space sums and its reference-method BA; the
 enumeration entropy kernel is forced (BDC_ENTROPY_DFS=1) and checked against
 the dense channel and the dense sweep kernel.  Native k > 20 contexts run
 against the oracle's BA at n = 22 and 24, k = n against the closed form
-C_{n,n} = n bits (Upper_Bounds.txt:34,72), and BDC_(29,24) at sampled inputs
-(entropy term, the closed form of the first step from uniform p, and the
-certificate's D(x||q)).
+C_{n,n} = n bits (Upper_Bounds.txt:34,72), and BDC_(29,24) and BDC_(31,29) at
+sampled inputs (entropy term, the closed form of the first step from uniform p,
+and the certificate's D(x||q)).
 """
 import math
 import sys
@@ -147,6 +147,37 @@ def test_bdc29_24_sampled(bdc, oracle):
         np.testing.assert_allclose(lp + n * LN2 + st[0, 2], D0, rtol=0, atol=1e-10)
         assert st[0, 1] >= D0.max() - 1e-10
         st = np.vstack([st, e.iterate(2, bdc.PREC_FP64)])
+        assert (np.diff(st[:, 3]) >= -1e-12).all()
+        say(f"BDC_({n},{k}): certificate")
+        cert = e.certify()
+        assert abs(cert["sum_p"] - 1.0) < 1e-12
+        logq = e.log_output()
+        ref = np.array([oracle.kl_x(n, k, int(x), logq) for x in xs])
+        np.testing.assert_allclose(e.sample(bdc.SAMPLE_KL, xs), ref, rtol=0, atol=1e-11)
+        assert cert["upper_cert"] >= ref.max() - 1e-11
+
+
+def test_bdc31_29_sampled(bdc, oracle):
+    """BDC_(31,29) (2^31 inputs; bit groups 9 + 9 + 13, where n = 29 has 8 + 8 +
+    13): entropy term at sampled inputs, the first step from uniform against its
+    closed form, and after two more steps the certificate's D(x||q) against the
+    oracle given the engine's ln q"""
+    n, k = 31, 29
+    C = math.comb(n, k)
+    rng = np.random.default_rng(3129)
+    xs = np.array([0, (1 << n) - 1, int("01" * 15 + "0", 2), 0x0F0F0F0F & ((1 << n) - 1)] +
+                  [int(v) for v in rng.integers(0, 1 << n, size=6, dtype=np.uint64)], dtype=np.uint32)
+    H = np.array([oracle.entropy_x(n, k, int(x)) for x in xs])
+    D0 = H / C - math.log(C) + k * LN2
+    say(f"BDC_({n},{k}): create")
+    with bdc.Engine(n, k) as e:
+        np.testing.assert_allclose(e.sample(bdc.SAMPLE_ENTROPY, xs), H / C - math.log(C), rtol=0, atol=1e-12)
+        e.set_p_uniform()
+        st = e.iterate(1, bdc.PREC_FP64)
+        lp = np.log(e.sample(bdc.SAMPLE_P, xs))
+        np.testing.assert_allclose(lp + n * LN2 + st[0, 2], D0, rtol=0, atol=1e-10)
+        assert st[0, 1] >= D0.max() - 1e-10
+        st = np.vstack([st, e.iterate(2, bdc.PREC_SPLIT16)])
         assert (np.diff(st[:, 3]) >= -1e-12).all()
         say(f"BDC_({n},{k}): certificate")
         cert = e.certify()
