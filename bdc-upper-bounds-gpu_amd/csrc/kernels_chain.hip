@@ -245,6 +245,8 @@ int launch_chain_epi(int epi, const double *S, double *P, const double *Hc, doub
 constexpr int kDfsBlock = 256;
 constexpr int kDfsCk = 3;  // checkpoints at depths 8, 16, 24 (depth 0 is all ones)
 
+// One instantiation per number of deletions d = DMAX (0..16): the band loops
+// are unrolled to exactly d + 1 entries.
 template <int DMAX>
 __global__ void __launch_bounds__(kDfsBlock) k_entropy_dfs(int n, int k, uint64_t first, uint64_t norb,
                                                            const uint32_t *__restrict__ orbits,
@@ -255,7 +257,7 @@ __global__ void __launch_bounds__(kDfsBlock) k_entropy_dfs(int n, int k, uint64_
   for (int i = threadIdx.x; i < 130; i += blockDim.x) lt[i] = logtab[i];
   __syncthreads();
   const double *lnF = lt, *invF = lt + 65;
-  const int d = n - k;
+  constexpr int d = DMAX;  // n - k
   const uint32_t wmask = (2u << d) - 1u;  // d + 1 bits
   const int tb = k >= 2 ? k - 2 : 0;      // depth of the final two-level (or one-level) sum
   const int tid = threadIdx.x;
@@ -387,11 +389,29 @@ int launch_entropy_dfs(int n, int k, uint64_t norb, const uint32_t *orbits, cons
     const uint64_t last = std::min(norb, first + chunk);
     const uint64_t want = (last - first + kDfsBlock - 1) / kDfsBlock;
     const unsigned blocks = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(want, 2048));
-    if (d <= 8)
-      hipLaunchKernelGGL(k_entropy_dfs<8>, dim3(blocks), dim3(kDfsBlock), 0, s, n, k, first, last, orbits, logtab, Hrep);
-    else
-      hipLaunchKernelGGL(k_entropy_dfs<16>, dim3(blocks), dim3(kDfsBlock), 0, s, n, k, first, last, orbits, logtab,
-                         Hrep);
+    const auto launch = [&](auto dd) {
+      hipLaunchKernelGGL(k_entropy_dfs<decltype(dd)::value>, dim3(blocks), dim3(kDfsBlock), 0, s, n, k, first, last,
+                         orbits, logtab, Hrep);
+    };
+    switch (d) {
+      case 0: launch(std::integral_constant<int, 0>{}); break;
+      case 1: launch(std::integral_constant<int, 1>{}); break;
+      case 2: launch(std::integral_constant<int, 2>{}); break;
+      case 3: launch(std::integral_constant<int, 3>{}); break;
+      case 4: launch(std::integral_constant<int, 4>{}); break;
+      case 5: launch(std::integral_constant<int, 5>{}); break;
+      case 6: launch(std::integral_constant<int, 6>{}); break;
+      case 7: launch(std::integral_constant<int, 7>{}); break;
+      case 8: launch(std::integral_constant<int, 8>{}); break;
+      case 9: launch(std::integral_constant<int, 9>{}); break;
+      case 10: launch(std::integral_constant<int, 10>{}); break;
+      case 11: launch(std::integral_constant<int, 11>{}); break;
+      case 12: launch(std::integral_constant<int, 12>{}); break;
+      case 13: launch(std::integral_constant<int, 13>{}); break;
+      case 14: launch(std::integral_constant<int, 14>{}); break;
+      case 15: launch(std::integral_constant<int, 15>{}); break;
+      default: launch(std::integral_constant<int, 16>{}); break;
+    }
     if (hipGetLastError() != hipSuccess) return BDC_ERR_HIP;
     if (verbose) {
       if (hipStreamSynchronize(s) != hipSuccess) return BDC_ERR_HIP;

@@ -80,10 +80,12 @@ def test_forced_chain_ba_matches_oracle(bdc, oracle, monkeypatch, n, k, iters):
         assert np.array_equal(e.iterate_noalloc(iters, bdc.PREC_FP64), st)
 
 
-@pytest.mark.parametrize("n,k", [(16, 10), (18, 12), (17, 15), (14, 7)])
+@pytest.mark.parametrize("n,k", [(16, 10), (18, 12), (17, 15), (14, 7), (16, 13), (16, 12), (18, 10), (19, 10),
+                                 (20, 10), (21, 10), (20, 8), (21, 8), (22, 8), (23, 8), (24, 8)])
 def test_entropy_enumeration_matches_dense_sweep(bdc, monkeypatch, n, k):
     """the distinct-subsequence entropy kernel against the dense 2^k sweep on
-    the same dense-path context shape"""
+    the same dense-path context shape, for every number of deletions d = n - k
+    from 2 to 16 (the kernel is instantiated per d)"""
     say(f"entropy enumeration n={n} k={k}")
     out = {}
     for flag in ("1", "0"):
