@@ -159,7 +159,7 @@ __global__ void __launch_bounds__(512) k_split_g1(int KLp, int Kdim, int lda, in
                                                   int bits, int ks, double *__restrict__ Tpart) {
   constexpr int BM = 256, BN = 32, BK = 64, LD = 72;
   constexpr int ROWS = BM + L * BN;  // A rows, then L limb slabs of BN rows (x2)
-  __shared__ __attribute__((aligned(16))) _Float16 sm[ROWS * LD];
+  __shared__ __attribute__((aligned(16))) _Float16 sm[2][ROWS * LD];  // two stage buffers
   const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int kz = blockIdx.x % ks;
   const int tile = blockIdx.x / ks;
@@ -236,15 +236,18 @@ __global__ void __launch_bounds__(512) k_split_g1(int KLp, int Kdim, int lda, in
       v[q][1] = (v[q][1] - l1) * radix;
     }
   };
-  auto sstore = [&]() {
+  auto sstore = [&](auto buf) {
+    _Float16 *b = sm[decltype(buf)::value];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) *reinterpret_cast<half8 *>(&sm[(rbase + 64 * i) * LD + q8]) = ra[i];
+    for (int i = 0; i < 4; ++i) *reinterpret_cast<half8 *>(&b[(rbase + 64 * i) * LD + q8]) = ra[i];
 #pragma unroll
     for (int l = 0; l < L; ++l)
 #pragma unroll
-      for (int q = 0; q < 2; ++q) *reinterpret_cast<half2_ *>(&sm[(BM + l * BN + 2 * cp + q) * LD + 2 * rp]) = hl[l][q];
+      for (int q = 0; q < 2; ++q) *reinterpret_cast<half2_ *>(&b[(BM + l * BN + 2 * cp + q) * LD + 2 * rp]) = hl[l][q];
   };
-  auto opnd = [&](int row, int kk) -> half8 { return *reinterpret_cast<const half8 *>(&sm[row * LD + kk * 16 + lk]); };
+  auto opnd = [&](auto buf, int row, int kk) -> half8 {
+    return *reinterpret_cast<const half8 *>(&sm[decltype(buf)::value][row * LD + kk * 16 + lk]);
+  };
 
   float16_ acc[L];
   double acc64[16];
@@ -253,29 +256,38 @@ __global__ void __launch_bounds__(512) k_split_g1(int KLp, int Kdim, int lda, in
 #pragma unroll
   for (int l = 0; l < L; ++l) acc[l] = (float16_){0};
   const int nst = krows / BK;
-  // one stage s: store its counts and limbs to LDS; load the counts of stage
-  // s + 1 and p of stage s + 3 into slot `cur` (stage s's p was split in the
-  // previous stage); multiply, splitting p of stage s + 1 (slot `nxt`) between
-  // the groups of multiplications; promote every 128 rows (odd s).  Past the end the last stage is loaded again, so
-  // the body has no branch (`odd` is a compile-time parity: a stage is one
-  // basic block, and the split can be scheduled beside the multiplications).
-  auto step = [&](int s, auto cur, auto nxt, auto odd) {
-    if (s > 0) __syncthreads();
-    const int e = et[decltype(cur)::value];
-    sstore();
-    __syncthreads();
-    gload_a(min(s + 1, nst - 1));
-    gload_x(min(s + 3, nst - 1), cur);
+  // Stage s is multiplied from LDS buffer s % 2 while stage s + 1 (its counts
+  // and limbs already in registers) is stored into the other buffer, so one
+  // barrier per stage suffices: the barrier at the end of stage s - 1 both
+  // frees buffer (s + 1) % 2 and publishes stage s.  Stage s: store stage
+  // s + 1; load the counts of stage s + 2 and p of stage s + 4 into slot
+  // `ld`; multiply stage s, splitting p of stage s + 2 (slot `sp`) between the
+  // groups of multiplications; promote every 128 rows (odd s) with the
+  // exponent recorded when stage s was split (eh, by stage parity).  Past the
+  // end the last stage is loaded again, so the body has no branch (`odd` is a
+  // compile-time parity: a stage is one basic block, and the split can be
+  // scheduled beside the multiplications).
+  int eh[2];
+  auto step = [&](int s, auto ld, auto sp, auto odd) {
+    constexpr int b = decltype(odd)::value ? 1 : 0;
+    using Cur = std::integral_constant<int, b>;
+    using Nxt = std::integral_constant<int, 1 - b>;
+    const int e = eh[b];
+    sstore(Nxt{});
+    gload_a(min(s + 2, nst - 1));
+    gload_x(min(s + 4, nst - 1), ld);
     double v[2][2];
-    split_begin(nxt, v);
+    split_begin(sp, v);
+    eh[b] = et[decltype(sp)::value];
 #pragma unroll
     for (int kk = 0; kk < BK / 16; ++kk) {
-      const half8 a = opnd(wave * 32 + lr, kk);
+      const half8 a = opnd(Cur{}, wave * 32 + lr, kk);
 #pragma unroll
-      for (int l = 0; l < L; ++l) acc[l] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, opnd(BM + l * BN + lr, kk), acc[l], 0, 0, 0);
+      for (int l = 0; l < L; ++l)
+        acc[l] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, opnd(Cur{}, BM + l * BN + lr, kk), acc[l], 0, 0, 0);
       // a share of the next stage's limbs per group of multiplications; the
       // scheduling barrier keeps it there (left alone, the scheduler sinks the
-      // split past the next barrier, to the LDS stores that consume it)
+      // split past the barrier, to the LDS stores that consume it)
 #pragma unroll
       for (int l = (kk * L) / (BK / 16); l < ((kk + 1) * L) / (BK / 16); ++l) split_limb(l, v);
       __builtin_amdgcn_sched_barrier(0);
@@ -294,6 +306,7 @@ __global__ void __launch_bounds__(512) k_split_g1(int KLp, int Kdim, int lda, in
 #pragma unroll
       for (int l = 0; l < L; ++l) acc[l] = (float16_){0};
     }
+    __syncthreads();
   };
   using U0 = std::integral_constant<int, 0>;
   using U1 = std::integral_constant<int, 1>;
@@ -301,6 +314,8 @@ __global__ void __launch_bounds__(512) k_split_g1(int KLp, int Kdim, int lda, in
   static_assert(D == 3, "the unrolled ring below has three slots");
   using E0 = std::false_type;
   using E1 = std::true_type;
+  // prologue: stage 0 into buffer 0; the counts and limbs of stage 1 in
+  // registers; p of stages 2 and 3 in slots 2 and 0
   gload_x(0, U0{});
   gload_x(min(1, nst - 1), U1{});
   gload_x(min(2, nst - 1), U2{});
@@ -308,29 +323,42 @@ __global__ void __launch_bounds__(512) k_split_g1(int KLp, int Kdim, int lda, in
   {
     double v[2][2];
     split_begin(U0{}, v);
+    eh[0] = et[0];
 #pragma unroll
     for (int l = 0; l < L; ++l) split_limb(l, v);
   }
+  sstore(U0{});
+  gload_a(min(1, nst - 1));
+  {
+    double v[2][2];
+    split_begin(U1{}, v);
+    eh[1] = et[1];
+#pragma unroll
+    for (int l = 0; l < L; ++l) split_limb(l, v);
+  }
+  gload_x(min(3, nst - 1), U0{});
+  __syncthreads();
   // six stages per iteration (three ring slots, promotion every second stage):
-  // stage s0 + i uses slot i % 3 and has parity i % 2 since s0 is a multiple of 6
+  // stage s0 + i loads into slot (i + 1) % 3, splits slot (i + 2) % 3 and has
+  // parity i % 2, since s0 is a multiple of 6
   int s0 = 0;
 #pragma unroll 1
   for (; s0 + 6 <= nst; s0 += 6) {
-    step(s0, U0{}, U1{}, E0{});
-    step(s0 + 1, U1{}, U2{}, E1{});
-    step(s0 + 2, U2{}, U0{}, E0{});
-    step(s0 + 3, U0{}, U1{}, E1{});
-    step(s0 + 4, U1{}, U2{}, E0{});
-    step(s0 + 5, U2{}, U0{}, E1{});
+    step(s0, U1{}, U2{}, E0{});
+    step(s0 + 1, U2{}, U0{}, E1{});
+    step(s0 + 2, U0{}, U1{}, E0{});
+    step(s0 + 3, U1{}, U2{}, E1{});
+    step(s0 + 4, U2{}, U0{}, E0{});
+    step(s0 + 5, U0{}, U1{}, E1{});
   }
   // the last nst mod 6 stages (nst is even: K rows come in multiples of 128)
   if (s0 < nst) {
-    step(s0, U0{}, U1{}, E0{});
-    step(s0 + 1, U1{}, U2{}, E1{});
+    step(s0, U1{}, U2{}, E0{});
+    step(s0 + 1, U2{}, U0{}, E1{});
   }
   if (s0 + 2 < nst) {
-    step(s0 + 2, U2{}, U0{}, E0{});
-    step(s0 + 3, U0{}, U1{}, E1{});
+    step(s0 + 2, U0{}, U1{}, E0{});
+    step(s0 + 3, U1{}, U2{}, E1{});
   }
   double *T = Tpart + (int64_t)kz * KLp * ncols;
 #pragma unroll
