@@ -74,9 +74,12 @@ int bdc_generate_cache(int n, int k, uint32_t *out);
 int bdc_write_caches(int n_max, const char *dir);
 
 /* ---- context: load_cache (BAA.cu:183-188) + orbit list (main_BAA.cu:46-50) ---- */
-/* Creates an engine for BDC_{n,k}, 2 <= n <= 32 even... see bdc_supported().
- * cache_dir: directory holding cache_{n'}_{k'} files to load (validated for
- * size); NULL or "" generates the needed half-length caches on the GPU. */
+/* Creates an engine for BDC_{n,k}: 2 <= n <= 32, 1 <= k <= n, k <= 30 (see
+ * bdc_supported()).  cache_dir: directory holding cache_{n'}_{k'} files to load
+ * (existence, exact size and counts <= 65535 checked before any device
+ * allocation: BDC_ERR_IO otherwise); NULL or "" generates the needed
+ * half-length caches on the GPU.  k > 20 and k = n read no caches (the
+ * deletion-chain path). */
 int bdc_create(int n, int k, const char *cache_dir, bdc_ctx **out);
 int bdc_destroy(bdc_ctx *ctx);
 /* 1 if (n,k) is supported by this build, else 0 (message in bdc_last_error) */
