@@ -1,5 +1,6 @@
-"""GPU parity at the full sizes of the precision config BDC_(28,14) (the metric
-workload) and the differentiation config BDC_(28,18), through the C ABI.
+"""GPU parity at the CPU-runnable config BDC_(16,8) run to its tolerance, and at
+the full sizes of the precision config BDC_(28,14) (the metric workload) and the
+differentiation config BDC_(28,18), through the C ABI.
 
 The full 2^28-input arrays are never enumerated on the CPU.  Exact references
 come from the oracle's single-input sums and from weights with a known output
@@ -48,6 +49,49 @@ def random_inputs(n, count, seed):
     xs = [0, (1 << n) - 1, int("01" * (n // 2), 2), (1 << (n // 2)) - 1]
     xs += [int(v) for v in rng.integers(0, 1 << n, size=count, dtype=np.uint64)]
     return np.array(xs, dtype=np.uint32)
+
+
+# ----------------------------------------------------------------- BDC_(16,8)
+def test_bdc16_8_to_tolerance(bdc, oracle):
+    """The CPU-runnable config: FP64 BA from the uniform p until the bound gap
+    mx < 1e-6 bits (the reference's stopping rule, main_BAA.cu:73; here
+    max_x D - I(p) < 1e-6 bits as well).  The first 300 steps against the
+    oracle step by step; the oracle cannot follow the ~6e4 steps to the
+    tolerance, so it certifies the engine's final p instead (I(p) and
+    max_x D(x||q), BAA.cu:533-571, 836-913); split16 from the same start stops
+    within a few steps of FP64 with bounds within 1e-9 bits."""
+    n, k, tol = 16, 8, 1e-6
+    say("BDC_(16,8): 300 steps against the oracle")
+    ref = oracle.BA(n, k)
+    mx_ref = [ref.iterate() for _ in range(300)]
+    with bdc.Engine(n, k) as e:
+        e.set_p_uniform()
+        st = e.iterate(300, bdc.PREC_FP64)
+        np.testing.assert_allclose(st[:, 0], mx_ref, rtol=0, atol=1e-10)
+        np.testing.assert_allclose(e.get_p_orbits(), ref.get_q(), rtol=1e-9)
+        say("BDC_(16,8): FP64 and split16 to mx < 1e-6 bits")
+        runs = {}
+        for prec in (bdc.PREC_FP64, bdc.PREC_SPLIT16):
+            e.set_p_uniform()
+            steps = 0
+            while steps < 400_000:
+                st = e.iterate(50_000, prec, tol=tol)
+                steps += len(st)
+                if st[-1, 0] < tol:
+                    break
+            assert st[-1, 0] < tol, (prec, steps)
+            cert = e.certify()
+            runs[prec] = (steps, cert, e.get_p_orbits())
+        (s64, c64, q64), (s16, c16, _) = runs[bdc.PREC_FP64], runs[bdc.PREC_SPLIT16]
+        assert (c64["upper_cert"] - c64["lower"]) / LN2 < tol
+        assert abs(s16 - s64) <= 5, (s16, s64)
+        assert abs(c16["upper_cert"] - c64["upper_cert"]) / LN2 < 1e-9
+        assert abs(c16["lower"] - c64["lower"]) / LN2 < 1e-9
+    say("BDC_(16,8): the oracle certifies the final FP64 p")
+    ref.set_q(q64)
+    assert abs(ref.rate() - c64["lower"]) / LN2 < 1e-10
+    assert abs(ref.max_kl() - c64["upper_cert"]) / LN2 < 1e-10
+    ref.close()
 
 
 # ----------------------------------------------------------------- BDC_(28,14)
