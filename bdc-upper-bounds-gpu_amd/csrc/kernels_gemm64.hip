@@ -386,6 +386,57 @@ __device__ inline void g64_phase(const void *a, int64_t lda, const void *b, int6
   }
 }
 
+// acc += A1[m0.., 0..K1) B1[0..K1), n0..] + A2[m0.., 0..K2) B2[0..K2), n0..]
+// (the cross term's two K phases, operand codes 1/0 then 2/1) in one loop:
+// only the stage loads and LDS stores switch on the phase (a uniform branch),
+// so the accumulators live in one register set (two loops, one per phase,
+// made the compiler keep a second set and copy between them)
+__device__ inline void g64_two_phase(const void *a1, int64_t lda1, const void *b1, int64_t ldb1, int K1,
+                                     const void *a2, int64_t lda2, const void *b2, int64_t ldb2, int K2, int m0,
+                                     int n0, double (*As)[kG64BK][kG64Row], double (*Bs)[kG64BK][kG64Row],
+                                     double4_ (&acc)[4][4]) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int nst1 = K1 / kG64BK, nst = nst1 + K2 / kG64BK;
+  if (nst <= 0) return;
+  double ra[8], rb[8];
+  auto load = [&](int st) {
+    if (st < nst1)
+      g64_load<1, 0>(a1, lda1, b1, ldb1, m0, n0, st * kG64BK, tid, ra, rb);
+    else
+      g64_load<2, 1>(a2, lda2, b2, ldb2, m0, n0, (st - nst1) * kG64BK, tid, ra, rb);
+  };
+  auto store = [&](int buf, int st) {
+    if (st < nst1)
+      g64_store<1, 0>(As[buf], Bs[buf], tid, ra, rb);
+    else
+      g64_store<2, 1>(As[buf], Bs[buf], tid, ra, rb);
+  };
+  load(0);
+  store(0, 0);
+  __syncthreads();
+  for (int st = 0; st < nst; ++st) {
+    const int cur = st & 1;
+    const bool more = st + 1 < nst;
+    if (more) load(st + 1);
+#pragma unroll
+    for (int ks = 0; ks < kG64BK / 4; ++ks) {
+      const int kr = ks * 4 + (lane >> 4);
+      double av[4], bv[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) av[i] = As[cur][kr][wm * 64 + i * 16 + (lane & 15)];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) bv[j] = Bs[cur][kr][wn * 64 + j * 16 + (lane & 15)];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i], bv[j], acc[i][j], 0, 0, 0);
+    }
+    if (more) store(cur ^ 1, st + 1);
+    __syncthreads();
+  }
+}
+
 // Output law: C[z] = Abar^T X over the K range of split z (C + z * split_stride)
 __global__ void __launch_bounds__(256) k_g64_g1(const uint16_t *__restrict__ A, int64_t lda,
                                                 const double *__restrict__ X, int64_t ldb, int kchunk,
@@ -417,7 +468,7 @@ __global__ void __launch_bounds__(256) k_g64_g1(const uint16_t *__restrict__ A, 
 // Cross term with the fused BA update (EPI_BA) or the D pass (EPI_D):
 // S = Abar[rows] Vbar + Wbar[rows] Bbar^T, then the epilogue of k_gemm64
 template <int EPI>
-__global__ void __launch_bounds__(256) k_g64_g2(const uint16_t *__restrict__ A1, int64_t lda1,
+__global__ void __launch_bounds__(256, 2) k_g64_g2(const uint16_t *__restrict__ A1, int64_t lda1,
                                                 const double *__restrict__ B1, int64_t ldb1, int K1,
                                                 const double *__restrict__ A2, int64_t lda2,
                                                 const uint16_t *__restrict__ B2, int64_t ldb2, int K2, EpiArgs ea) {
@@ -433,8 +484,7 @@ __global__ void __launch_bounds__(256) k_g64_g2(const uint16_t *__restrict__ A1,
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = (double4_){0.0, 0.0, 0.0, 0.0};
-  g64_phase<1, 0>(A1, lda1, B1, ldb1, m0, n0, 0, K1, As, Bs, acc);
-  g64_phase<2, 1>(A2, lda2, B2, ldb2, m0, n0, 0, K2, As, Bs, acc);
+  g64_two_phase(A1, lda1, B1, ldb1, K1, A2, lda2, B2, ldb2, K2, m0, n0, As, Bs, acc);
   __syncthreads();  // etab visible even when no K stage ran
   const double mu = ea.scal[kScalMu], dref = ea.scal[kScalDRef];
   double s0 = 0.0, m1 = -INFINITY, s2 = 0.0;
