@@ -1045,6 +1045,15 @@ static int ba_rows(bdc_ctx *c, int prec) {
   return c->N1 / 2;
 }
 
+// after quarter steps that updated only the canonical elements: the rest of
+// the half space from their partners (before the complement image is written)
+static int complete_mirrors(bdc_ctx *c) {
+  if (!c->p_mirror_stale) return BDC_OK;
+  BDC_LAUNCH(launch_fill_mirror(c->P.as<double>(), c->N2, c->stream));
+  c->p_mirror_stale = false;
+  return BDC_OK;
+}
+
 // one BA step on the device over the rows x1 < rows; stats land in hist[idx]
 static int ba_step(bdc_ctx *c, int prec, int idx, int rows, double *hist) {
   double *q = c->qv.as<double>(), *L = q + ((size_t)1 << c->k);
@@ -1210,7 +1219,7 @@ int bdc_iterate(bdc_ctx *c, int prec, int iters, double tol, bdc_iter_stats *sta
     while (ran < iters) {
       const int nb = std::min(batch, iters - ran);
       const bool snap = nb > 1;
-      bool limb_valid = c->plimb_valid;
+      bool limb_valid = c->plimb_valid, mirror_stale = c->p_mirror_stale;
       int limb_rows = c->plimb_rows;
       if (snap) {
         if (c->ck_p.bytes < pbytes) BDC_TRY(c->ck_p.alloc(pbytes));
@@ -1239,12 +1248,14 @@ int bdc_iterate(bdc_ctx *c, int prec, int iters, double tol, bdc_iter_stats *sta
         if (split) BDC_HIP(hipMemcpyAsync(c->ptexp.p, c->ck_t.p, c->ptexp.bytes, hipMemcpyDeviceToDevice, c->stream));
         c->plimb_valid = limb_valid;
         c->plimb_rows = limb_rows;
+        c->p_mirror_stale = mirror_stale;
         for (int it = 0; it <= stop; ++it) BDC_TRY(ba_step(c, prec, ran + it, rows, c->hist.as<double>()));
       }
       ran += stop + 1;
       break;
     }
   }
+  BDC_TRY(complete_mirrors(c));
   if (ran && rows < c->N1) BDC_LAUNCH(launch_reflect_half(c->P.as<double>(), c->nx, c->stream));
   if (stats && ran) BDC_HIP(hipMemcpyAsync(stats, c->hist.p, (size_t)ran * sizeof(bdc_iter_stats), hipMemcpyDeviceToHost, c->stream));
   double last_mx = c->last_mx_bits;
@@ -1271,6 +1282,7 @@ int bdc_iterate_noalloc(bdc_ctx *c, int prec, int iters, bdc_iter_stats *stats) 
   for (int it = 0; it < iters && rc == BDC_OK; ++it) rc = ba_step(c, prec, it, rows, hist);
   g_forbid_alloc = false;
   if (rc != BDC_OK) return rc;
+  BDC_TRY(complete_mirrors(c));
   if (iters && rows < c->N1) BDC_LAUNCH(launch_reflect_half(c->P.as<double>(), c->nx, c->stream));
   if (stats && iters) BDC_HIP(hipMemcpyAsync(stats, hist, (size_t)iters * sizeof(bdc_iter_stats), hipMemcpyDeviceToHost, c->stream));
   double last_mx = c->last_mx_bits;
@@ -1635,6 +1647,16 @@ static int exact_bits(double mass) {
   return 0;
 }
 
+// the BA step of the split path runs on the quarter's tiles (columns of P stored
+// bit-reversed, the half space, quarter tile lists built)
+static bool quarter_step(const bdc_ctx *c, int rows) { return c->cr && rows == c->N1 / 2 && c->qtiles[0].p; }
+// quarter steps update only the canonical elements and the output law reads
+// the triangle; BDC_QUARTER_TRI=0 writes both members of each reversal pair
+static bool tri_reads() {
+  const char *e = getenv("BDC_QUARTER_TRI");
+  return !(e && e[0] == '0');
+}
+
 // G1 split-K: enough blocks (>= 512) for the 256 CUs, each K range >= 1024 rows
 static int g1_splits(int rows, int blocks) {
   int ks = 1;
@@ -1835,8 +1857,15 @@ int split_output_law_g1(bdc_ctx *c, const double *X, int rows) {
   if (rows % 128) return fail(BDC_ERR_ARG, "split output law needs a multiple of 128 rows (%d)", rows);
   // tile exponents of X: written by the previous BA update when X is the current P
   const bool isP = X == c->P.as<double>();
-  if (!(isP && c->plimb_valid && c->plimb_rows == rows))
+  // P of a quarter step: read from the triangle whenever the mirrors are stale.
+  // The exponent of a chunk of a strip (128 rows x 32 columns) is that of the
+  // values read, whichever element holds them: the tile exponent of the
+  // complete P here, the quarter cross term's accumulation after a step.
+  const int tri = isP && quarter_step(c, rows) && (tri_reads() || c->p_mirror_stale) ? 1 : 0;
+  if (!(isP && c->plimb_valid && c->plimb_rows == rows)) {
+    if (c->p_mirror_stale) return fail(BDC_ERR_STATE, "tile exponents of a stale half space");
     BDC_LAUNCH(launch_tile_exp(X, rows, c->N2, sp.g1_bits, c->ptexp.as<int32_t>(), c->stream));
+  }
   c->plimb_valid = isP;
   c->plimb_rows = rows;
   const int ks = g1_splits(rows, (c->N2 / 32) * ((c->KLp + 255) / 256));
@@ -1845,7 +1874,7 @@ int split_output_law_g1(bdc_ctx *c, const double *X, int rows) {
   if (ks > 1 && c->g1part.bytes < ks * slab * 8) BDC_TRY(c->g1part.alloc(ks * slab * 8));
   double *dstT = ks > 1 ? c->g1part.as<double>() : c->T.as<double>();
   BDC_LAUNCH(launch_split_g1(c->KLp, rows, c->N1, c->N2, c->abarTh.as<_Float16>(), X, c->ptexp.as<int32_t>(),
-                             sp.g1_limbs, sp.g1_bits, ks, dstT, c->stream));
+                             sp.g1_limbs, sp.g1_bits, ks, dstT, tri, c->stream));
   if (ks > 1) BDC_LAUNCH(launch_reduce_splits(dstT, ks, (int64_t)slab, (uint64_t)slab, c->T.as<double>(), c->stream));
   BDC_LAUNCH(launch_resid_g1(c->residA.as<int32_t>(), c->n_resid_a, X, rows, c->N2, c->T.as<double>(), c->stream));
   return BDC_OK;
@@ -1897,8 +1926,11 @@ int split_cross_term(bdc_ctx *c, int epi, const EpiArgs &ea, int *nb, int rows) 
   const int h = rows == c->N1 ? 0 : 1;
   a.res_tiles = c->restiles[h].as<int32_t>();
   a.nres_tiles = c->nrestiles[h];
-  // the half-space BA step with bit-reversed columns runs on the quarter's tiles
-  a.quarter = fuse && c->cr && rows == c->N1 / 2 && c->qtiles[0].p ? 1 : 0;
+  // the half-space BA step with bit-reversed columns runs on the quarter's tiles,
+  // updating the canonical elements only (the output law reads the triangle)
+  a.quarter = fuse && quarter_step(c, rows) ? 1 : 0;
+  a.mirror = tri_reads() ? 0 : 1;
+  if (a.quarter && !a.mirror) c->p_mirror_stale = true;
   a.q_tiles = c->qtiles[0].as<int32_t>();
   a.qres_tiles = c->qtiles[1].as<int32_t>();
   a.nq_tiles = c->nqtiles[0];

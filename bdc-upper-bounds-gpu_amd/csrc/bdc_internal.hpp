@@ -120,6 +120,7 @@ struct bdc_ctx {
   bdc::DevBuf P;   // p-tilde (unnormalised), FP64; p = pscale * P
   bdc::DevBuf Hc;  // H(x)/C - ln C, FP64
   bool have_p = false;
+  bool p_mirror_stale = false;  // quarter steps left the non-canonical half-space elements stale
   double last_mx_bits = NAN;  // mx of the step that produced the current p (NaN if p was set)
   bool have_h = false;
   bool have_d = false;  // dvec holds D(x||q) of the last bdc_certify

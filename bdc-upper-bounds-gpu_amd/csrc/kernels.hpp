@@ -192,8 +192,10 @@ int launch_split_vw(const double *V, int KL, int KLp, int N2, const double *W, i
                     int bits, _Float16 *vT, _Float16 *wl, double *part, int np_pre, double *scal, hipStream_t s);
 // ks partial slabs (ks * KLp * ncols FP64) of Tbar = aT X over the first Kdim rows of the FP64 array X
 // (row stride ncols; split into limbs on the fly with the tile exponents texp), aT row stride lda
+// tri: X = P of the quarter-layout BA step (Kdim = ncols / 2), read from the
+// triangle of canonical elements (the others may be stale)
 int launch_split_g1(int KLp, int Kdim, int lda, int ncols, const _Float16 *aT, const double *X, const int32_t *texp,
-                    int limbs, int bits, int ks, double *Tpart, hipStream_t s);
+                    int limbs, int bits, int ks, double *Tpart, int tri, hipStream_t s);
 // residual corrections of G1 from the rows x1 < rows
 int launch_resid_g1(const int32_t *res, int nres, const double *X, int rows, int N2, double *T, hipStream_t s);
 struct SplitG2Args {
@@ -213,7 +215,10 @@ struct SplitG2Args {
   int quarter;
   const int32_t *q_tiles, *qres_tiles;
   int nq_tiles, nqres_tiles;
+  int mirror;  // quarter mode: also write the mirror of every canonical element
 };
 int launch_split_g2(const SplitG2Args &a, const EpiArgs &ea, hipStream_t s, int *nb);
+// the non-canonical elements of a quarter-layout half space (N/2 x N) from their partners
+int launch_fill_mirror(double *P, int N, hipStream_t s);
 
 }  // namespace bdc

@@ -16,6 +16,7 @@
 // contraction reads P limbs transposed ([l][x2][x1]) so that its K index x1 is
 // contiguous.  The BA step keeps only the rows x1 < N1/2 of P (the other half
 // is its complement image), so K runs over those rows.
+#include <climits>
 #include <type_traits>
 
 #include "device_common.hpp"
@@ -153,7 +154,18 @@ __global__ void k_split_w(const double *__restrict__ W, int64_t len, int KR, int
 // MFMA operand reads conflict-free.  The FP32 sums (exact integers) are
 // promoted to FP64 every 128 K rows, the height of an exponent tile.  ks > 1
 // splits K (partial slabs, reduced afterwards).
-template <int L>
+//
+// TRI (the BA step in the quarter layout: X = P with its columns stored
+// bit-reversed, Kdim = N/2 rows of N = ncols columns, and only the canonical
+// elements current, r <= c left of the middle column and r + c <= N - 1 right
+// of it): column c of P is read from the triangle.  Rows above the diagonal
+// band of the block's 32 columns are read in place; rows below it from the
+// partner element, P[r][c] = P[c][r] (left) or P[N-1-c][N-1-r] (right), i.e.
+// transposed, 8 bytes per element; the one stage holding the band picks each
+// element's source.  texp[chunk][strip] is the exponent of the 32 x 128 values
+// read, which the quarter cross term accumulates from the tiles they come from
+// (and which equals the tile exponent of the complete P).
+template <int L, bool TRI>
 __global__ void __launch_bounds__(512) k_split_g1(int KLp, int Kdim, int lda, int ncols, const _Float16 *__restrict__ aT,
                                                   const double *__restrict__ X, const int32_t *__restrict__ texp,
                                                   int bits, int ks, double *__restrict__ Tpart) {
@@ -182,6 +194,19 @@ __global__ void __launch_bounds__(512) k_split_g1(int KLp, int Kdim, int lda, in
   const char *cX = reinterpret_cast<const char *>(X + (int64_t)kbase * ncols + n0);
   const uint32_t offA = (uint32_t)(rbase * lda + q8) * 2u;
   const uint32_t offX = (uint32_t)(2 * rp * ncols + 2 * cp) * 8u;
+  // triangle reads.  Byte offsets from X (< 2^30: quarter layouts have n <= 28): element (r, c) in
+  // place at ((r0 + i) ncols + n0 + q) 8 + offX, transposed at ut(r0, i, q) +
+  // offT; it is canonical iff dq[q] <= -(r0 + i)
+  const bool left = n0 < ncols / 2;
+  const uint32_t offT = left ? (uint32_t)(2 * cp * ncols + 2 * rp) * 8u
+                             : (uint32_t)((30 - 2 * cp) * ncols + 30 - 2 * rp) * 8u;
+  auto ut = [&](int r0, int i, int q) -> uint32_t {
+    return (uint32_t)(left ? ((int64_t)(n0 + q) * ncols + r0 + i) * 8
+                           : ((int64_t)(ncols - 31 - n0 - q) * ncols + ncols - 31 - r0 - i) * 8);
+  };
+  int dq[2];
+#pragma unroll
+  for (int q = 0; q < 2; ++q) dq[q] = 2 * rp - (left ? n0 + 2 * cp + q : ncols - 1 - n0 - 2 * cp - q);
   // Prefetch: the counts (L2 hits: every block reads the same rows of aT) are
   // loaded one stage ahead into one register slot; p (an HBM stream, too few
   // bytes in flight per CU to cover its latency with one stage) comes through
@@ -210,9 +235,29 @@ __global__ void __launch_bounds__(512) k_split_g1(int KLp, int Kdim, int lda, in
   auto gload_x = [&](int s, auto slot) {
     constexpr int u = decltype(slot)::value;
     const char *xs = cX + (int64_t)s * BK * ncols * 8;  // stage base (64-bit, uniform)
+    const int r0 = kbase + s * BK;
+    if constexpr (!TRI) {
 #pragma unroll
-    for (int i = 0; i < 2; ++i) rx[u][i] = *reinterpret_cast<const double2 *>(xs + (int64_t)i * ncols * 8 + offX);
-    et[u] = texp[((kbase + s * BK) / 128) * tcols + n0 / 32];
+      for (int i = 0; i < 2; ++i) rx[u][i] = *reinterpret_cast<const double2 *>(xs + (int64_t)i * ncols * 8 + offX);
+    } else {  // each element in place or transposed (a branch on the stage's
+              // position here makes the unrolled loop spill)
+      const char *Xc = reinterpret_cast<const char *>(X);
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        double d[2];
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+          const uint32_t on = (uint32_t)(((int64_t)(r0 + i) * ncols + n0 + q) * 8) + offX;
+          const uint32_t ot = ut(r0, i, q) + offT;
+          d[q] = *reinterpret_cast<const double *>(Xc + (dq[q] <= -(r0 + i) ? on : ot));
+        }
+        rx[u][i].x = d[0];
+        rx[u][i].y = d[1];
+      }
+    }
+    // (a chunk of zeros may have no exponent, INT_MIN: any value below the
+    // FP64 range keeps its limbs and scales zero)
+    et[u] = max(texp[(r0 / 128) * tcols + n0 / 32], -1100);
   };
   const double radix = ldexp(1.0, bits);
   // limbs of rows (2 rp, 2 rp + 1) of column 2 cp + q, one packed half2 per
@@ -401,6 +446,7 @@ struct G2Out {
   int M;        // rows x1 computed (N1/2 in the half-space BA step, else N1)
   const double *exptab;
   int part_off;  // quarter mode: partials at part[part_off + blockIdx.x]
+  int mirror;    // quarter mode: also write each canonical element's mirror
 };
 
 // S[x1][x2] = sum_kappa A16[x1][kappa] V[kappa][x2] + sum_kappa' W[x1][kappa'] B16[x2][kappa']
@@ -677,7 +723,7 @@ __global__ void __launch_bounds__(256, QTR ? 2 : 3) k_split_g2(int N2, int KLp, 
       }
       s0 += wt * w;
       s2 = fma(wt * p, D, s2);
-      tl[(wave * 32 + mfma32_row(lane, r)) * 33 + lr] = canon && !self ? w : -1.0;
+      if (go.mirror) tl[(wave * 32 + mfma32_row(lane, r)) * 33 + lr] = canon && !self ? w : -1.0;
     } else {
       m1 = fmax(m1, D);
       if constexpr (EPI == EPI_BA) {
@@ -702,14 +748,30 @@ __global__ void __launch_bounds__(256, QTR ? 2 : 3) k_split_g2(int N2, int KLp, 
   // ---- BA step: exponent of the new P tile for the next output law (whose
   // contraction splits P into limbs as it reads it)
   if constexpr (EPI == EPI_BA) {
-    wmax = block_max(wmax, red + 16);
+    double wmax_w = wmax;  // this wave's 32 rows
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) wmax_w = fmax(wmax_w, __shfl_xor(wmax_w, o, 64));
+    const double wmax_b = block_max(wmax, red + 16);
+    wmax = wmax_b;
     if constexpr (QTR) {
-      if (tid == 0) atomicMax(&go.texp[(m0 / 128) * go.tcols + n0 / 32], tile_exponent(wmax, go.bits_p));
+      // (a tile or wave whose canonical p are all zero enters no exponent)
+      if (tid == 0 && wmax_b > 0.0) atomicMax(&go.texp[(m0 / 128) * go.tcols + n0 / 32], tile_exponent(wmax_b, go.bits_p));
+      if (!go.mirror && lane == 0 && wmax_w > 0.0) {
+        // the output law reads this wave's 32 rows of the tile transposed, as
+        // part of the strip of columns m0 + 32 wave (left of the middle) or
+        // N - 32 - m0 - 32 wave (right), at the 128-row chunk holding the
+        // tile's columns: they enter that chunk's exponent
+        const bool lt = n0 < halfq;
+        const int c0 = lt ? (n0 & ~127) : ((Nq - 32 - n0) & ~127);
+        const int s0 = lt ? m0 + 32 * wave : Nq - 32 - m0 - 32 * wave;
+        if (c0 + 127 > (lt ? s0 : Nq - 32 - s0))
+          atomicMax(&go.texp[(c0 / 128) * go.tcols + s0 / 32], tile_exponent(wmax_w, go.bits_p));
+      }
     } else {
       if (tid == 0) go.texp[(m0 / 128) * go.tcols + n0 / 32] = tile_exponent(wmax, go.bits_p);
     }
   }
-  if constexpr (QTR) {
+  if (QTR && go.mirror) {
     // mirror image of the tile (canonical, non-self elements) from LDS: wave w
     // writes mirror rows 8w..8w+7 (tile columns), each in two passes of 64
     // consecutive entries (tile rows), so a store instruction covers 512
@@ -758,6 +820,36 @@ __global__ void k_fill_i32(int32_t *__restrict__ a, int len, int32_t v) {
   if (i < len) a[i] = v;
 }
 
+// Completes the half space (rows r < N/2 of N columns) of a quarter-layout P
+// after BA steps that updated only its canonical elements: every other element
+// takes its partner's value, P[r][c] = P[c][r] left of the middle column and
+// P[N-1-c][N-1-r] right of it (the partner is canonical in both cases).  One
+// 64 x 64 tile per block; the partner tile comes through LDS so that reads and
+// writes both run along rows.  Entries of the partner tile that are not
+// partners of this tile's elements are read but unused.
+__global__ void __launch_bounds__(256) k_fill_mirror(double *__restrict__ P, int N) {
+  __shared__ double t[64][65];
+  const int r0 = blockIdx.y * 64, c0 = blockIdx.x * 64;
+  const bool left = c0 < N / 2;
+  if (left ? r0 + 63 <= c0 : r0 + c0 + 126 <= N - 1) return;  // canonical throughout
+  const int pr0 = left ? c0 : N - 64 - c0, pc0 = left ? r0 : N - 64 - r0;
+  for (int i = threadIdx.x; i < 64 * 64; i += 256) {
+    const int rr = i >> 6, cc = i & 63;
+    t[rr][cc] = P[(int64_t)(pr0 + rr) * N + pc0 + cc];
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < 64 * 64; i += 256) {
+    const int rr = i >> 6, cc = i & 63, r = r0 + rr, c = c0 + cc;
+    if (left ? r > c : r + c > N - 1) P[(int64_t)r * N + c] = left ? t[cc][rr] : t[63 - cc][63 - rr];
+  }
+}
+
+int launch_fill_mirror(double *P, int N, hipStream_t s) {
+  if (N < 128 || N % 128) return BDC_ERR_ARG;
+  hipLaunchKernelGGL(k_fill_mirror, dim3(N / 64, N / 128), dim3(256), 0, s, P, N);
+  return hipGetLastError() == hipSuccess ? BDC_OK : BDC_ERR_HIP;
+}
+
 // ----------------------------------------------------------------- launchers
 #define BDC_SPLIT_L(L, call)     \
   switch (L) {                   \
@@ -797,12 +889,17 @@ int launch_split_vw(const double *V, int KL, int KLp, int N2, const double *W, i
 }
 
 int launch_split_g1(int KLp, int Kdim, int lda, int ncols, const _Float16 *aT, const double *X, const int32_t *texp,
-                    int limbs, int bits, int ks, double *Tpart, hipStream_t s) {
+                    int limbs, int bits, int ks, double *Tpart, int tri, hipStream_t s) {
   if (ks < 1 || Kdim % (ks * 128) || ncols % 64 || lda < Kdim || lda % 8) return BDC_ERR_ARG;
+  if (tri && (ncols != 2 * Kdim || ncols < 256)) return BDC_ERR_ARG;
   const int blocks = ks * (ncols / 32) * ((KLp + 255) / 256);
-#define LG(LL)                                                                                                  \
-  hipLaunchKernelGGL(k_split_g1<LL>, dim3(blocks), dim3(512), 0, s, KLp, Kdim, lda, ncols, aT, X, texp, bits, ks, \
-                     Tpart)
+#define LG(LL)                                                                                                    \
+  if (tri)                                                                                                        \
+    hipLaunchKernelGGL((k_split_g1<LL, true>), dim3(blocks), dim3(512), 0, s, KLp, Kdim, lda, ncols, aT, X, texp, \
+                       bits, ks, Tpart);                                                                          \
+  else                                                                                                            \
+    hipLaunchKernelGGL((k_split_g1<LL, false>), dim3(blocks), dim3(512), 0, s, KLp, Kdim, lda, ncols, aT, X, texp, \
+                       bits, ks, Tpart)
   // five limbs for the narrow limbs of half length >= 15 (7 bits: 35 bits of p per tile)
   if (limbs == 5) {
     LG(5);
@@ -830,6 +927,7 @@ int launch_split_g2(const SplitG2Args &a, const EpiArgs &ea, hipStream_t s, int 
   go.patches = (TX % 8 == 0 && TY % 8 == 0 && ((TX / 8) * (TY / 8)) % 8 == 0) ? 1 : 0;
   go.M = a.M;
   go.exptab = a.exptab;
+  go.mirror = a.mirror;
   if (a.epi == EPI_BA && !a.texp_next) return BDC_ERR_ARG;
   const int blocks = TX * TY;
   if (nb && !a.quarter) *nb = blocks;
