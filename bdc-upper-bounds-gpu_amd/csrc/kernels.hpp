@@ -196,8 +196,9 @@ int launch_split_vw(const double *V, int KL, int KLp, int N2, const double *W, i
 // triangle of canonical elements (the others may be stale)
 int launch_split_g1(int KLp, int Kdim, int lda, int ncols, const _Float16 *aT, const double *X, const int32_t *texp,
                     int limbs, int bits, int ks, double *Tpart, int tri, hipStream_t s);
-// residual corrections of G1 from the rows x1 < rows
-int launch_resid_g1(const int32_t *res, int nres, const double *X, int rows, int N2, double *T, hipStream_t s);
+// residual corrections of G1 from the rows x1 < rows (tri: X read from the triangle, as launch_split_g1)
+int launch_resid_g1(const int32_t *res, int nres, const double *X, int rows, int N2, int tri, double *T,
+                    hipStream_t s);
 struct SplitG2Args {
   int N1, N2, M, KL, KLp, KR, KRp;  // M: rows x1 computed (< N1 in the half-space step)
   const _Float16 *A16, *vT, *wl, *B16;

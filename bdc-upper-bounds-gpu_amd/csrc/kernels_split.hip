@@ -415,14 +415,20 @@ __global__ void __launch_bounds__(512) k_split_g1(int KLp, int Kdim, int lda, in
 
 // T[kappa][c] += r * X(x1, c) for the residuals of the rounded counts in the
 // rows x1 < rows.  One thread owns a column and walks every residual, so
-// updates never race.
+// updates never race.  tri: X is a quarter-layout P whose non-canonical
+// elements may be stale; X(x1, c) is read from the canonical partner.
 __global__ void k_resid_g1(const int32_t *__restrict__ res, int nres, const double *__restrict__ X, int rows, int N2,
-                           double *__restrict__ T) {
+                           int tri, double *__restrict__ T) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= N2) return;
+  const bool left = c < N2 / 2;
   for (int e = 0; e < nres; ++e) {
     const int x1 = res[3 * e], kap = res[3 * e + 1], r = res[3 * e + 2];
-    if (x1 < rows) T[(int64_t)kap * N2 + c] += (double)r * X[(int64_t)x1 * N2 + c];
+    if (x1 >= rows) continue;
+    int64_t at = (int64_t)x1 * N2 + c;
+    if (tri && !(left ? x1 <= c : x1 + c <= N2 - 1))
+      at = left ? (int64_t)c * N2 + x1 : (int64_t)(N2 - 1 - c) * N2 + (N2 - 1 - x1);
+    T[(int64_t)kap * N2 + c] += (double)r * X[at];
   }
 }
 
@@ -910,9 +916,11 @@ int launch_split_g1(int KLp, int Kdim, int lda, int ncols, const _Float16 *aT, c
   return hipGetLastError() == hipSuccess ? BDC_OK : BDC_ERR_HIP;
 }
 
-int launch_resid_g1(const int32_t *res, int nres, const double *X, int rows, int N2, double *T, hipStream_t s) {
+int launch_resid_g1(const int32_t *res, int nres, const double *X, int rows, int N2, int tri, double *T,
+                    hipStream_t s) {
   if (nres == 0) return BDC_OK;
-  hipLaunchKernelGGL(k_resid_g1, dim3((unsigned)((N2 + 255) / 256)), dim3(256), 0, s, res, nres, X, rows, N2, T);
+  if (tri && N2 != 2 * rows) return BDC_ERR_ARG;
+  hipLaunchKernelGGL(k_resid_g1, dim3((unsigned)((N2 + 255) / 256)), dim3(256), 0, s, res, nres, X, rows, N2, tri, T);
   return hipGetLastError() == hipSuccess ? BDC_OK : BDC_ERR_HIP;
 }
 

@@ -193,6 +193,27 @@ def test_bdc28_14_sparse_support(bdc, oracle, e2814):
     assert abs(cert["sum_p"] - 1.0) < 1e-12
 
 
+def test_bdc28_14_triangle_reads_match_mirrored(bdc, e2814, monkeypatch):
+    """At the metric size (half length 14, where counts above 2048 leave
+    residuals in the output law's row x1 = 0): split steps that update only the
+    canonical elements, the output law reading the triangle, agree bitwise
+    with steps that also write every mirror (BDC_QUARTER_TRI=0), from a
+    Dirichlet(1) p and across two calls."""
+    e = e2814
+    out = {}
+    for mode in ("1", "0"):
+        say(f"BDC_(28,14): split steps, BDC_QUARTER_TRI={mode}")
+        monkeypatch.setenv("BDC_QUARTER_TRI", mode)
+        e.set_p_dirichlet(1.0, 3)
+        st = np.vstack([e.iterate(4, bdc.PREC_SPLIT16), e.iterate(2, bdc.PREC_SPLIT16)])
+        out[mode] = (st, e.certify(), e.sample(bdc.SAMPLE_P, random_inputs(28, 64, 9)))
+    for a, b in zip(out["1"], out["0"]):
+        if isinstance(a, dict):
+            assert a == b
+        else:
+            assert np.array_equal(a, b)
+
+
 def test_bdc28_14_from_cache_files(bdc, e2814, tmp_path):
     """Caches loaded from cache_{n'}_{k'} files (load_gpu_cache, BAA.cu:27-93;
     written by bdc_write_caches, byte-identical to the reference generator)
