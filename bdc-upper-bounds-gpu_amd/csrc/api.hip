@@ -581,7 +581,7 @@ int bdc_destroy(bdc_ctx *c) {
                          &c->Hc, &c->T, &c->U, &c->qpart, &c->qv, &c->g2a, &c->g2b, &c->splitk, &c->part, &c->scal, &c->exptab,
                          &c->hist, &c->ptexp, &c->vlimb, &c->wlimb, &c->theta, &c->adam_m,
                          &c->adam_v, &c->grad, &c->dvec, &c->ck_p, &c->ck_s, &c->ck_t, &c->lvl, &c->g64part,
-                         &c->T2, &c->V2, &c->qtiles[0], &c->qtiles[1]})
+                         &c->T2, &c->V2, &c->qtiles[0], &c->qtiles[1], &c->qtiles[2]})
     b->release();
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -1045,6 +1045,8 @@ static int ba_rows(bdc_ctx *c, int prec) {
   return c->N1 / 2;
 }
 
+static bool quarter_step(const bdc_ctx *c, int rows);
+
 // after quarter steps that updated only the canonical elements: the rest of
 // the half space from their partners (before the complement image is written)
 static int complete_mirrors(bdc_ctx *c) {
@@ -1061,7 +1063,7 @@ static int ba_step(bdc_ctx *c, int prec, int idx, int rows, double *hist) {
   if (prec == BDC_PREC_FP64) {
     BDC_TRY(cross_operands(c, L, true, rows));
   } else {
-    BDC_TRY(split_cross_operands(c, L, true, rows));
+    BDC_TRY(split_cross_operands(c, L, true, rows, quarter_step(c, rows)));
   }
   int nb = 0;
   BDC_TRY(cross_term(c, prec, EPI_BA, nullptr, &nb, rows));
@@ -1774,7 +1776,7 @@ int split_prepare(bdc_ctx *c) {
     // right of it), without and with residuals, in 8 x 8 patches dealt to the
     // eight XCDs in turn (block b runs on XCD b mod 8)
     const int N = (int)N2, TX = N / 32, TY = N / 2 / 128;
-    std::vector<std::vector<int32_t>> lists(2);
+    std::vector<std::vector<int32_t>> lists(3);
     for (int h = 0; h < 2; ++h) {
       std::vector<std::vector<int32_t>> queues(8);
       int patch = 0;
@@ -1804,6 +1806,14 @@ int split_prepare(bdc_ctx *c) {
         if (!more) break;
       }
       c->nqtiles[h] = (int)lists[h].size();
+    }
+    // the launch list: residual tiles first (blocks below nqtiles[1] add the
+    // residual terms), then the others.  The offset moves a patch to another
+    // XCD but keeps its tiles together (consecutive tiles of a patch are eight
+    // list positions apart while every XCD queue still has tiles)
+    lists[2] = lists[1];
+    lists[2].insert(lists[2].end(), lists[0].begin(), lists[0].end());
+    for (int h = 0; h < 3; ++h) {
       if (lists[h].empty()) lists[h].push_back(0);
       BDC_TRY(c->qtiles[h].alloc(lists[h].size() * 4));
       BDC_HIP(hipMemcpyAsync(c->qtiles[h].p, lists[h].data(), lists[h].size() * 4, hipMemcpyHostToDevice, c->stream));
@@ -1880,16 +1890,21 @@ int split_output_law_g1(bdc_ctx *c, const double *X, int rows) {
   return BDC_OK;
 }
 
-int split_cross_operands(bdc_ctx *c, const double *L, bool mirror, int rows) {
+int split_cross_operands(bdc_ctx *c, const double *L, bool mirror, int rows, bool reset_texp) {
   if (c->chain) return chain_cross(c, L);
   // V, W in FP64 (stage contractions); the deletion chain also leaves the block
   // maxima of |V| for the limb exponent when W is a permutation of V
   int namax = 0;
   BDC_TRY(cross_operands(c, L, mirror, rows, c->part.as<double>(), c->part.bytes / 8, &namax));
   const Split &sp = c->split;
+  // the quarter step's tile exponents are reset on the way (its cross term
+  // accumulates them with atomic maxima)
+  const int nt = reset_texp ? (rows / 128) * (c->N2 / 32) : 0;
   BDC_LAUNCH(launch_split_vw(vbar(c), c->KL, c->KLp, c->N2, c->g2a.as<double>(), rows, c->KR, c->KRp,
                              sp.g2_limbs, sp.g2_bits, c->vlimb.as<_Float16>(), c->wlimb.as<_Float16>(),
-                             c->part.as<double>(), namax, c->scal.as<double>(), c->stream));
+                             c->part.as<double>(), namax, c->scal.as<double>(), c->ptexp.as<int32_t>(), nt,
+                             c->stream));
+  c->texp_reset = nt > 0;
   return BDC_OK;
 }
 
@@ -1930,9 +1945,12 @@ int split_cross_term(bdc_ctx *c, int epi, const EpiArgs &ea, int *nb, int rows) 
   // updating the canonical elements only (the output law reads the triangle)
   a.quarter = fuse && quarter_step(c, rows) ? 1 : 0;
   a.mirror = tri_reads() ? 0 : 1;
+  a.texp_reset = a.quarter && c->texp_reset ? 1 : 0;
+  c->texp_reset = false;
   if (a.quarter && !a.mirror) c->p_mirror_stale = true;
   a.q_tiles = c->qtiles[0].as<int32_t>();
   a.qres_tiles = c->qtiles[1].as<int32_t>();
+  a.qall_tiles = c->qtiles[2].as<int32_t>();
   a.nq_tiles = c->nqtiles[0];
   a.nqres_tiles = c->nqtiles[1];
   BDC_LAUNCH(launch_split_g2(a, ea, c->stream, nb));

@@ -120,7 +120,8 @@ struct bdc_ctx {
   bdc::DevBuf P;   // p-tilde (unnormalised), FP64; p = pscale * P
   bdc::DevBuf Hc;  // H(x)/C - ln C, FP64
   bool have_p = false;
-  bool p_mirror_stale = false;  // quarter steps left the non-canonical half-space elements stale
+  bool p_mirror_stale = false;
+  bool texp_reset = false;  // the last split of V reset ptexp for a quarter cross term  // quarter steps left the non-canonical half-space elements stale
   double last_mx_bits = NAN;  // mx of the step that produced the current p (NaN if p was set)
   bool have_h = false;
   bool have_d = false;  // dvec holds D(x||q) of the last bdc_certify
@@ -144,7 +145,7 @@ struct bdc_ctx {
   int cr = 0;
   bool g2b_stored = false;
   bdc::DevBuf T2, V2;
-  bdc::DevBuf qtiles[2];     // quarter-mode tile lists of the BA step: without / with residuals
+  bdc::DevBuf qtiles[3];     // quarter-mode tile lists of the BA step: without / with residuals / both (launch order)
   int nqtiles[2] = {0, 0};
   size_t splitk_elems = 0;
   bdc::DevBuf part;   // per-block partials of the fused epilogue
@@ -197,6 +198,7 @@ struct EpiArgs;
 // split-FP16 path (split.hip)
 int split_prepare(bdc_ctx *c);
 int split_output_law_g1(bdc_ctx *c, const double *X, int rows);
-int split_cross_operands(bdc_ctx *c, const double *L, bool mirror, int rows);
+// reset_texp: the quarter BA step's cross term follows (its P tile exponents are reset on the way)
+int split_cross_operands(bdc_ctx *c, const double *L, bool mirror, int rows, bool reset_texp = false);
 int split_cross_term(bdc_ctx *c, int epi, const EpiArgs &ea, int *nblocks, int rows);
 }  // namespace bdc

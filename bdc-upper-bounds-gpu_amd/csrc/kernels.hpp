@@ -187,9 +187,12 @@ int launch_entropy_dfs(int n, int k, uint64_t norb, const uint32_t *orbits, cons
 // exponents of the 128 x 32 tiles of a P-like array (rows x1 < M, N2 columns), row-major by tile
 int launch_tile_exp(const double *P, int M, int N2, int bits, int32_t *texp, hipStream_t s);
 // np_pre > 0: part[0 .. np_pre) already holds block maxima of |V| and |W| (W a
-// permutation of entries of V), so the max pass over V and W is skipped
+// permutation of entries of V), so the max pass over V and W is skipped.
+// fill (may be null): fill_n int32 set to INT_MIN on the way (the P tile
+// exponents before a quarter step)
 int launch_split_vw(const double *V, int KL, int KLp, int N2, const double *W, int M, int KR, int KRp, int limbs,
-                    int bits, _Float16 *vT, _Float16 *wl, double *part, int np_pre, double *scal, hipStream_t s);
+                    int bits, _Float16 *vT, _Float16 *wl, double *part, int np_pre, double *scal, int32_t *fill,
+                    int fill_n, hipStream_t s);
 // ks partial slabs (ks * KLp * ncols FP64) of Tbar = aT X over the first Kdim rows of the FP64 array X
 // (row stride ncols; split into limbs on the fly with the tile exponents texp), aT row stride lda
 // tri: X = P of the quarter-layout BA step (Kdim = ncols / 2), read from the
@@ -216,7 +219,9 @@ struct SplitG2Args {
   int quarter;
   const int32_t *q_tiles, *qres_tiles;
   int nq_tiles, nqres_tiles;
+  const int32_t *qall_tiles;  // the residual tiles, then the others
   int mirror;  // quarter mode: also write the mirror of every canonical element
+  int texp_reset;  // quarter mode: texp_next already reset to INT_MIN (by launch_split_vw)
 };
 int launch_split_g2(const SplitG2Args &a, const EpiArgs &ea, hipStream_t s, int *nb);
 // the non-canonical elements of a quarter-layout half space (N/2 x N) from their partners

@@ -93,21 +93,29 @@ __global__ void __launch_bounds__(256) k_absmax_vw(const double *__restrict__ V,
   m = block_max(m, red);
   if (threadIdx.x == 0) part[blockIdx.x] = m;
 }
-__global__ void __launch_bounds__(256) k_vw_exponent(const double *__restrict__ part, int np, int bits,
-                                                     double *__restrict__ scal) {
+// V (KLp x N2, row kappa) -> vT[l][x2][kappa] (N2 x KLp), 64x64 tiles through
+// LDS.  The limb scale is the exponent of the maximum over the np block maxima
+// of |V| and |W| in part (every block reduces them alike; block 0 records it in
+// the scalars for the W split and the cross term).  fill: fill_n entries of
+// fill set to INT_MIN by the grid (the P tile exponents reset before a
+// quarter step, which accumulates them with atomic maxima).
+template <int L>
+__global__ void __launch_bounds__(256) k_split_v(const double *__restrict__ V, int KL, int KLp, int N2, int bits,
+                                                 const double *__restrict__ part, int np, double *__restrict__ scal,
+                                                 _Float16 *__restrict__ vT, int32_t *__restrict__ fill, int fill_n) {
+  __shared__ double t[64][65];
   __shared__ double red[16];
+  const int k0 = blockIdx.y * 64, x0 = blockIdx.x * 64;
+  {
+    const int nthreads = gridDim.x * gridDim.y * blockDim.x;
+    for (int i = (blockIdx.y * gridDim.x + blockIdx.x) * blockDim.x + threadIdx.x; i < fill_n; i += nthreads)
+      fill[i] = INT_MIN;
+  }
   double m = 0.0;
   for (int i = threadIdx.x; i < np; i += blockDim.x) m = fmax(m, part[i]);
   m = block_max(m, red);
-  if (threadIdx.x == 0) scal[kScalVScale] = (double)tile_exponent(m, bits);
-}
-// V (KLp x N2, row kappa) -> vT[l][x2][kappa] (N2 x KLp), 64x64 tiles through LDS
-template <int L>
-__global__ void __launch_bounds__(256) k_split_v(const double *__restrict__ V, int KL, int KLp, int N2, int bits,
-                                                 const double *__restrict__ scal, _Float16 *__restrict__ vT) {
-  __shared__ double t[64][65];
-  const int k0 = blockIdx.y * 64, x0 = blockIdx.x * 64;
-  const int e = (int)scal[kScalVScale];
+  const int e = tile_exponent(m, bits);
+  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) scal[kScalVScale] = (double)e;
   for (int i = threadIdx.x; i < 64 * 64; i += 256) {
     const int kk = i >> 6, xx = i & 63;
     t[kk][xx] = (k0 + kk < KL) ? ldexp(V[(int64_t)(k0 + kk) * N2 + x0 + xx], -e) : 0.0;
@@ -453,6 +461,7 @@ struct G2Out {
   const double *exptab;
   int part_off;  // quarter mode: partials at part[part_off + blockIdx.x]
   int mirror;    // quarter mode: also write each canonical element's mirror
+  int nres_first;  // RES == 2: the first nres_first blocks' tiles have residuals
 };
 
 // S[x1][x2] = sum_kappa A16[x1][kappa] V[kappa][x2] + sum_kappa' W[x1][kappa'] B16[x2][kappa']
@@ -493,7 +502,9 @@ struct G2Rows {
 // atomic maxima (the array is reset to INT_MIN before the launch).  The
 // quarter variant runs at two waves per SIMD: its epilogue needs ~180 VGPRs,
 // and at three waves the four-limb kernel spills in the main loop.
-template <int L, int EPI, bool RES, bool QTR>
+// RES: 0 no residual tiles, 1 every tile has residuals, 2 the first
+// go.nres_first blocks' tiles do.
+template <int L, int EPI, int RES, bool QTR>
 __global__ void __launch_bounds__(256, QTR ? 2 : 3) k_split_g2(int N2, int KLp, int KRp, const _Float16 *__restrict__ A16,
                                                     const _Float16 *__restrict__ vT, const _Float16 *__restrict__ wl,
                                                     const _Float16 *__restrict__ B16, int bits, EpiArgs ea,
@@ -509,7 +520,7 @@ __global__ void __launch_bounds__(256, QTR ? 2 : 3) k_split_g2(int N2, int KLp, 
   const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int TX = N2 / BN;
   int ty, tx;
-  if constexpr (RES || QTR) {  // tiles from a list (residual tiles, or the quarter's tiles)
+  if constexpr (RES != 0 || QTR) {  // tiles from a list (residual tiles, or the quarter's tiles)
     ty = tiles[blockIdx.x] / TX;
     tx = tiles[blockIdx.x] % TX;
   } else if (go.patches) {
@@ -693,16 +704,25 @@ __global__ void __launch_bounds__(256, QTR ? 2 : 3) k_split_g2(int N2, int KLp, 
     double S = 0.0;
 #pragma unroll
     for (int l = 0; l < L; ++l) S = fma((double)acc[l][r], sc[l], S);
-    const int xl1 = wave * 32 + mfma32_row(lane, r);
-    if constexpr (RES) {
-      const int x1 = m0 + xl1, x2 = n0 + lr;
+    dv[r] = S;
+  }
+  // residual terms of rounded counts (every tile of a residual launch; in the
+  // quarter launch, the residual tiles listed first), added to S in the same
+  // order either way
+  if (RES == 1 || (RES == 2 && (int)blockIdx.x < go.nres_first)) {
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int x1 = m0 + wave * 32 + mfma32_row(lane, r), x2 = n0 + lr;
+      double S = dv[r];
       for (int e = rs.rowptrA[x1]; e < rs.rowptrA[x1 + 1]; ++e)
         S += (double)rs.entA[2 * e + 1] * rs.V64[(int64_t)rs.entA[2 * e] * N2 + x2];
       for (int e = rs.rowptrB[x2]; e < rs.rowptrB[x2 + 1]; ++e)
         S += rs.W64[(int64_t)x1 * rs.KRp + rs.entB[2 * e]] * (double)rs.entB[2 * e + 1];
+      dv[r] = S;
     }
-    dv[r] = fma(-S, ea.inv_c, hv[r] - mu);
   }
+#pragma unroll
+  for (int r = 0; r < 16; ++r) dv[r] = fma(-dv[r], ea.inv_c, hv[r] - mu);
   // the accumulators and H are dead now: room for the P tile
   p_loads();
   const int Nq = N2, halfq = N2 / 2;  // quarter mode: N1 = N2
@@ -872,16 +892,17 @@ int launch_tile_exp(const double *P, int M, int N2, int bits, int32_t *texp, hip
 }
 
 int launch_split_vw(const double *V, int KL, int KLp, int N2, const double *W, int M, int KR, int KRp, int limbs,
-                    int bits, _Float16 *vT, _Float16 *wl, double *part, int np_pre, double *scal, hipStream_t s) {
+                    int bits, _Float16 *vT, _Float16 *wl, double *part, int np_pre, double *scal, int32_t *fill,
+                    int fill_n, hipStream_t s) {
   int np = np_pre;
   if (np <= 0) {
     np = 512;
     hipLaunchKernelGGL(k_absmax_vw, dim3(np), dim3(256), 0, s, V, (int64_t)KL * N2, W, M, KR, KRp, part);
   }
-  hipLaunchKernelGGL(k_vw_exponent, dim3(1), dim3(256), 0, s, part, np, bits, scal);
   if (N2 % 64 || KLp % 64) return BDC_ERR_ARG;
   dim3 gv(N2 / 64, KLp / 64);
-#define LV(LL) hipLaunchKernelGGL(k_split_v<LL>, gv, dim3(256), 0, s, V, KL, KLp, N2, bits, scal, vT)
+#define LV(LL) \
+  hipLaunchKernelGGL(k_split_v<LL>, gv, dim3(256), 0, s, V, KL, KLp, N2, bits, part, np, scal, vT, fill, fill_n)
   BDC_SPLIT_L(limbs, LV)
 #undef LV
   if (KR > 0) {
@@ -942,29 +963,25 @@ int launch_split_g2(const SplitG2Args &a, const EpiArgs &ea, hipStream_t s, int 
   if (a.quarter) {  // the BA step over the quarter's tiles (lists), partials by list position
     if (a.epi != EPI_BA || a.N1 != a.N2 || a.M != a.N2 / 2) return BDC_ERR_ARG;
     const int nt = (a.M / 128) * (a.N2 / 32);
-    hipLaunchKernelGGL(k_fill_i32, dim3((nt + 255) / 256), dim3(256), 0, s, a.texp_next, nt, (int32_t)INT32_MIN);
+    if (!a.texp_reset)  // (else the V limb split has reset the exponents)
+      hipLaunchKernelGGL(k_fill_i32, dim3((nt + 255) / 256), dim3(256), 0, s, a.texp_next, nt, (int32_t)INT32_MIN);
     if (nb) *nb = a.nq_tiles + a.nqres_tiles;
-#define LQ(LL)                                                                                                    \
-  do {                                                                                                            \
-    go.part_off = 0;                                                                                              \
-    if (a.nq_tiles)                                                                                               \
-      hipLaunchKernelGGL((k_split_g2<LL, EPI_BA, false, true>), dim3(a.nq_tiles), dim3(256), 0, s, a.N2, a.KLp,   \
-                         a.KRp, a.A16, a.vT, a.wl, a.B16, a.bits, ea, rs, go, a.q_tiles);                          \
-    go.part_off = a.nq_tiles;                                                                                     \
-    if (a.nqres_tiles)                                                                                            \
-      hipLaunchKernelGGL((k_split_g2<LL, EPI_BA, true, true>), dim3(a.nqres_tiles), dim3(256), 0, s, a.N2, a.KLp, \
-                         a.KRp, a.A16, a.vT, a.wl, a.B16, a.bits, ea, rs, go, a.qres_tiles);                       \
-  } while (0)
+    // one launch over the residual tiles, then the rest (a.qall_tiles)
+    go.part_off = 0;
+    go.nres_first = a.nqres_tiles;
+#define LQ(LL)                                                                                                      \
+  hipLaunchKernelGGL((k_split_g2<LL, EPI_BA, 2, true>), dim3(a.nq_tiles + a.nqres_tiles), dim3(256), 0, s, a.N2, \
+                     a.KLp, a.KRp, a.A16, a.vT, a.wl, a.B16, a.bits, ea, rs, go, a.qall_tiles)
     BDC_SPLIT_L(a.limbs, LQ)
 #undef LQ
     return hipGetLastError() == hipSuccess ? BDC_OK : BDC_ERR_HIP;
   }
 #define LG2(LL, EE)                                                                                               \
   do {                                                                                                            \
-    hipLaunchKernelGGL((k_split_g2<LL, EE, false, false>), dim3(blocks), dim3(256), 0, s, a.N2, a.KLp, a.KRp,     \
+    hipLaunchKernelGGL((k_split_g2<LL, EE, 0, false>), dim3(blocks), dim3(256), 0, s, a.N2, a.KLp, a.KRp,     \
                        a.A16, a.vT, a.wl, a.B16, a.bits, ea, rs, go, nullptr);                                    \
     if (a.nres_tiles)                                                                                             \
-      hipLaunchKernelGGL((k_split_g2<LL, EE, true, false>), dim3(a.nres_tiles), dim3(256), 0, s, a.N2, a.KLp,     \
+      hipLaunchKernelGGL((k_split_g2<LL, EE, 1, false>), dim3(a.nres_tiles), dim3(256), 0, s, a.N2, a.KLp,     \
                          a.KRp, a.A16, a.vT, a.wl, a.B16, a.bits, ea, rs, go, a.res_tiles);                        \
   } while (0)
 #define LG2_E(EE)                \
