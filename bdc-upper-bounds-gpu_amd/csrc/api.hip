@@ -1046,6 +1046,7 @@ static int ba_rows(bdc_ctx *c, int prec) {
 }
 
 static bool quarter_step(const bdc_ctx *c, int rows);
+static bool tri_reads();
 
 // after quarter steps that updated only the canonical elements: the rest of
 // the half space from their partners (before the complement image is written)
@@ -1603,12 +1604,15 @@ int bdc_work_model(bdc_ctx *c, int prec, double *g1, double *g2, double *bytes) 
   // G2 epilogue: p read and written, H read (24 B/x) at both precisions (the
   // split16 output law forms p's limbs itself from the FP64 p)
   if (bytes) *bytes = nx * 24.0;
-  if (prec == BDC_PREC_SPLIT16 && c->cr && rows == c->N1 / 2 && c->qtiles[0].p) {
+  if (prec == BDC_PREC_SPLIT16 && quarter_step(c, rows)) {
     // quarter mode: the listed 128 x 32 tiles are contracted and read (p and
-    // H), and the whole half space is written (each element or its mirror)
+    // H); written are the canonical elements (as many as orbits: M (M + 1) for
+    // the M = N1/2 rows) or, with BDC_QUARTER_TRI=0, the whole half space (each
+    // element or its mirror)
     const double tiled = 4096.0 * (c->nqtiles[0] + c->nqtiles[1]);
+    const double M = (double)rows;
     if (g2) *g2 = 2.0 * (c->KL + c->KR) * tiled;
-    if (bytes) *bytes = 16.0 * tiled + 8.0 * nx;
+    if (bytes) *bytes = 16.0 * tiled + 8.0 * (tri_reads() ? M * (M + 1.0) : nx);
   }
   return BDC_OK;
 }

@@ -493,8 +493,8 @@ def main():
     # does (main_BAA.cu:46-50, BAA.cu:230-241).  The kernel's own layout bytes
     # (from the work model) are reported beside them: 24 B per input of the half
     # space, or in the split step's quarter mode 16 B per input of the
-    # contracted tiles (about a quarter of the space) plus 8 B written per input
-    # of the half space.
+    # contracted tiles (about a quarter of the space) plus 8 B written per
+    # canonical element (one per orbit).
     orbit_bytes = 24.0 * e.num_orbits
     g2s = r["g2_ms"] * 1e-3
     traffic = measured_traffic(n, k, "g2_split" if head == "split" else "g2_fp64")
@@ -531,7 +531,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"BDC_({n},{k}) Blahut-Arimoto step from the uniform p: 2^{n} inputs, "
                                f"{e.num_orbits} complement/reversal orbits, updated on the half space x1 < 2^{n // 2 - 1}"
-                               + (" (split step: the reversal-canonical quarter, mirrored on write)"
+                               + (" (split step: the reversal-canonical quarter, canonical elements updated)"
                                   if quarter else ""),
                    "n": n, "k": k, "iterations": args.steps, "precision": head},
         "roofline": roof,
