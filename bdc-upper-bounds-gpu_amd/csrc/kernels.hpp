@@ -196,9 +196,10 @@ int launch_split_vw(const double *V, int KL, int KLp, int N2, const double *W, i
 // ks partial slabs (ks * KLp * ncols FP64) of Tbar = aT X over the first Kdim rows of the FP64 array X
 // (row stride ncols; split into limbs on the fly with the tile exponents texp), aT row stride lda
 // tri: X = P of the quarter-layout BA step (Kdim = ncols / 2), read from the
-// triangle of canonical elements (the others may be stale)
+// triangle of canonical elements (the others may be stale).  merge (limbs >= 4):
+// the last two limbs share one FP32 accumulator
 int launch_split_g1(int KLp, int Kdim, int lda, int ncols, const _Float16 *aT, const double *X, const int32_t *texp,
-                    int limbs, int bits, int ks, double *Tpart, int tri, hipStream_t s);
+                    int limbs, int bits, int ks, double *Tpart, int tri, int merge, hipStream_t s);
 // residual corrections of G1 from the rows x1 < rows (tri: X read from the triangle, as launch_split_g1)
 int launch_resid_g1(const int32_t *res, int nres, const double *X, int rows, int N2, int tri, double *T,
                     hipStream_t s);

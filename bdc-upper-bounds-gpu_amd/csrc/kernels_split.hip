@@ -9,8 +9,10 @@
 // count x limb is an integer below 2^22, and the limb width b and the K extent
 // accumulated in FP32 before promotion to FP64 are chosen so that every
 // partial sum stays below 2^24 in magnitude — the FP32 accumulator then adds
-// integers exactly, whatever order the matrix core uses.  The only error left
-// is the truncation of the FP64 operand to L*b bits.
+// integers exactly, whatever order the matrix core uses.  The error left is the
+// truncation of the FP64 operand to L*b bits, and in the output law with four
+// or more limbs the FP32 rounding of the accumulator the last two limbs share
+// (below that truncation, see k_split_g1).
 //
 // P tiles are 128 (x1) x 32 (x2) with one exponent each.  The output-law
 // contraction reads P limbs transposed ([l][x2][x1]) so that its K index x1 is
@@ -173,7 +175,15 @@ __global__ void k_split_w(const double *__restrict__ W, int64_t len, int KR, int
 // element's source.  texp[chunk][strip] is the exponent of the 32 x 128 values
 // read, which the quarter cross term accumulates from the tiles they come from
 // (and which equals the tile exponent of the complete P).
-template <int L, bool TRI>
+//
+// MG (L >= 4): the last two limbs share one FP32 accumulator, the last one
+// entering as l 2^-b (an exact fp16).  That accumulator's sums are no longer
+// exact integers: over a 128-row chunk its FP32 roundings add at most about
+// 32 units of limb L - 2, 2^5 2^(-b (L - 2)) in the units of p's tile maximum,
+// below the truncation of p to L limbs (the chunk's count mass times
+// 2^(-b (L - 1)), 2^-21 against 2^-22 at (28,14)), and the promotion converts
+// one accumulator fewer per element.
+template <int L, bool TRI, bool MG>
 __global__ void __launch_bounds__(512) k_split_g1(int KLp, int Kdim, int lda, int ncols, const _Float16 *__restrict__ aT,
                                                   const double *__restrict__ X, const int32_t *__restrict__ texp,
                                                   int bits, int ks, double *__restrict__ Tpart) {
@@ -267,7 +277,7 @@ __global__ void __launch_bounds__(512) k_split_g1(int KLp, int Kdim, int lda, in
     // FP64 range keeps its limbs and scales zero)
     et[u] = max(texp[(r0 / 128) * tcols + n0 / 32], -1100);
   };
-  const double radix = ldexp(1.0, bits);
+  const double radix = ldexp(1.0, bits), rradix = ldexp(1.0, -bits);
   // limbs of rows (2 rp, 2 rp + 1) of column 2 cp + q, one packed half2 per
   // limb: split_begin scales the p values of a slot by its tile exponent,
   // split_limb(l) takes limb l off them (called for l = 0, 1, ... in order)
@@ -284,7 +294,10 @@ __global__ void __launch_bounds__(512) k_split_g1(int KLp, int Kdim, int lda, in
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
       const double l0 = rint(v[q][0]), l1 = rint(v[q][1]);
-      hl[l][q] = int_to_half2(l0, l1);
+      if (MG && l == L - 1)
+        hl[l][q] = int_to_half2(l0 * rradix, l1 * rradix);
+      else
+        hl[l][q] = int_to_half2(l0, l1);
       v[q][0] = (v[q][0] - l0) * radix;
       v[q][1] = (v[q][1] - l1) * radix;
     }
@@ -302,12 +315,13 @@ __global__ void __launch_bounds__(512) k_split_g1(int KLp, int Kdim, int lda, in
     return *reinterpret_cast<const half8 *>(&sm[decltype(buf)::value][row * LD + kk * 16 + lk]);
   };
 
-  float16_ acc[L];
+  constexpr int NA = MG ? L - 1 : L;  // accumulators
+  float16_ acc[NA];
   double acc64[16];
 #pragma unroll
   for (int r = 0; r < 16; ++r) acc64[r] = 0.0;
 #pragma unroll
-  for (int l = 0; l < L; ++l) acc[l] = (float16_){0};
+  for (int l = 0; l < NA; ++l) acc[l] = (float16_){0};
   const int nst = krows / BK;
   // Stage s is multiplied from LDS buffer s % 2 while stage s + 1 (its counts
   // and limbs already in registers) is stored into the other buffer, so one
@@ -336,8 +350,10 @@ __global__ void __launch_bounds__(512) k_split_g1(int KLp, int Kdim, int lda, in
     for (int kk = 0; kk < BK / 16; ++kk) {
       const half8 a = opnd(Cur{}, wave * 32 + lr, kk);
 #pragma unroll
-      for (int l = 0; l < L; ++l)
-        acc[l] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, opnd(Cur{}, BM + l * BN + lr, kk), acc[l], 0, 0, 0);
+      for (int l = 0; l < L; ++l) {
+        const int la = l < NA ? l : NA - 1;
+        acc[la] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, opnd(Cur{}, BM + l * BN + lr, kk), acc[la], 0, 0, 0);
+      }
       // a share of the next stage's limbs per group of multiplications; the
       // scheduling barrier keeps it there (left alone, the scheduler sinks the
       // split past the barrier, to the LDS stores that consume it)
@@ -346,18 +362,18 @@ __global__ void __launch_bounds__(512) k_split_g1(int KLp, int Kdim, int lda, in
       __builtin_amdgcn_sched_barrier(0);
     }
     if constexpr (decltype(odd)::value) {  // 128 K rows done: promote the exact integers to FP64
-      double sc[L];
+      double sc[NA];
 #pragma unroll
-      for (int l = 0; l < L; ++l) sc[l] = ldexp(1.0, e - bits * l);
+      for (int l = 0; l < NA; ++l) sc[l] = ldexp(1.0, e - bits * l);
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
         double t = 0.0;
 #pragma unroll
-        for (int l = 0; l < L; ++l) t = fma((double)acc[l][r], sc[l], t);
+        for (int l = 0; l < NA; ++l) t = fma((double)acc[l][r], sc[l], t);
         acc64[r] += t;
       }
 #pragma unroll
-      for (int l = 0; l < L; ++l) acc[l] = (float16_){0};
+      for (int l = 0; l < NA; ++l) acc[l] = (float16_){0};
     }
     __syncthreads();
   };
@@ -916,23 +932,26 @@ int launch_split_vw(const double *V, int KL, int KLp, int N2, const double *W, i
 }
 
 int launch_split_g1(int KLp, int Kdim, int lda, int ncols, const _Float16 *aT, const double *X, const int32_t *texp,
-                    int limbs, int bits, int ks, double *Tpart, int tri, hipStream_t s) {
+                    int limbs, int bits, int ks, double *Tpart, int tri, int merge, hipStream_t s) {
   if (ks < 1 || Kdim % (ks * 128) || ncols % 64 || lda < Kdim || lda % 8) return BDC_ERR_ARG;
   if (tri && (ncols != 2 * Kdim || ncols < 256)) return BDC_ERR_ARG;
   const int blocks = ks * (ncols / 32) * ((KLp + 255) / 256);
-#define LG(LL)                                                                                                    \
-  if (tri)                                                                                                        \
-    hipLaunchKernelGGL((k_split_g1<LL, true>), dim3(blocks), dim3(512), 0, s, KLp, Kdim, lda, ncols, aT, X, texp, \
-                       bits, ks, Tpart);                                                                          \
-  else                                                                                                            \
-    hipLaunchKernelGGL((k_split_g1<LL, false>), dim3(blocks), dim3(512), 0, s, KLp, Kdim, lda, ncols, aT, X, texp, \
-                       bits, ks, Tpart)
+#define LG3(LL, TT, MM)                                                                                            \
+  hipLaunchKernelGGL((k_split_g1<LL, TT, MM>), dim3(blocks), dim3(512), 0, s, KLp, Kdim, lda, ncols, aT, X, texp, bits, \
+                     ks, Tpart)
+#define LG(LL)                                              \
+  if (merge && LL >= 4) {                                   \
+    if (tri) LG3(LL, true, (LL >= 4)); else LG3(LL, false, (LL >= 4)); \
+  } else {                                                  \
+    if (tri) LG3(LL, true, false); else LG3(LL, false, false); \
+  }
   // five limbs for the narrow limbs of half length >= 15 (7 bits: 35 bits of p per tile)
   if (limbs == 5) {
     LG(5);
   } else {
     BDC_SPLIT_L(limbs, LG)
   }
+#undef LG3
 #undef LG
   return hipGetLastError() == hipSuccess ? BDC_OK : BDC_ERR_HIP;
 }
