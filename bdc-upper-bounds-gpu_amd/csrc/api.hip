@@ -485,6 +485,7 @@ int bdc_create(int n, int k, const char *cache_dir, bdc_ctx **out) {
   BDC_TRY_C(c->Hc.alloc(c->nx * 8));
   const size_t ny = (size_t)1 << k;
   BDC_TRY_C(c->qv.alloc(4 * ny * 8));  // q, L, q2 (scratch), grad
+  BDC_TRY_C(c->qsum.alloc((ny / 256 + 1) * 8));
   if (!chain) {
     BDC_TRY_C(c->T.alloc((size_t)c->KLp * c->N2 * 8));
     if (c->KR) BDC_TRY_C(c->U.alloc((size_t)c->KRp * c->N1 * 8));
@@ -579,7 +580,7 @@ int bdc_destroy(bdc_ctx *c) {
   for (bdc::DevBuf *b : {&c->abar16, &c->bbar16, &c->abarh, &c->abarTh, &c->bbarh, &c->residA, &c->rowptrA,
                          &c->entA, &c->rowptrB, &c->entB, &c->blkA, &c->blkB, &c->restiles[0], &c->restiles[1], &c->g1part, &c->repmask, &c->wordbase, &c->orbits, &c->P,
                          &c->Hc, &c->T, &c->U, &c->qpart, &c->qv, &c->g2a, &c->g2b, &c->splitk, &c->part, &c->scal, &c->exptab,
-                         &c->hist, &c->ptexp, &c->vlimb, &c->wlimb, &c->theta, &c->adam_m,
+                         &c->hist, &c->ptexp, &c->qsum, &c->vlimb, &c->wlimb, &c->theta, &c->adam_m,
                          &c->adam_v, &c->grad, &c->dvec, &c->ck_p, &c->ck_s, &c->ck_t, &c->lvl, &c->g64part,
                          &c->T2, &c->V2, &c->qtiles[0], &c->qtiles[1], &c->qtiles[2]})
     b->release();
@@ -751,8 +752,11 @@ static int chain_epi_blocks(const bdc_ctx *c) {
 // mirror: use q_j(y) = q_{k-j}(rev y) for j in JR (X reversal-symmetric, n even).
 // rows = N1/2: X holds only the half space x1 < N1/2 of a complement-symmetric
 // p; the contractions run over those rows and q_j(y) = R_j(y) + R_j(~y).
+// defer_center: leave L = ln q uncentred for the stage contraction that
+// consumes it next (cross_operands centres it, in the deletion chain when that
+// runs)
 static int output_law(bdc_ctx *c, int prec, const double *X, int scale_slot, bool mirror, int rows, double *qout,
-                      double *Lout) {
+                      double *Lout, bool defer_center = false) {
   const int k = c->k, N1 = c->N1, N2 = c->N2;
   const size_t ny = (size_t)1 << k;
   const Opnd none{nullptr, 0, 0, 0};
@@ -873,7 +877,9 @@ static int output_law(bdc_ctx *c, int prec, const double *X, int scale_slot, boo
     qc.nR = (int)c->JR.size();
     qc.qR = qR;
   }
-  BDC_LAUNCH(launch_q_combine(qc, c->inv_c, c->scal.as<double>(), qout, Lout, c->part.as<double>(), c->stream));
+  BDC_LAUNCH(launch_q_combine(qc, c->inv_c, c->scal.as<double>(), qout, Lout, c->qsum.as<double>(),
+                              defer_center ? 0 : 1, c->stream));
+  c->l_uncentred = defer_center;
   return BDC_OK;
 }
 
@@ -923,12 +929,19 @@ static int cross_operands_nat(bdc_ctx *c, const double *L, bool mirror, int rows
       jb.row0[jb.count] = nrows;
       jb.rev = c->cr ? 1 : 0;  // Vbar straight into the stored column order
       const bool fuse_max = amax && mirror && c->n1 == c->n2 && (size_t)nrows <= amax_cap;
-      const int rc = launch_chain_del(L, c->g2b.as<double>(), c->n2, jb, fuse_max ? amax : nullptr, c->stream);
+      const int rc = launch_chain_del(L, c->g2b.as<double>(), c->n2, jb, fuse_max ? amax : nullptr,
+                                      c->l_uncentred ? c->qsum.as<double>() : nullptr, k, c->scal.as<double>(),
+                                      c->stream);
+      if (rc == BDC_OK) c->l_uncentred = false;
       if (rc == BDC_ERR_HIP) return fail(rc, "chain_del launch failed");
       staged = rc == BDC_OK;
       if (staged && fuse_max && namax) *namax = nrows;
       c->g2b_stored = staged && c->cr;
     }
+  }
+  if (c->l_uncentred) {  // another stage form reads L: centre it first
+    BDC_LAUNCH(launch_q_center(k, c->qsum.as<double>(), c->scal.as<double>(), const_cast<double *>(L), c->stream));
+    c->l_uncentred = false;
   }
   for (size_t i = 0; i < c->JL.size() && !staged; ++i) {
     const int j = c->JL[i], m = k - j;
@@ -1060,7 +1073,7 @@ static int complete_mirrors(bdc_ctx *c) {
 // one BA step on the device over the rows x1 < rows; stats land in hist[idx]
 static int ba_step(bdc_ctx *c, int prec, int idx, int rows, double *hist) {
   double *q = c->qv.as<double>(), *L = q + ((size_t)1 << c->k);
-  BDC_TRY(output_law(c, prec, c->P.as<double>(), kScalPScale, true, rows, q, L));
+  BDC_TRY(output_law(c, prec, c->P.as<double>(), kScalPScale, true, rows, q, L, !c->chain));
   if (prec == BDC_PREC_FP64) {
     BDC_TRY(cross_operands(c, L, true, rows));
   } else {

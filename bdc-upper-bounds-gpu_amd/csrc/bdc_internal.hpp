@@ -132,6 +132,8 @@ struct bdc_ctx {
   bdc::DevBuf U;      // Ubar (KRp x N1) FP64 (general path)
   bdc::DevBuf qpart;  // per-j output-law slices (|J| x 2^k) FP64
   bdc::DevBuf qv;     // q (2^k) FP64, then L = ln q - mu
+  bdc::DevBuf qsum;   // block sums of ln q (2^k / 256) from the output-law combine
+  bool l_uncentred = false;  // L holds ln q; its centring is left to the stage contraction
   bdc::DevBuf g2a;    // Wbar part: N1 x KRp FP64
   bdc::DevBuf g2b;    // Vbar part: KLp x N2 FP64
   bdc::DevBuf splitk; // split-K partials for stage GEMMs

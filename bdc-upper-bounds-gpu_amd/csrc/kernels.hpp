@@ -93,9 +93,12 @@ struct QCombine {
   int scale_slot;              // device scalar holding the scale of the weights (p = scale * P)
   int comp;                    // 1: slices are half-space sums R_j; q_j(y) = R_j(y) + R_j(~y)
 };
-// part: 2^k / 256 doubles of scratch
+// part: 2^k / 256 doubles of scratch (the block sums of ln q).  center = 0
+// leaves L = ln q uncentred: launch_q_center, or a deletion chain given the
+// block sums (launch_chain_del), centres it later
 int launch_q_combine(const QCombine &qc, double inv_c, double *scal, double *qout, double *Lout, double *part,
-                     hipStream_t s);
+                     int center, hipStream_t s);
+int launch_q_center(int k, const double *part, double *scal, double *Lout, hipStream_t s);
 // theta -> q = softmax(theta), L = ln q - mu (dual bound)
 int launch_softmax_log(const double *theta, int k, double *q, double *L, double *scal, hipStream_t s);
 // D-pass reductions for the dual: max D, then pi = softmax(beta D) and U over
@@ -145,7 +148,10 @@ struct ChainJobs {
 };
 int launch_chain_ins(const double *src, double *dst, int L, const ChainJobs &jobs, hipStream_t s);
 // amax (may be null): one double per block (row), the largest |entry| written
-int launch_chain_del(const double *src, double *dst, int L, const ChainJobs &jobs, double *amax, hipStream_t s);
+// mu_part (may be null): src is ln q uncentred and mu_part its 2^k/256 block
+// sums; the chain subtracts their mean mu on load and records it in scal
+int launch_chain_del(const double *src, double *dst, int L, const ChainJobs &jobs, double *amax,
+                     const double *mu_part, int mu_k, double *scal, hipStream_t s);
 struct MirrorW {  // for each j in JR: column offset in Wbar, m = k - j, row offset of V_{k-j} in Vbar
   int count;
   int offR[18], m[18], offL[18];

@@ -323,30 +323,45 @@ __global__ void __launch_bounds__(1024) k_chain_ins(const double *__restrict__ s
   }
 }
 
-// amax (may be null): amax[block] = max |out| over the row the block writes
+// amax (may be null): amax[block] = max |out| over the row the block writes.
+// mu_part (may be null): the source is ln q uncentred; mu is the mean over its
+// 2^mu_k entries from the 2^mu_k / 256 block sums in mu_part, formed exactly as
+// k_q_center forms it (threads below 256 sum every 256th partial, then a block
+// sum whose other waves add +0.0), and subtracted on load; block 0 records it.
 __global__ void __launch_bounds__(1024) k_chain_del(const double *__restrict__ src, double *__restrict__ dst, int L,
-                                                    ChainJobs jb, double *__restrict__ amax) {
+                                                    ChainJobs jb, double *__restrict__ amax,
+                                                    const double *__restrict__ mu_part, int mu_k,
+                                                    double *__restrict__ scal) {
   extern __shared__ double lds[];
   __shared__ double red[16];
   const int i = chain_job(jb, blockIdx.x), y = blockIdx.x - jb.row0[i], d = jb.d[i], m = L - d;
   const double *in = src + jb.src_off[i] + (int64_t)y * jb.src_ld[i];
   double *out = dst + jb.dst_off[i] + (int64_t)y * jb.dst_ld[i];
   const uint32_t tid = threadIdx.x, bd = blockDim.x;
-  double mx = 0.0;
+  double mx = 0.0, mu = 0.0;
+  if (mu_part) {  // uniform
+    const int np = (int)(((1u << mu_k) + 255) / 256);
+    double sm = 0.0;
+    if (tid < 256)
+      for (int j = (int)tid; j < np; j += 256) sm += mu_part[j];
+    sm = block_sum(sm, red);
+    mu = sm / (double)(1u << mu_k);
+    if (blockIdx.x == 0 && tid == 0) scal[kScalMu] = mu;
+  }
   // reversed outputs: the steps run on the source read at reversed indices
   // (c(rev u, rev v) = c(u, v)), so output u' is the natural output rev u'
   const int rv = jb.rev;
   auto srci = [&](uint32_t t) { return rv ? rev_bits(t, m) : t; };
   if (d == 0) {
     for (uint32_t t = tid; t < (1u << L); t += bd) {
-      const double v = in[srci(t)];
+      const double v = in[srci(t)] - mu;
       out[t] = v;
       mx = fmax(mx, fabs(v));
     }
   } else {
     const double f = factorial(d);
     double *cur = lds, *nxt = lds + (1u << (L - 1));  // each region holds up to 2^(L-1) entries
-    for (uint32_t t = tid; t < (1u << m); t += bd) cur[t] = in[srci(t)];
+    for (uint32_t t = tid; t < (1u << m); t += bd) cur[t] = in[srci(t)] - mu;
     __syncthreads();
     int l = m;  // bits of cur
     for (int s = 1; s <= d; ++s, ++l) {
@@ -413,10 +428,11 @@ int launch_chain_ins(const double *src, double *dst, int L, const ChainJobs &job
   hipLaunchKernelGGL(k_chain_ins, dim3(jobs.row0[jobs.count]), dim3(1024), sizeof(double) << L, s, src, dst, L, jobs);
   return hipGetLastError() == hipSuccess ? BDC_OK : BDC_ERR_HIP;
 }
-int launch_chain_del(const double *src, double *dst, int L, const ChainJobs &jobs, double *amax, hipStream_t s) {
-  if (chain_check(L, jobs) != BDC_OK) return BDC_ERR_ARG;
+int launch_chain_del(const double *src, double *dst, int L, const ChainJobs &jobs, double *amax,
+                     const double *mu_part, int mu_k, double *scal, hipStream_t s) {
+  if (chain_check(L, jobs) != BDC_OK || (mu_part && !scal)) return BDC_ERR_ARG;
   hipLaunchKernelGGL(k_chain_del, dim3(jobs.row0[jobs.count]), dim3(1024), sizeof(double) << L, s, src, dst, L, jobs,
-                     amax);
+                     amax, mu_part, mu_k, scal);
   return hipGetLastError() == hipSuccess ? BDC_OK : BDC_ERR_HIP;
 }
 

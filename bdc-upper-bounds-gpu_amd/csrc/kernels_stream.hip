@@ -147,11 +147,16 @@ __global__ void __launch_bounds__(256) k_q_center(int k, const double *__restric
   if (blockIdx.x == 0 && threadIdx.x == 0) scal[kScalMu] = mu;
 }
 int launch_q_combine(const QCombine &qc, double inv_c, double *scal, double *qout, double *Lout, double *part,
-                     hipStream_t s) {
+                     int center, hipStream_t s) {
   const uint32_t ny = 1u << qc.k;
   const int nb = (int)((ny + 255) / 256);
   hipLaunchKernelGGL(k_q_combine, dim3(nb), dim3(256), 0, s, qc, inv_c, scal, qout, Lout, part);
-  hipLaunchKernelGGL(k_q_center, dim3(nb), dim3(256), 0, s, qc.k, part, nb, scal, Lout);
+  if (center) hipLaunchKernelGGL(k_q_center, dim3(nb), dim3(256), 0, s, qc.k, part, nb, scal, Lout);
+  return hipGetLastError() == hipSuccess ? BDC_OK : BDC_ERR_HIP;
+}
+int launch_q_center(int k, const double *part, double *scal, double *Lout, hipStream_t s) {
+  const int nb = (int)(((1u << k) + 255) / 256);
+  hipLaunchKernelGGL(k_q_center, dim3(nb), dim3(256), 0, s, k, part, nb, scal, Lout);
   return hipGetLastError() == hipSuccess ? BDC_OK : BDC_ERR_HIP;
 }
 
