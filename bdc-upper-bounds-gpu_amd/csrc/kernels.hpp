@@ -203,9 +203,13 @@ int launch_split_vw(const double *V, int KL, int KLp, int N2, const double *W, i
 // (row stride ncols; split into limbs on the fly with the tile exponents texp), aT row stride lda
 // tri: X = P of the quarter-layout BA step (Kdim = ncols / 2), read from the
 // triangle of canonical elements (the others may be stale).  merge (limbs >= 4):
-// the last two limbs share one FP32 accumulator
+// the last two limbs share one FP32 accumulator; fields: five limbs of 7 to 9
+// bits (with merge) are taken as bit fields of the rounded value; inplace_loop
+// (tri): the leading stages wholly in the triangle run a loop without the
+// per-element source select
 int launch_split_g1(int KLp, int Kdim, int lda, int ncols, const _Float16 *aT, const double *X, const int32_t *texp,
-                    int limbs, int bits, int ks, double *Tpart, int tri, int merge, hipStream_t s);
+                    int limbs, int bits, int ks, double *Tpart, int tri, int merge, int fields, int inplace_loop,
+                    hipStream_t s);
 // residual corrections of G1 from the rows x1 < rows (tri: X read from the triangle, as launch_split_g1)
 int launch_resid_g1(const int32_t *res, int nres, const double *X, int rows, int N2, int tri, double *T,
                     hipStream_t s);

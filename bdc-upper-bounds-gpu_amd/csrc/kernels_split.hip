@@ -41,6 +41,59 @@ __device__ inline half2_ int_to_half2(double a, double b) {
 __device__ inline _Float16 int_to_half(double a) { return int_to_half2(a, 0.0)[0]; }
 typedef float float16_ __attribute__((ext_vector_type(16)));
 
+// ---- limbs by bit-field extraction (L limbs of B <= 9 bits, L B + 1 <= 51)
+// Adding the constant ix_magic<L, B>() to a scaled value v (|v| <= 2^(B-1))
+// rounds v to F = (L - 1) B fractional bits (to nearest, ties to even: the
+// limbs of the rint chain represent the same value) and leaves J + Bias in the
+// low mantissa bits, J = v 2^F as an integer and Bias = 2^(B-1) in every B-bit
+// digit position.  Each biased digit is then a bit field: B bits (B + 1 for the
+// leading one, whose biased value reaches 2^B), and limb l is field (L-1-l)
+// minus 2^(B-1), in [-2^(B-1), 2^(B-1)].  The field becomes an FP16 by writing
+// it under the exponent of 1024 (the mantissa holds 10 bits) and subtracting
+// 1024 + 2^(B-1) exactly: four integer operations and one packed FP16 one per
+// pair of values and limb, where the rint chain takes nine FP64 operations and
+// conversions.
+template <int L, int B>
+__device__ constexpr double ix_magic() {
+  uint64_t bias = 0;
+  for (int j = 0; j < L; ++j) bias += (1ull << (B - 1)) << (j * B);
+  return (double)((3ull << 51) + bias) / (double)(1ull << ((L - 1) * B));
+}
+__device__ inline uint32_t rotr32(uint32_t x, int s) { return s ? __builtin_amdgcn_alignbit(x, x, s) : x; }
+// limb l of two values that carry ix_magic (a in the low half, b in the high
+// half); with S the limb is scaled by 2^-B (exact: a multiple of 2^-B of
+// magnitude at most 1/2)
+template <int L, int B, bool S>
+__device__ inline half2_ ix_limb(int l, double a, double b) {
+  static_assert(B >= 2 && B <= 9 && L * B + 1 <= 51, "fields must fit the mantissa and an FP16 mantissa");
+  const uint64_t ua = __builtin_bit_cast(uint64_t, a), ub = __builtin_bit_cast(uint64_t, b);
+  const uint32_t la = (uint32_t)ua, ha = (uint32_t)(ua >> 32), lb = (uint32_t)ub, hb = (uint32_t)(ub >> 32);
+  const int o = (L - 1 - l) * B;  // bit offset of the field
+  const uint32_t m = (l == 0 ? (2u << B) : (1u << B)) - 1u;
+  uint32_t fa, fb;  // the field of a at bit 0, that of b at bit 16
+  if (o >= 32) {
+    fa = rotr32(ha, o - 32);
+    fb = rotr32(hb, (o - 48) & 31);
+  } else if (o >= 16) {
+    fa = __builtin_amdgcn_alignbit(ha, la, o);
+    fb = o == 16 ? lb : __builtin_amdgcn_alignbit(hb, lb, o - 16);
+  } else {
+    fa = o ? __builtin_amdgcn_alignbit(ha, la, o) : la;
+    fb = rotr32(lb, (o - 16) & 31);
+  }
+  const uint32_t w = (fb & (m << 16)) | ((fa & m) | 0x64006400u);
+  const half2_ h = __builtin_bit_cast(half2_, w);
+  constexpr _Float16 off = (_Float16)(1024 + (1 << (B - 1)));
+  if constexpr (S) {
+    constexpr _Float16 r = (_Float16)(1.0f / (float)(1 << B)), c = (_Float16)(-(float)(1024 + (1 << (B - 1))) / (float)(1 << B));
+    return __builtin_elementwise_fma(h, (half2_){r, r}, (half2_){c, c});
+  }
+  return h - (half2_){off, off};
+}
+
+// sources of a stage of p in the output law's triangle reads
+enum : int { kGeneral = 0, kInPlace = 1 };
+
 // row of the 32x32 MFMA result held by `lane` in register r (verified layout)
 __device__ inline int mfma32_row(int lane, int r) { return (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3); }
 
@@ -183,10 +236,13 @@ __global__ void k_split_w(const double *__restrict__ W, int64_t len, int KR, int
 // below the truncation of p to L limbs (the chunk's count mass times
 // 2^(-b (L - 1)), 2^-21 against 2^-22 at (28,14)), and the promotion converts
 // one accumulator fewer per element.
-template <int L, bool TRI, bool MG>
+//
+// IB > 0 (= bits, with MG): the limbs come from bit fields of the value
+// rounded to its last limb (ix_limb), not from the rint chain.
+template <int L, bool TRI, bool MG, int IB>
 __global__ void __launch_bounds__(512) k_split_g1(int KLp, int Kdim, int lda, int ncols, const _Float16 *__restrict__ aT,
                                                   const double *__restrict__ X, const int32_t *__restrict__ texp,
-                                                  int bits, int ks, double *__restrict__ Tpart) {
+                                                  int bits, int ks, double *__restrict__ Tpart, int inplace_loop) {
   constexpr int BM = 256, BN = 32, BK = 64, LD = 72;
   constexpr int ROWS = BM + L * BN;  // A rows, then L limb slabs of BN rows (x2)
   __shared__ __attribute__((aligned(16))) _Float16 sm[2][ROWS * LD];  // two stage buffers
@@ -214,7 +270,8 @@ __global__ void __launch_bounds__(512) k_split_g1(int KLp, int Kdim, int lda, in
   const uint32_t offX = (uint32_t)(2 * rp * ncols + 2 * cp) * 8u;
   // triangle reads.  Byte offsets from X (< 2^30: quarter layouts have n <= 28): element (r, c) in
   // place at ((r0 + i) ncols + n0 + q) 8 + offX, transposed at ut(r0, i, q) +
-  // offT; it is canonical iff dq[q] <= -(r0 + i)
+  // offT; it is canonical iff dq[q] <= -(r0 + i).  Right of the middle offT
+  // can be negative (wrapped): add it to ut in 32 bits, never to a pointer.
   const bool left = n0 < ncols / 2;
   const uint32_t offT = left ? (uint32_t)(2 * cp * ncols + 2 * rp) * 8u
                              : (uint32_t)((30 - 2 * cp) * ncols + 30 - 2 * rp) * 8u;
@@ -250,11 +307,14 @@ __global__ void __launch_bounds__(512) k_split_g1(int KLp, int Kdim, int lda, in
       ra[i] = *reinterpret_cast<const half8 *>(cA + (int64_t)dr * lda * 2 + (offA + k2));
     }
   };
-  auto gload_x = [&](int s, auto slot) {
+  // where: kGeneral picks each element's source; kInPlace (every element of
+  // the stage canonical) reads as without TRI
+  auto gload_x = [&](int s, auto slot, auto where) {
     constexpr int u = decltype(slot)::value;
+    constexpr int w = decltype(where)::value;
     const char *xs = cX + (int64_t)s * BK * ncols * 8;  // stage base (64-bit, uniform)
     const int r0 = kbase + s * BK;
-    if constexpr (!TRI) {
+    if constexpr (!TRI || w == kInPlace) {
 #pragma unroll
       for (int i = 0; i < 2; ++i) rx[u][i] = *reinterpret_cast<const double2 *>(xs + (int64_t)i * ncols * 8 + offX);
     } else {  // each element in place or transposed (a branch on the stage's
@@ -289,8 +349,23 @@ __global__ void __launch_bounds__(512) k_split_g1(int KLp, int Kdim, int lda, in
     v[0][1] = ldexp(rx[u][1].x, -e);
     v[1][0] = ldexp(rx[u][0].y, -e);
     v[1][1] = ldexp(rx[u][1].y, -e);
+    if constexpr (IB > 0) {
+      constexpr double mg = ix_magic<L, IB ? IB : 2>();
+#pragma unroll
+      for (int q = 0; q < 2; ++q)
+#pragma unroll
+        for (int i = 0; i < 2; ++i) v[q][i] += mg;
+    }
   };
   auto split_limb = [&](int l, double(&v)[2][2]) {
+    if constexpr (IB > 0) {
+      static_assert(MG, "the bit-field limbs come with the merged accumulator");
+#pragma unroll
+      for (int q = 0; q < 2; ++q)
+        hl[l][q] = l == L - 1 ? ix_limb<L, IB ? IB : 2, true>(l, v[q][0], v[q][1])
+                              : ix_limb<L, IB ? IB : 2, false>(l, v[q][0], v[q][1]);
+      return;
+    }
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
       const double l0 = rint(v[q][0]), l1 = rint(v[q][1]);
@@ -335,14 +410,14 @@ __global__ void __launch_bounds__(512) k_split_g1(int KLp, int Kdim, int lda, in
   // compile-time parity: a stage is one basic block, and the split can be
   // scheduled beside the multiplications).
   int eh[2];
-  auto step = [&](int s, auto ld, auto sp, auto odd) {
+  auto step = [&](int s, auto ld, auto sp, auto odd, auto where) {
     constexpr int b = decltype(odd)::value ? 1 : 0;
     using Cur = std::integral_constant<int, b>;
     using Nxt = std::integral_constant<int, 1 - b>;
     const int e = eh[b];
     sstore(Nxt{});
     gload_a(min(s + 2, nst - 1));
-    gload_x(min(s + 4, nst - 1), ld);
+    gload_x(min(s + 4, nst - 1), ld, where);
     double v[2][2];
     split_begin(sp, v);
     eh[b] = et[decltype(sp)::value];
@@ -362,15 +437,15 @@ __global__ void __launch_bounds__(512) k_split_g1(int KLp, int Kdim, int lda, in
       __builtin_amdgcn_sched_barrier(0);
     }
     if constexpr (decltype(odd)::value) {  // 128 K rows done: promote the exact integers to FP64
-      double sc[NA];
-#pragma unroll
-      for (int l = 0; l < NA; ++l) sc[l] = ldexp(1.0, e - bits * l);
+      // (Horner in units of the last accumulator: exact while the sums span at
+      // most 53 bits, then one rounding into acc64)
+      const double sc = ldexp(1.0, e - bits * (NA - 1));
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
-        double t = 0.0;
+        double t = (double)acc[0][r];
 #pragma unroll
-        for (int l = 0; l < NA; ++l) t = fma((double)acc[l][r], sc[l], t);
-        acc64[r] += t;
+        for (int l = 1; l < NA; ++l) t = fma(t, radix, (double)acc[l][r]);
+        acc64[r] = fma(t, sc, acc64[r]);
       }
 #pragma unroll
       for (int l = 0; l < NA; ++l) acc[l] = (float16_){0};
@@ -385,9 +460,10 @@ __global__ void __launch_bounds__(512) k_split_g1(int KLp, int Kdim, int lda, in
   using E1 = std::true_type;
   // prologue: stage 0 into buffer 0; the counts and limbs of stage 1 in
   // registers; p of stages 2 and 3 in slots 2 and 0
-  gload_x(0, U0{});
-  gload_x(min(1, nst - 1), U1{});
-  gload_x(min(2, nst - 1), U2{});
+  using GN0 = std::integral_constant<int, kGeneral>;
+  gload_x(0, U0{}, GN0{});
+  gload_x(min(1, nst - 1), U1{}, GN0{});
+  gload_x(min(2, nst - 1), U2{}, GN0{});
   gload_a(0);
   {
     double v[2][2];
@@ -405,29 +481,46 @@ __global__ void __launch_bounds__(512) k_split_g1(int KLp, int Kdim, int lda, in
 #pragma unroll
     for (int l = 0; l < L; ++l) split_limb(l, v);
   }
-  gload_x(min(3, nst - 1), U0{});
+  gload_x(min(3, nst - 1), U0{}, GN0{});
   __syncthreads();
   // six stages per iteration (three ring slots, promotion every second stage):
   // stage s0 + i loads into slot (i + 1) % 3, splits slot (i + 2) % 3 and has
   // parity i % 2, since s0 is a multiple of 6
   int s0 = 0;
+  auto six = [&](int s, auto where) {
+    step(s, U1{}, U2{}, E0{}, where);
+    step(s + 1, U2{}, U0{}, E1{}, where);
+    step(s + 2, U0{}, U1{}, E0{}, where);
+    step(s + 3, U1{}, U2{}, E1{}, where);
+    step(s + 4, U2{}, U0{}, E0{}, where);
+    step(s + 5, U0{}, U1{}, E1{}, where);
+  };
+  if constexpr (TRI) {
+    // A strip's rows above the diagonal band of its columns lie in the
+    // triangle (rows r <= cmin of every column are canonical): while the
+    // stages loaded (four ahead) are among them, a copy of the loop reads in
+    // place without the per-element source select.  (A third copy reading the
+    // rows below the band as 16-byte pairs of the partner rows was slower than
+    // the per-element loads: 0.428 against 0.420 ms per step.)
+    const int cmin = left ? n0 : ncols - 32 - n0;
+    const int sin = inplace_loop && cmin - (BK - 1) - kbase >= 0 ? (cmin - (BK - 1) - kbase) / BK + 1 : 0;
 #pragma unroll 1
-  for (; s0 + 6 <= nst; s0 += 6) {
-    step(s0, U1{}, U2{}, E0{});
-    step(s0 + 1, U2{}, U0{}, E1{});
-    step(s0 + 2, U0{}, U1{}, E0{});
-    step(s0 + 3, U1{}, U2{}, E1{});
-    step(s0 + 4, U2{}, U0{}, E0{});
-    step(s0 + 5, U0{}, U1{}, E1{});
+    for (; s0 + 6 <= nst && s0 + 10 <= sin; s0 += 6) six(s0, std::integral_constant<int, kInPlace>{});
+#pragma unroll 1
+    for (; s0 + 6 <= nst; s0 += 6) six(s0, std::integral_constant<int, kGeneral>{});
+  } else {
+#pragma unroll 1
+    for (; s0 + 6 <= nst; s0 += 6) six(s0, std::integral_constant<int, kInPlace>{});
   }
   // the last nst mod 6 stages (nst is even: K rows come in multiples of 128)
+  using GN = std::integral_constant<int, kGeneral>;
   if (s0 < nst) {
-    step(s0, U1{}, U2{}, E0{});
-    step(s0 + 1, U2{}, U0{}, E1{});
+    step(s0, U1{}, U2{}, E0{}, GN{});
+    step(s0 + 1, U2{}, U0{}, E1{}, GN{});
   }
   if (s0 + 2 < nst) {
-    step(s0 + 2, U0{}, U1{}, E0{});
-    step(s0 + 3, U1{}, U2{}, E1{});
+    step(s0 + 2, U0{}, U1{}, E0{}, GN{});
+    step(s0 + 3, U1{}, U2{}, E1{}, GN{});
   }
   double *T = Tpart + (int64_t)kz * KLp * ncols;
 #pragma unroll
@@ -932,25 +1025,40 @@ int launch_split_vw(const double *V, int KL, int KLp, int N2, const double *W, i
 }
 
 int launch_split_g1(int KLp, int Kdim, int lda, int ncols, const _Float16 *aT, const double *X, const int32_t *texp,
-                    int limbs, int bits, int ks, double *Tpart, int tri, int merge, hipStream_t s) {
+                    int limbs, int bits, int ks, double *Tpart, int tri, int merge, int fields, int inplace_loop,
+                    hipStream_t s) {
   if (ks < 1 || Kdim % (ks * 128) || ncols % 64 || lda < Kdim || lda % 8) return BDC_ERR_ARG;
   if (tri && (ncols != 2 * Kdim || ncols < 256)) return BDC_ERR_ARG;
   const int blocks = ks * (ncols / 32) * ((KLp + 255) / 256);
-#define LG3(LL, TT, MM)                                                                                            \
-  hipLaunchKernelGGL((k_split_g1<LL, TT, MM>), dim3(blocks), dim3(512), 0, s, KLp, Kdim, lda, ncols, aT, X, texp, bits, \
-                     ks, Tpart)
+#define LG4(LL, TT, MM, IB)                                                                                          \
+  hipLaunchKernelGGL((k_split_g1<LL, TT, MM, IB>), dim3(blocks), dim3(512), 0, s, KLp, Kdim, lda, ncols, aT, X, texp, \
+                     bits, ks, Tpart, inplace_loop)
+#define LG3(LL, TT, MM) LG4(LL, TT, MM, 0)
 #define LG(LL)                                              \
   if (merge && LL >= 4) {                                   \
     if (tri) LG3(LL, true, (LL >= 4)); else LG3(LL, false, (LL >= 4)); \
   } else {                                                  \
     if (tri) LG3(LL, true, false); else LG3(LL, false, false); \
   }
+#define LFB(IB)                                  \
+  if (tri) LG4(5, true, true, IB); else LG4(5, false, true, IB);
   // five limbs for the narrow limbs of half length >= 15 (7 bits: 35 bits of p per tile)
-  if (limbs == 5) {
+  if (limbs == 5 && merge && fields && bits >= 7 && bits <= 9) {
+    // limbs from bit fields (7 to 9 bits: half lengths 14 to 16)
+    if (bits == 9) {
+      LFB(9);
+    } else if (bits == 8) {
+      LFB(8);
+    } else {
+      LFB(7);
+    }
+  } else if (limbs == 5) {
     LG(5);
   } else {
     BDC_SPLIT_L(limbs, LG)
   }
+#undef LFB
+#undef LG4
 #undef LG3
 #undef LG
   return hipGetLastError() == hipSuccess ? BDC_OK : BDC_ERR_HIP;

@@ -540,6 +540,58 @@ def test_quarter_triangle_reads_match_mirrored(bdc, monkeypatch, n, k):
     assert np.array_equal(pa, pb)
 
 
+@pytest.mark.parametrize("n,k,bits", [(16, 8, 9), (20, 10, 8), (24, 12, 7)])
+def test_g1_bit_field_limbs_match_rint_chain(bdc, monkeypatch, n, k, bits):
+    """The output law's five limbs of p taken as bit fields of the value
+    rounded to its last limb (the default for limbs of 7 to 9 bits, forced here
+    through the limb width) against the chain of rint steps
+    (BDC_G1_FIELDS=0).  Both represent the same rounded values (ties to even),
+    so the output law and the steps agree to the FP32 rounding of the
+    accumulator the last two limbs share; signed weights exercise negative
+    leading limbs, and both agree with FP64 to 2^-40 of the law of |w|."""
+    say(f"G1 bit-field limbs n={n} k={k} bits={bits}")
+    monkeypatch.setenv("BDC_SPLIT_G1_BITS", str(bits))
+    rng = np.random.default_rng(n + bits)
+    w = rng.standard_normal(1 << n)
+    out = {}
+    for mode in ("1", "0"):
+        monkeypatch.setenv("BDC_G1_FIELDS", mode)
+        with bdc.Engine(n, k) as e:
+            cfg = e.split_config()
+            assert (cfg["g1_bits"], cfg["g1_limbs"]) == (bits, 5)
+            law = (e.output_law(w, bdc.PREC_SPLIT16), e.output_law(w), e.output_law(np.abs(w)))
+            e.set_p_dirichlet(1.0, 5)
+            st = e.iterate(6, bdc.PREC_SPLIT16)
+            out[mode] = (law, st, e.certify())
+    (la, sa, ca), (lb, sb, cb) = out["1"], out["0"]
+    for q16, q64, qabs in (la, lb):
+        assert np.all(np.abs(q16 - q64) <= 2.0**-40 * qabs)
+    assert np.all(np.abs(la[0] - lb[0]) <= 2.0**-44 * la[2])
+    np.testing.assert_allclose(sa, sb, rtol=1e-12, atol=1e-14)
+    assert abs(ca["upper_cert"] - cb["upper_cert"]) < 1e-12
+    assert abs(ca["lower"] - cb["lower"]) < 1e-12
+
+
+@pytest.mark.parametrize("n,k", [(24, 12)])
+def test_g1_inplace_loop_bitwise(bdc, monkeypatch, n, k):
+    """The output law's copy of its loop for the stages wholly inside the
+    triangle (read in place, no per-element source select) against the general
+    loop alone (BDC_G1_INPLACE_LOOP=0): the same values in the same order, so
+    the steps agree bitwise."""
+    say(f"G1 in-place loop n={n} k={k}")
+    out = {}
+    for mode in ("1", "0"):
+        monkeypatch.setenv("BDC_G1_INPLACE_LOOP", mode)
+        with bdc.Engine(n, k) as e:
+            e.set_p_dirichlet(1.0, 17)
+            st = e.iterate(5, bdc.PREC_SPLIT16)
+            out[mode] = (st, e.certify(), e.get_p_orbits())
+    (sa, ca, pa), (sb, cb, pb) = out["1"], out["0"]
+    assert np.array_equal(sa, sb)
+    assert ca == cb
+    assert np.array_equal(pa, pb)
+
+
 @pytest.mark.parametrize("n,k", [(16, 8), (20, 10), (24, 12)])
 def test_column_order_matches_natural(bdc, monkeypatch, n, k):
     """P, H and D stored with bit-reversed columns (the reversal becomes a
