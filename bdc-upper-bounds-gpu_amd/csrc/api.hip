@@ -888,23 +888,26 @@ static int output_law(bdc_ctx *c, int prec, const double *X, int scale_slot, boo
 // only the first `rows` rows of Wbar are formed then.  amax (may be null):
 // receives one maximum of |Vbar| per row block of the deletion-chain launch,
 // and *namax their number, when that launch formed Vbar and Wbar is its
-// permutation (else *namax = 0).
+// permutation (else *namax = 0).  defer_w (may be null): with those maxima,
+// Wbar is not formed; its mirror description goes to *defer_w (count > 0) for
+// a launch that also splits it into limbs.
 static int cross_operands_nat(bdc_ctx *c, const double *L, bool mirror, int rows, double *amax, size_t amax_cap,
-                              int *namax);
+                              int *namax, MirrorW *defer_w);
 // Vbar in the stored column order of P when the columns are bit-reversed: the
 // factored stage writes it so (g2b_stored), the other stage forms write natural
 // order into g2b and it is permuted into V2
 static double *vbar(bdc_ctx *c) { return c->cr && !c->g2b_stored ? c->V2.as<double>() : c->g2b.as<double>(); }
 static int cross_operands(bdc_ctx *c, const double *L, bool mirror = false, int rows = 0, double *amax = nullptr,
-                          size_t amax_cap = 0, int *namax = nullptr) {
-  BDC_TRY(cross_operands_nat(c, L, mirror, rows, amax, amax_cap, namax));
+                          size_t amax_cap = 0, int *namax = nullptr, MirrorW *defer_w = nullptr) {
+  BDC_TRY(cross_operands_nat(c, L, mirror, rows, amax, amax_cap, namax, defer_w));
   if (c->cr && !c->chain && !c->g2b_stored)
     BDC_LAUNCH(launch_permute_cols(c->g2b.as<double>(), c->V2.as<double>(), (uint64_t)c->KLp, c->cr, c->stream));
   return BDC_OK;
 }
 static int cross_operands_nat(bdc_ctx *c, const double *L, bool mirror, int rows, double *amax, size_t amax_cap,
-                              int *namax) {
+                              int *namax, MirrorW *defer_w) {
   if (namax) *namax = 0;
+  if (defer_w) defer_w->count = 0;
   c->g2b_stored = false;
   if (c->chain) return chain_cross(c, L);
   const int k = c->k, N1 = c->N1, N2 = c->N2;
@@ -975,6 +978,10 @@ static int cross_operands_nat(bdc_ctx *c, const double *L, bool mirror, int rows
       if (mw.offL[i] < 0) return fail(BDC_ERR_STATE, "mirror slice for j=%d missing", c->JR[i]);
     }
     mw.stored = c->g2b_stored ? 1 : 0;
+    if (defer_w && namax && *namax > 0) {
+      *defer_w = mw;
+      return BDC_OK;
+    }
     BDC_LAUNCH(launch_w_from_v(mw, c->g2b.as<double>(), N2, c->n2, c->g2a.as<double>(), rows > 0 ? rows : N1, c->KR,
                                c->KRp, c->stream));
     return BDC_OK;
@@ -1913,15 +1920,24 @@ int split_cross_operands(bdc_ctx *c, const double *L, bool mirror, int rows, boo
   // V, W in FP64 (stage contractions); the deletion chain also leaves the block
   // maxima of |V| for the limb exponent when W is a permutation of V
   int namax = 0;
-  BDC_TRY(cross_operands(c, L, mirror, rows, c->part.as<double>(), c->part.bytes / 8, &namax));
+  MirrorW mw{};
+  BDC_TRY(cross_operands(c, L, mirror, rows, c->part.as<double>(), c->part.bytes / 8, &namax,
+                         env_int("BDC_VW_FUSED", 1) ? &mw : nullptr));
   const Split &sp = c->split;
   // the quarter step's tile exponents are reset on the way (its cross term
   // accumulates them with atomic maxima)
   const int nt = reset_texp ? (rows / 128) * (c->N2 / 32) : 0;
-  BDC_LAUNCH(launch_split_vw(vbar(c), c->KL, c->KLp, c->N2, c->g2a.as<double>(), rows, c->KR, c->KRp,
-                             sp.g2_limbs, sp.g2_bits, c->vlimb.as<_Float16>(), c->wlimb.as<_Float16>(),
-                             c->part.as<double>(), namax, c->scal.as<double>(), c->ptexp.as<int32_t>(), nt,
-                             c->stream));
+  if (mw.count > 0) {  // Wbar formed from Vbar with its limbs, beside the V limbs
+    BDC_LAUNCH(launch_vw_limbs(vbar(c), c->KL, c->KLp, c->N2, mw, c->g2b.as<double>(), c->n2, c->g2a.as<double>(),
+                               rows > 0 ? rows : c->N1, c->KR, c->KRp, sp.g2_limbs, sp.g2_bits,
+                               c->vlimb.as<_Float16>(), c->wlimb.as<_Float16>(), c->part.as<double>(), namax,
+                               c->scal.as<double>(), c->ptexp.as<int32_t>(), nt, c->stream));
+  } else {
+    BDC_LAUNCH(launch_split_vw(vbar(c), c->KL, c->KLp, c->N2, c->g2a.as<double>(), rows, c->KR, c->KRp,
+                               sp.g2_limbs, sp.g2_bits, c->vlimb.as<_Float16>(), c->wlimb.as<_Float16>(),
+                               c->part.as<double>(), namax, c->scal.as<double>(), c->ptexp.as<int32_t>(), nt,
+                               c->stream));
+  }
   c->texp_reset = nt > 0;
   return BDC_OK;
 }

@@ -199,6 +199,12 @@ int launch_tile_exp(const double *P, int M, int N2, int bits, int32_t *texp, hip
 int launch_split_vw(const double *V, int KL, int KLp, int N2, const double *W, int M, int KR, int KRp, int limbs,
                     int bits, _Float16 *vT, _Float16 *wl, double *part, int np_pre, double *scal, int32_t *fill,
                     int fill_n, hipStream_t s);
+// launch_w_from_v (the M rows of Wbar under the mirror rule, from Vm) and
+// launch_split_vw in one launch, when the tiled form of the former applies and
+// the np block maxima of |Vbar| are given (else the two launches)
+int launch_vw_limbs(const double *V, int KL, int KLp, int N2, const MirrorW &mw, const double *Vm, int n2, double *W,
+                    int M, int KR, int KRp, int limbs, int bits, _Float16 *vT, _Float16 *wl, double *part, int np,
+                    double *scal, int32_t *fill, int fill_n, hipStream_t s);
 // ks partial slabs (ks * KLp * ncols FP64) of Tbar = aT X over the first Kdim rows of the FP64 array X
 // (row stride ncols; split into limbs on the fly with the tile exponents texp), aT row stride lda
 // tri: X = P of the quarter-layout BA step (Kdim = ncols / 2), read from the

@@ -204,6 +204,81 @@ __global__ void k_split_w(const double *__restrict__ W, int64_t len, int KR, int
   }
 }
 
+// The V limbs and, under the mirror rule (W a permutation of the entries of
+// V), W and its limbs in one launch: blocks [0, nbv) split 64 x 64 tiles of V
+// as k_split_v; the others form 64 (kappa') x 32 (x1) tiles of W from Vm
+// through LDS (as k_w_from_v_tiled: V rows offL + rev(kappa' - offR), x1 the
+// columns c0 .. c0 + 31, or their bit reversals for Vm in natural order) and
+// split them as k_split_w, writing zero limbs in the padding columns.  The same
+// arithmetic on the same values: bitwise the limbs of the separate kernels.
+template <int L>
+__global__ void __launch_bounds__(256) k_vw_limbs(const double *__restrict__ V, int KL, int KLp, int N2, int bits,
+                                                  const double *__restrict__ part, int np, double *__restrict__ scal,
+                                                  _Float16 *__restrict__ vT, int32_t *__restrict__ fill, int fill_n,
+                                                  MirrorW mw, const double *__restrict__ Vm, int n2, int sh,
+                                                  double *__restrict__ W, _Float16 *__restrict__ wl, int M, int KR,
+                                                  int KRp, int nbv) {
+  __shared__ double t[64][65];
+  __shared__ double red[16];
+  const int tid = threadIdx.x;
+  for (int i = blockIdx.x * blockDim.x + tid; i < fill_n; i += gridDim.x * blockDim.x) fill[i] = INT_MIN;
+  double m = 0.0;
+  for (int i = tid; i < np; i += blockDim.x) m = fmax(m, part[i]);
+  m = block_max(m, red);
+  const int e = tile_exponent(m, bits);
+  if (blockIdx.x == 0 && tid == 0) scal[kScalVScale] = (double)e;
+  const double radix = ldexp(1.0, bits);
+  if ((int)blockIdx.x < nbv) {
+    const int ntx = N2 / 64, x0 = ((int)blockIdx.x % ntx) * 64, k0 = ((int)blockIdx.x / ntx) * 64;
+    for (int i = tid; i < 64 * 64; i += 256) {
+      const int kk = i >> 6, xx = i & 63;
+      t[kk][xx] = (k0 + kk < KL) ? ldexp(V[(int64_t)(k0 + kk) * N2 + x0 + xx], -e) : 0.0;
+    }
+    __syncthreads();
+    for (int i = tid; i < 64 * 64; i += 256) {
+      const int xx = i >> 6, kk = i & 63;
+      double v = t[kk][xx];
+      for (int l = 0; l < L; ++l) {
+        const double li = rint(v);
+        vT[(int64_t)l * N2 * KLp + (int64_t)(x0 + xx) * KLp + k0 + kk] = int_to_half(li);
+        v = (v - li) * radix;
+      }
+    }
+    return;
+  }
+  const int b = (int)blockIdx.x - nbv, nkw = (KR + 63) / 64;
+  const int k0 = (b % nkw) * 64, c0 = (b / nkw) * 32, cl = tid & 31;
+  for (int i = 0; i < 8; ++i) {
+    const int kl = (tid >> 5) + 8 * i, kap = k0 + kl;
+    if (kap < KR) {
+      int s = 0;
+      while (s + 1 < mw.count && kap >= mw.offR[s + 1]) ++s;
+      const uint32_t rz = rev_bits((uint32_t)(kap - mw.offR[s]), mw.m[s]);
+      const uint32_t col = mw.stored ? (uint32_t)(c0 + cl) : (uint32_t)(c0 + cl) << sh;
+      t[kl][cl] = Vm[(int64_t)(mw.offL[s] + rz) * N2 + col];
+    }
+  }
+  __syncthreads();
+  const int kl = tid & 63, kap = k0 + kl;
+  if (kap >= KRp) return;
+  const int64_t len = (int64_t)M * KRp;
+  for (int i = 0; i < 8; ++i) {
+    const int r = (tid >> 6) + 4 * i;
+    const uint32_t x1 = mw.stored ? (uint32_t)(c0 + r) : rev_bits((uint32_t)(c0 + r) << sh, n2);
+    const int64_t o = (int64_t)x1 * KRp + kap;
+    double v = 0.0;
+    if (kap < KR) {
+      W[o] = t[kl][r];
+      v = ldexp(t[kl][r], -e);
+    }
+    for (int l = 0; l < L; ++l) {
+      const double li = rint(v);
+      wl[(int64_t)l * len + o] = int_to_half(li);
+      v = (v - li) * radix;
+    }
+  }
+}
+
 // ----------------------------------------------------------------- G1
 // Tbar[kappa][c] = sum_k aT[kappa][k] X[k][c] over the first Kdim rows of the
 // FP64 array X (row stride ncols), lda = row stride of aT (>= Kdim: the
@@ -1021,6 +1096,26 @@ int launch_split_vw(const double *V, int KL, int KLp, int N2, const double *W, i
     BDC_SPLIT_L(limbs, LW)
 #undef LW
   }
+  return hipGetLastError() == hipSuccess ? BDC_OK : BDC_ERR_HIP;
+}
+
+int launch_vw_limbs(const double *V, int KL, int KLp, int N2, const MirrorW &mw, const double *Vm, int n2, double *W,
+                    int M, int KR, int KRp, int limbs, int bits, _Float16 *vT, _Float16 *wl, double *part, int np,
+                    double *scal, int32_t *fill, int fill_n, hipStream_t s) {
+  int b = 0;
+  while (b < 31 && (1 << b) < M) ++b;
+  const bool tiled = (1 << b) == M && M >= 32 && b <= n2 && N2 == (1 << n2) && KR > 0;
+  if (!tiled || np <= 0 || N2 % 64 || KLp % 64) {  // the separate launches
+    const int rc = launch_w_from_v(mw, Vm, N2, n2, W, M, KR, KRp, s);
+    if (rc != BDC_OK) return rc;
+    return launch_split_vw(V, KL, KLp, N2, W, M, KR, KRp, limbs, bits, vT, wl, part, np, scal, fill, fill_n, s);
+  }
+  const int nbv = (N2 / 64) * (KLp / 64), nbw = ((KR + 63) / 64) * (M / 32);
+#define LVW(LL)                                                                                                       \
+  hipLaunchKernelGGL(k_vw_limbs<LL>, dim3((unsigned)(nbv + nbw)), dim3(256), 0, s, V, KL, KLp, N2, bits, part, np, scal, \
+                     vT, fill, fill_n, mw, Vm, n2, n2 - b, W, wl, M, KR, KRp, nbv)
+  BDC_SPLIT_L(limbs, LVW)
+#undef LVW
   return hipGetLastError() == hipSuccess ? BDC_OK : BDC_ERR_HIP;
 }
 

@@ -592,6 +592,26 @@ def test_g1_inplace_loop_bitwise(bdc, monkeypatch, n, k):
     assert np.array_equal(pa, pb)
 
 
+@pytest.mark.parametrize("n,k", [(16, 8), (24, 12)])
+def test_vw_limbs_one_launch_bitwise(bdc, monkeypatch, n, k):
+    """W formed from V under the mirror rule together with the V and W limbs in
+    one launch (the default) against W from V, then the limbs of V and of W in
+    their own launches (BDC_VW_FUSED=0): the same arithmetic on the same
+    values, so split steps agree bitwise."""
+    say(f"V/W limbs in one launch n={n} k={k}")
+    out = {}
+    for mode in ("1", "0"):
+        monkeypatch.setenv("BDC_VW_FUSED", mode)
+        with bdc.Engine(n, k) as e:
+            e.set_p_dirichlet(1.0, 23)
+            st = e.iterate(5, bdc.PREC_SPLIT16)
+            out[mode] = (st, e.certify(), e.get_p_orbits())
+    (sa, ca, pa), (sb, cb, pb) = out["1"], out["0"]
+    assert np.array_equal(sa, sb)
+    assert ca == cb
+    assert np.array_equal(pa, pb)
+
+
 @pytest.mark.parametrize("n,k", [(16, 8), (20, 10), (24, 12)])
 def test_column_order_matches_natural(bdc, monkeypatch, n, k):
     """P, H and D stored with bit-reversed columns (the reversal becomes a
