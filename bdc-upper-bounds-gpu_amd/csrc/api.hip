@@ -1909,7 +1909,7 @@ int split_output_law_g1(bdc_ctx *c, const double *X, int rows) {
   double *dstT = ks > 1 ? c->g1part.as<double>() : c->T.as<double>();
   BDC_LAUNCH(launch_split_g1(c->KLp, rows, c->N1, c->N2, c->abarTh.as<_Float16>(), X, c->ptexp.as<int32_t>(),
                              sp.g1_limbs, sp.g1_bits, ks, dstT, tri, env_int("BDC_G1_MERGE", 1),
-                             env_int("BDC_G1_FIELDS", 1), env_int("BDC_G1_INPLACE_LOOP", 1), c->stream));
+                             env_int("BDC_G1_FIELDS", 1), env_int("BDC_G1_UNIFORM_LOOPS", 1), c->stream));
   if (ks > 1) BDC_LAUNCH(launch_reduce_splits(dstT, ks, (int64_t)slab, (uint64_t)slab, c->T.as<double>(), c->stream));
   BDC_LAUNCH(launch_resid_g1(c->residA.as<int32_t>(), c->n_resid_a, X, rows, c->N2, tri, c->T.as<double>(), c->stream));
   return BDC_OK;

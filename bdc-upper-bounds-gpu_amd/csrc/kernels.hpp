@@ -210,11 +210,11 @@ int launch_vw_limbs(const double *V, int KL, int KLp, int N2, const MirrorW &mw,
 // tri: X = P of the quarter-layout BA step (Kdim = ncols / 2), read from the
 // triangle of canonical elements (the others may be stale).  merge (limbs >= 4):
 // the last two limbs share one FP32 accumulator; fields: five limbs of 7 to 9
-// bits (with merge) are taken as bit fields of the rounded value; inplace_loop
-// (tri): the leading stages wholly in the triangle run a loop without the
-// per-element source select
+// bits (with merge) are taken as bit fields of the rounded value; uniform_loops
+// (tri): the stages wholly inside the triangle and wholly outside it run
+// copies of the loop without the per-element source select
 int launch_split_g1(int KLp, int Kdim, int lda, int ncols, const _Float16 *aT, const double *X, const int32_t *texp,
-                    int limbs, int bits, int ks, double *Tpart, int tri, int merge, int fields, int inplace_loop,
+                    int limbs, int bits, int ks, double *Tpart, int tri, int merge, int fields, int uniform_loops,
                     hipStream_t s);
 // residual corrections of G1 from the rows x1 < rows (tri: X read from the triangle, as launch_split_g1)
 int launch_resid_g1(const int32_t *res, int nres, const double *X, int rows, int N2, int tri, double *T,

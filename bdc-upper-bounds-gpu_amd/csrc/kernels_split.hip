@@ -92,7 +92,7 @@ __device__ inline half2_ ix_limb(int l, double a, double b) {
 }
 
 // sources of a stage of p in the output law's triangle reads
-enum : int { kGeneral = 0, kInPlace = 1 };
+enum : int { kGeneral = 0, kInPlace = 1, kTransposed = 2 };
 
 // row of the 32x32 MFMA result held by `lane` in register r (verified layout)
 __device__ inline int mfma32_row(int lane, int r) { return (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3); }
@@ -314,13 +314,16 @@ __global__ void __launch_bounds__(256) k_vw_limbs(const double *__restrict__ V, 
 //
 // IB > 0 (= bits, with MG): the limbs come from bit fields of the value
 // rounded to its last limb (ix_limb), not from the rint chain.
-template <int L, bool TRI, bool MG, int IB>
-__global__ void __launch_bounds__(512) k_split_g1(int KLp, int Kdim, int lda, int ncols, const _Float16 *__restrict__ aT,
-                                                  const double *__restrict__ X, const int32_t *__restrict__ texp,
-                                                  int bits, int ks, double *__restrict__ Tpart, int inplace_loop) {
-  constexpr int BM = 256, BN = 32, BK = 64, LD = 72;
-  constexpr int ROWS = BM + L * BN;  // A rows, then L limb slabs of BN rows (x2)
-  __shared__ __attribute__((aligned(16))) _Float16 sm[2][ROWS * LD];  // two stage buffers
+// The body, with the side of the block's columns (SIDE 1: left of the middle,
+// 2: right; 0: either, only without TRI) a compile-time constant.
+template <int L, bool TRI, bool MG, int IB, int SIDE, int ROWS, int LD>
+__device__ __forceinline__ void split_g1_body(_Float16 (&sm)[2][ROWS * LD], int KLp, int Kdim, int lda, int ncols,
+                                              const _Float16 *__restrict__ aT, const double *__restrict__ X,
+                                              const int32_t *__restrict__ texp, int bits, int ks,
+                                              double *__restrict__ Tpart, int uniform_loops) {
+  constexpr int BM = 256, BN = 32, BK = 64;
+  static_assert(ROWS == BM + L * BN && LD == 72, "stage buffer shape");
+  static_assert(SIDE != 0 || !TRI, "the triangle reads depend on the side");
   const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int kz = blockIdx.x % ks;
   const int tile = blockIdx.x / ks;
@@ -347,7 +350,7 @@ __global__ void __launch_bounds__(512) k_split_g1(int KLp, int Kdim, int lda, in
   // place at ((r0 + i) ncols + n0 + q) 8 + offX, transposed at ut(r0, i, q) +
   // offT; it is canonical iff dq[q] <= -(r0 + i).  Right of the middle offT
   // can be negative (wrapped): add it to ut in 32 bits, never to a pointer.
-  const bool left = n0 < ncols / 2;
+  constexpr bool left = SIDE == 1;
   const uint32_t offT = left ? (uint32_t)(2 * cp * ncols + 2 * rp) * 8u
                              : (uint32_t)((30 - 2 * cp) * ncols + 30 - 2 * rp) * 8u;
   auto ut = [&](int r0, int i, int q) -> uint32_t {
@@ -383,7 +386,8 @@ __global__ void __launch_bounds__(512) k_split_g1(int KLp, int Kdim, int lda, in
     }
   };
   // where: kGeneral picks each element's source; kInPlace (every element of
-  // the stage canonical) reads as without TRI
+  // the stage canonical) reads as without TRI; kTransposed (none canonical)
+  // reads from the partners as 16-byte pairs
   auto gload_x = [&](int s, auto slot, auto where) {
     constexpr int u = decltype(slot)::value;
     constexpr int w = decltype(where)::value;
@@ -392,6 +396,14 @@ __global__ void __launch_bounds__(512) k_split_g1(int KLp, int Kdim, int lda, in
     if constexpr (!TRI || w == kInPlace) {
 #pragma unroll
       for (int i = 0; i < 2; ++i) rx[u][i] = *reinterpret_cast<const double2 *>(xs + (int64_t)i * ncols * 8 + offX);
+    } else if constexpr (w == kTransposed) {
+      // rows r0 + 2 rp + i of column c are adjacent in the partner row (left:
+      // row c from column r0 + 2 rp; right: row N - 1 - c from column
+      // N - 2 - r0 - 2 rp, descending): one 16-byte load per column, kept in
+      // the order loaded (a reordering move here would wait for the load)
+      const char *Xc = reinterpret_cast<const char *>(X);
+#pragma unroll
+      for (int q = 0; q < 2; ++q) rx[u][q] = *reinterpret_cast<const double2 *>(Xc + (ut(r0, left ? 0 : 1, q) + offT));
     } else {  // each element in place or transposed (a branch on the stage's
               // position here makes the unrolled loop spill)
       const char *Xc = reinterpret_cast<const char *>(X);
@@ -417,13 +429,24 @@ __global__ void __launch_bounds__(512) k_split_g1(int KLp, int Kdim, int lda, in
   // limb: split_begin scales the p values of a slot by its tile exponent,
   // split_limb(l) takes limb l off them (called for l = 0, 1, ... in order)
   half2_ hl[L][2];
-  auto split_begin = [&](auto slot, double(&v)[2][2]) {
+  // (tp: the slot holds each column's two rows as loaded in the partner row,
+  // rx[u][q] = lower address first: rows (0, 1) left of the middle, (1, 0)
+  // right of it; else rx[u][i] = the row's two columns)
+  auto split_begin = [&](auto slot, double(&v)[2][2], auto tp) {
     constexpr int u = decltype(slot)::value;
     const int e = et[u];
-    v[0][0] = ldexp(rx[u][0].x, -e);
-    v[0][1] = ldexp(rx[u][1].x, -e);
-    v[1][0] = ldexp(rx[u][0].y, -e);
-    v[1][1] = ldexp(rx[u][1].y, -e);
+    if constexpr (decltype(tp)::value) {
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        v[q][0] = ldexp(left ? rx[u][q].x : rx[u][q].y, -e);
+        v[q][1] = ldexp(left ? rx[u][q].y : rx[u][q].x, -e);
+      }
+    } else {
+      v[0][0] = ldexp(rx[u][0].x, -e);
+      v[0][1] = ldexp(rx[u][1].x, -e);
+      v[1][0] = ldexp(rx[u][0].y, -e);
+      v[1][1] = ldexp(rx[u][1].y, -e);
+    }
     if constexpr (IB > 0) {
       constexpr double mg = ix_magic<L, IB ? IB : 2>();
 #pragma unroll
@@ -494,7 +517,7 @@ __global__ void __launch_bounds__(512) k_split_g1(int KLp, int Kdim, int lda, in
     gload_a(min(s + 2, nst - 1));
     gload_x(min(s + 4, nst - 1), ld, where);
     double v[2][2];
-    split_begin(sp, v);
+    split_begin(sp, v, std::integral_constant<bool, decltype(where)::value == kTransposed>{});
     eh[b] = et[decltype(sp)::value];
 #pragma unroll
     for (int kk = 0; kk < BK / 16; ++kk) {
@@ -542,7 +565,7 @@ __global__ void __launch_bounds__(512) k_split_g1(int KLp, int Kdim, int lda, in
   gload_a(0);
   {
     double v[2][2];
-    split_begin(U0{}, v);
+    split_begin(U0{}, v, std::false_type{});
     eh[0] = et[0];
 #pragma unroll
     for (int l = 0; l < L; ++l) split_limb(l, v);
@@ -551,7 +574,7 @@ __global__ void __launch_bounds__(512) k_split_g1(int KLp, int Kdim, int lda, in
   gload_a(min(1, nst - 1));
   {
     double v[2][2];
-    split_begin(U1{}, v);
+    split_begin(U1{}, v, std::false_type{});
     eh[1] = et[1];
 #pragma unroll
     for (int l = 0; l < L; ++l) split_limb(l, v);
@@ -572,17 +595,39 @@ __global__ void __launch_bounds__(512) k_split_g1(int KLp, int Kdim, int lda, in
   };
   if constexpr (TRI) {
     // A strip's rows above the diagonal band of its columns lie in the
-    // triangle (rows r <= cmin of every column are canonical): while the
-    // stages loaded (four ahead) are among them, a copy of the loop reads in
-    // place without the per-element source select.  (A third copy reading the
-    // rows below the band as 16-byte pairs of the partner rows was slower than
-    // the per-element loads: 0.428 against 0.420 ms per step.)
-    const int cmin = left ? n0 : ncols - 32 - n0;
-    const int sin = inplace_loop && cmin - (BK - 1) - kbase >= 0 ? (cmin - (BK - 1) - kbase) / BK + 1 : 0;
+    // triangle, those below it are all read from their partners (rows
+    // r <= cmin of every column are canonical, rows r > cmax of none): while
+    // the stages loaded (four ahead) are all of one kind, a copy of the loop
+    // reads them without the per-element source select.  The two slots loaded
+    // and not yet split when the transposed copy starts and ends (ring slots 2
+    // and 0) are reordered between the two register orders there.
+    const int cmin = left ? n0 : ncols - 32 - n0, cmax = left ? n0 + 31 : ncols - 1 - n0;
+    const int sin = uniform_loops && cmin - (BK - 1) - kbase >= 0 ? (cmin - (BK - 1) - kbase) / BK + 1 : 0;
+    const int str = !uniform_loops ? nst : cmax - kbase >= 0 ? (cmax - kbase) / BK + 1 : 0;
 #pragma unroll 1
     for (; s0 + 6 <= nst && s0 + 10 <= sin; s0 += 6) six(s0, std::integral_constant<int, kInPlace>{});
 #pragma unroll 1
-    for (; s0 + 6 <= nst; s0 += 6) six(s0, std::integral_constant<int, kGeneral>{});
+    for (; s0 + 6 <= nst && s0 + 4 < str; s0 += 6) six(s0, std::integral_constant<int, kGeneral>{});
+    if (s0 + 6 <= nst) {
+      auto to_tp = [&](auto slot) {  // rx[u][i] (columns) -> rx[u][q] (rows, as loaded)
+        constexpr int u = decltype(slot)::value;
+        const double2 a = rx[u][0], b = rx[u][1];
+        rx[u][0] = left ? make_double2(a.x, b.x) : make_double2(b.x, a.x);
+        rx[u][1] = left ? make_double2(a.y, b.y) : make_double2(b.y, a.y);
+      };
+      auto from_tp = [&](auto slot) {
+        constexpr int u = decltype(slot)::value;
+        const double2 a = rx[u][0], b = rx[u][1];
+        rx[u][0] = left ? make_double2(a.x, b.x) : make_double2(a.y, b.y);
+        rx[u][1] = left ? make_double2(a.y, b.y) : make_double2(a.x, b.x);
+      };
+      to_tp(U2{});
+      to_tp(U0{});
+#pragma unroll 1
+      for (; s0 + 6 <= nst; s0 += 6) six(s0, std::integral_constant<int, kTransposed>{});
+      from_tp(U2{});
+      from_tp(U0{});
+    }
   } else {
 #pragma unroll 1
     for (; s0 + 6 <= nst; s0 += 6) six(s0, std::integral_constant<int, kInPlace>{});
@@ -602,6 +647,23 @@ __global__ void __launch_bounds__(512) k_split_g1(int KLp, int Kdim, int lda, in
   for (int r = 0; r < 16; ++r) {
     const int row = m0 + wave * 32 + mfma32_row(lane, r);
     if (row < KLp) T[(int64_t)row * ncols + n0 + lr] = acc64[r];
+  }
+}
+
+template <int L, bool TRI, bool MG, int IB>
+__global__ void __launch_bounds__(512) k_split_g1(int KLp, int Kdim, int lda, int ncols, const _Float16 *__restrict__ aT,
+                                                  const double *__restrict__ X, const int32_t *__restrict__ texp,
+                                                  int bits, int ks, double *__restrict__ Tpart, int uniform_loops) {
+  constexpr int LD = 72, ROWS = 256 + L * 32;  // counts, then L limb slabs of 32 rows (x2)
+  __shared__ __attribute__((aligned(16))) _Float16 sm[2][ROWS * LD];  // two stage buffers
+  if constexpr (TRI) {
+    const int n0 = ((int)(blockIdx.x / ks) % (ncols / 32)) * 32;  // (as in the body)
+    if (n0 < ncols / 2)
+      split_g1_body<L, TRI, MG, IB, 1, ROWS, LD>(sm, KLp, Kdim, lda, ncols, aT, X, texp, bits, ks, Tpart, uniform_loops);
+    else
+      split_g1_body<L, TRI, MG, IB, 2, ROWS, LD>(sm, KLp, Kdim, lda, ncols, aT, X, texp, bits, ks, Tpart, uniform_loops);
+  } else {
+    split_g1_body<L, TRI, MG, IB, 0, ROWS, LD>(sm, KLp, Kdim, lda, ncols, aT, X, texp, bits, ks, Tpart, uniform_loops);
   }
 }
 
@@ -1120,14 +1182,14 @@ int launch_vw_limbs(const double *V, int KL, int KLp, int N2, const MirrorW &mw,
 }
 
 int launch_split_g1(int KLp, int Kdim, int lda, int ncols, const _Float16 *aT, const double *X, const int32_t *texp,
-                    int limbs, int bits, int ks, double *Tpart, int tri, int merge, int fields, int inplace_loop,
+                    int limbs, int bits, int ks, double *Tpart, int tri, int merge, int fields, int uniform_loops,
                     hipStream_t s) {
   if (ks < 1 || Kdim % (ks * 128) || ncols % 64 || lda < Kdim || lda % 8) return BDC_ERR_ARG;
   if (tri && (ncols != 2 * Kdim || ncols < 256)) return BDC_ERR_ARG;
   const int blocks = ks * (ncols / 32) * ((KLp + 255) / 256);
 #define LG4(LL, TT, MM, IB)                                                                                          \
   hipLaunchKernelGGL((k_split_g1<LL, TT, MM, IB>), dim3(blocks), dim3(512), 0, s, KLp, Kdim, lda, ncols, aT, X, texp, \
-                     bits, ks, Tpart, inplace_loop)
+                     bits, ks, Tpart, uniform_loops)
 #define LG3(LL, TT, MM) LG4(LL, TT, MM, 0)
 #define LG(LL)                                              \
   if (merge && LL >= 4) {                                   \

@@ -1,10 +1,10 @@
 """CPU check of the output law's triangle reads (k_split_g1 with TRI, the
 quarter-layout BA step): for every block, thread and stage, the byte offsets the
-kernel forms -- in place, or per element in place or from the partner -- evaluated
-with the kernel's 32-bit wrap-around, must stay inside the half space and land on
-the canonical representative of the element the thread multiplies.  The stage
-plan (which of the kernel's two loops loads each stage, four stages ahead)
-follows the kernel's loop bounds.  Offsets right of the middle column are sums of
+kernel forms -- in place, per element in place or from the partner, or from the
+partner as 16-byte pairs -- evaluated with the kernel's 32-bit wrap-around, must
+stay inside the half space and land on the canonical representative of the
+element the thread multiplies.  The stage plan (which of the kernel's three loops
+loads each stage, four stages ahead) follows the kernel's loop bounds.  Offsets right of the middle column are sums of
 a per-lane part that can be negative: they wrap in 32 bits before they meet a
 pointer."""
 import numpy as np
@@ -20,9 +20,10 @@ def g1_splits(rows, blocks):
     return ks
 
 
-def stage_plan(nst, kbase, cmin):
-    """{stage: set of load kinds}: 'I' in place, 'G' per-element select."""
+def stage_plan(nst, kbase, cmin, cmax):
+    """{stage: set of load kinds}: 'I' in place, 'G' per-element select, 'T' partner pairs."""
     sin = (cmin - (BK - 1) - kbase) // BK + 1 if cmin - (BK - 1) - kbase >= 0 else 0
+    st = (cmax - kbase) // BK + 1 if cmax - kbase >= 0 else 0
     plan = {}
 
     def load(s, kind):
@@ -31,7 +32,7 @@ def stage_plan(nst, kbase, cmin):
     for t in range(min(4, nst)):  # prologue
         plan.setdefault(t, set()).add("G")
     s0 = 0
-    for kind, more in (("I", lambda s: s + 10 <= sin), ("G", lambda s: True)):
+    for kind, more in (("I", lambda s: s + 10 <= sin), ("G", lambda s: s + 4 < st), ("T", lambda s: True)):
         while s0 + 6 <= nst and more(s0):
             for j in range(6):
                 load(s0 + j, kind)
@@ -67,11 +68,11 @@ def check(n):
         offx = (2 * rp * ncols + 2 * cp) * 8
         offt = (2 * cp * ncols + 2 * rp) * 8 if left else ((30 - 2 * cp) * ncols + 30 - 2 * rp) * 8
         offt = offt.astype(np.int64) % (1 << 32)  # uint32 in the kernel
-        cmin = n0 if left else ncols - 32 - n0
+        cmin, cmax = (n0, n0 + 31) if left else (ncols - 32 - n0, ncols - 1 - n0)
         for kz in range(ks):
             krows = kdim // ks
             kbase, nst = kz * krows, krows // BK
-            for s, kinds in stage_plan(nst, kbase, cmin).items():
+            for s, kinds in stage_plan(nst, kbase, cmin, cmax).items():
                 r0 = kbase + s * BK
                 for kind in kinds:
                     seen.add(kind)
@@ -85,6 +86,10 @@ def check(n):
                         for q in range(2):
                             if kind == "I":
                                 a = (r0 * ncols + n0) * 8 + i * ncols * 8 + offx + 8 * q
+                            elif kind == "T":  # rows (0, 1) left, (1, 0) right, at ascending addresses
+                                pair = (ut(0 if left else 1, q) + offt) % (1 << 32)
+                                assert np.all(pair % 16 == 0)
+                                a = pair + 8 * (i if left else 1 - i)
                             else:
                                 on = ((r0 + i) * ncols + n0 + q) * 8 + offx
                                 ot = (ut(i, q) + offt) % (1 << 32)
@@ -98,7 +103,7 @@ def check(n):
     return ks, seen
 
 
-@pytest.mark.parametrize("n,kinds", [(16, {"G"}), (20, {"G"}), (24, {"G", "I"})])
+@pytest.mark.parametrize("n,kinds", [(16, {"G"}), (20, {"G", "T"}), (24, {"G", "I", "T"})])
 def test_triangle_reads_land_on_canonical_partners(n, kinds):
     ks, seen = check(n)
     assert seen == kinds

@@ -572,16 +572,18 @@ def test_g1_bit_field_limbs_match_rint_chain(bdc, monkeypatch, n, k, bits):
     assert abs(ca["lower"] - cb["lower"]) < 1e-12
 
 
-@pytest.mark.parametrize("n,k", [(24, 12)])
-def test_g1_inplace_loop_bitwise(bdc, monkeypatch, n, k):
-    """The output law's copy of its loop for the stages wholly inside the
-    triangle (read in place, no per-element source select) against the general
-    loop alone (BDC_G1_INPLACE_LOOP=0): the same values in the same order, so
-    the steps agree bitwise."""
-    say(f"G1 in-place loop n={n} k={k}")
+@pytest.mark.parametrize("n,k", [(20, 10), (24, 12)])
+def test_g1_uniform_loops_bitwise(bdc, monkeypatch, n, k):
+    """The output law's copies of its loop for the stages wholly inside the
+    triangle (read in place) and wholly outside it (read from the partners as
+    16-byte pairs), without the per-element source select, against the general
+    loop alone (BDC_G1_UNIFORM_LOOPS=0): the same values in the same order, so
+    the steps agree bitwise.  (20,10) has the transposed copy only; (24,12) all
+    three, and a second K slab."""
+    say(f"G1 uniform loops n={n} k={k}")
     out = {}
     for mode in ("1", "0"):
-        monkeypatch.setenv("BDC_G1_INPLACE_LOOP", mode)
+        monkeypatch.setenv("BDC_G1_UNIFORM_LOOPS", mode)
         with bdc.Engine(n, k) as e:
             e.set_p_dirichlet(1.0, 17)
             st = e.iterate(5, bdc.PREC_SPLIT16)
