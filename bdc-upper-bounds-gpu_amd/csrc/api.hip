@@ -911,6 +911,14 @@ static int cross_operands_nat(bdc_ctx *c, const double *L, bool mirror, int rows
   c->g2b_stored = false;
   if (c->chain) return chain_cross(c, L);
   const int k = c->k, N1 = c->N1, N2 = c->N2;
+  // ln q left uncentred by the output law is centred by the deletion chain on
+  // load, unless W is formed from L itself below (not the mirror rule): then
+  // it is centred in place first
+  const bool w_from_l = !(mirror && c->n1 == c->n2) && !c->JR.empty();
+  if (c->l_uncentred && w_from_l) {
+    BDC_LAUNCH(launch_q_center(k, c->qsum.as<double>(), c->scal.as<double>(), const_cast<double *>(L), c->stream));
+    c->l_uncentred = false;
+  }
   // V_j = L_j B_{k-j}^T for all j in JL in one launch of factored
   // single-deletion steps (row y1 of L_j, 2^(k-j) entries, to 2^n2 entries)
   bool staged = false;

@@ -157,6 +157,33 @@ def test_ba_fp64_matches_oracle(bdc, oracle, n, k, iters):
         assert (np.diff(stats[:, 3]) >= -1e-12).all()
 
 
+@pytest.mark.parametrize("n,k,precs", [(13, 6, (0,)), (14, 7, (0, 1)), (15, 7, (0, 1))])
+def test_step_statistics_match_oracle(bdc, oracle, n, k, precs):
+    """Every column of the per-step record -- mx, max_x D, ln sum_x p e^D and
+    I(p) -- against the oracle's D(x||q_t) over the orbits before each of its
+    steps.  max_x D and ln Z move with a constant shift of D (the update does
+    not), so this pins the centring of ln q where W is formed from ln q itself
+    (odd n) and where it is a permutation of V (even n)."""
+    say(f"step statistics n={n} k={k}")
+    iters = 6
+    ref = oracle.BA(n, k)
+    reps = ref.orbits()
+    size = np.array([oracle.orbit_size(int(x), n) for x in reps], dtype=np.float64)
+    want = []
+    for _ in range(iters):
+        w, d = size * ref.get_q(), ref.kl()
+        want.append((0.0, d.max(), math.log(np.sum(w * np.exp(d))), np.sum(w * d)))
+        mx = ref.iterate()
+        want[-1] = (mx,) + want[-1][1:]
+    want = np.array(want)
+    with bdc.Engine(n, k) as e:
+        for prec in precs:
+            e.set_p_uniform()
+            st = e.iterate(iters, prec)
+            tol = 1e-10 if prec == bdc.PREC_FP64 else 1e-9
+            np.testing.assert_allclose(st, want, rtol=0, atol=tol, err_msg=f"precision {prec}")
+
+
 @pytest.mark.parametrize("k", [4, 6, 8, 10, 12, 14])
 def test_ba_dirichlet_k_sweep(bdc, oracle, k):
     """The k-sweep workload (Dirichlet(1) init with seed 0 over orbits) at
