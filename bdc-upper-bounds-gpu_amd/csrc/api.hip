@@ -1723,7 +1723,14 @@ int split_prepare(bdc_ctx *c) {
     }
     return h;
   };
-  std::vector<_Float16> ah(N1 * KLp), aTh(KLp * N1), bh(N2 * KRp);
+  // aTh: Abar^T for the output law in MFMA operand order -- 32 kappa x 16 x1
+  // chunks of 64 lanes x 8 values (lane l: kappa 32 t + l % 32, x1 16 c +
+  // 8 (l / 32) + e), chunk (t, c) at (t N1 / 16 + c) * 512 values, kappa
+  // padded with zero rows to a multiple of 256 (the output law's block height)
+  std::vector<_Float16> ah(N1 * KLp), aTh((KLp + 255) / 256 * 256 * N1, (_Float16)0.0f), bh(N2 * KRp);
+  auto opidx = [&](size_t kk, size_t x1) {
+    return (((kk / 32) * (N1 / 16) + x1 / 16) * 64 + (kk % 32) + 32 * ((x1 % 16) / 8)) * 8 + x1 % 8;
+  };
   std::vector<int32_t> resA, rowptrA(N1 + 1, 0), entA, rowptrB(N2 + 1, 0), entB;
   double rowA = 0, rowB = 0, chunk = 0;
   std::vector<double> csum(KLp, 0.0);
@@ -1733,7 +1740,7 @@ int split_prepare(bdc_ctx *c) {
       const uint16_t v = a[x1 * KLp + kk];
       const _Float16 h = round16(v);
       ah[x1 * KLp + kk] = h;
-      aTh[kk * N1 + x1] = h;
+      aTh[opidx(kk, x1)] = h;
       const float hf = (float)h;
       rs += std::fabs(hf);
       csum[kk] += std::fabs(hf);

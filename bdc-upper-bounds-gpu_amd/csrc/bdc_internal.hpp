@@ -108,7 +108,7 @@ struct bdc_ctx {
   bdc::DevBuf abar16;   // Abar as u16, N1 x KLp (zero padded)
   bdc::DevBuf bbar16;   // Bbar as u16, N2 x KRp
   bdc::DevBuf abarh;    // Abar as fp16 (counts rounded to fp16; residual lists below), N1 x KLp
-  bdc::DevBuf abarTh;   // Abar^T as fp16, KLp x N1
+  bdc::DevBuf abarTh;   // Abar^T as fp16 in MFMA operand order (the output law), KLp up to 256s x N1
   bdc::DevBuf bbarh;    // Bbar as fp16, N2 x KRp
 
   // symmetry / orbits

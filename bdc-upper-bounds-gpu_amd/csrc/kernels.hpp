@@ -206,7 +206,8 @@ int launch_vw_limbs(const double *V, int KL, int KLp, int N2, const MirrorW &mw,
                     int M, int KR, int KRp, int limbs, int bits, _Float16 *vT, _Float16 *wl, double *part, int np,
                     double *scal, int32_t *fill, int fill_n, hipStream_t s);
 // ks partial slabs (ks * KLp * ncols FP64) of Tbar = aT X over the first Kdim rows of the FP64 array X
-// (row stride ncols; split into limbs on the fly with the tile exponents texp), aT row stride lda
+// (row stride ncols; split into limbs on the fly with the tile exponents texp).  aT: Abar^T in MFMA
+// operand order over K = lda rows of Abar (abar_operand_order), 16-byte aligned
 // tri: X = P of the quarter-layout BA step (Kdim = ncols / 2), read from the
 // triangle of canonical elements (the others may be stale).  merge (limbs >= 4):
 // the last two limbs share one FP32 accumulator; fields: five limbs of 7 to 9

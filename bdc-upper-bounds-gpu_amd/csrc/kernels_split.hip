@@ -281,15 +281,16 @@ __global__ void __launch_bounds__(256) k_vw_limbs(const double *__restrict__ V, 
 
 // ----------------------------------------------------------------- G1
 // Tbar[kappa][c] = sum_k aT[kappa][k] X[k][c] over the first Kdim rows of the
-// FP64 array X (row stride ncols), lda = row stride of aT (>= Kdim: the
-// half-space step reads the leading rows of Abar).
+// FP64 array X (row stride ncols); the counts aT come in MFMA operand order
+// (split_prepare), lda = the K extent of that array (>= Kdim: the half-space
+// step reads the leading rows of Abar).
 // Block: 256 kappa x 32 columns (all of Tbar's rows for KLp <= 256, so X is
 // read once), 8 waves of 32 x 32 with all L limbs.  A K stage covers 64 rows
-// of X: the counts come as one 128-byte line per row of aT (8 lanes of 16
-// bytes); the 64 x 32 block of X is read as 16-byte pairs, split (one stage ahead)
-// into L integer limbs with the exponent of its 128 x 32 tile, and written to
-// LDS transposed (column-major) next to the counts; rows of 144 bytes keep the
-// MFMA operand reads conflict-free.  The FP32 sums (exact integers) are
+// of X: each wave loads its counts straight into its operand registers, four
+// contiguous 1 KiB chunks; the 64 x 32 block of X is read as 16-byte pairs,
+// split (one stage ahead) into L integer limbs with the exponent of its
+// 128 x 32 tile, and written to LDS transposed (column-major); rows of 144
+// bytes keep the MFMA operand reads conflict-free.  The FP32 sums (exact integers) are
 // promoted to FP64 every 128 K rows, the height of an exponent tile.  ks > 1
 // splits K (partial slabs, reduced afterwards).
 //
@@ -318,11 +319,11 @@ __global__ void __launch_bounds__(256) k_vw_limbs(const double *__restrict__ V, 
 // 2: right; 0: either, only without TRI) a compile-time constant.
 template <int L, bool TRI, bool MG, int IB, int SIDE, int ROWS, int LD>
 __device__ __forceinline__ void split_g1_body(_Float16 (&sm)[2][ROWS * LD], int KLp, int Kdim, int lda, int ncols,
-                                              const _Float16 *__restrict__ aT, const double *__restrict__ X,
+                                              const half8 *__restrict__ aO, const double *__restrict__ X,
                                               const int32_t *__restrict__ texp, int bits, int ks,
                                               double *__restrict__ Tpart, int uniform_loops) {
   constexpr int BM = 256, BN = 32, BK = 64;
-  static_assert(ROWS == BM + L * BN && LD == 72, "stage buffer shape");
+  static_assert(ROWS == L * BN && LD == 72, "stage buffer shape");
   static_assert(SIDE != 0 || !TRI, "the triangle reads depend on the side");
   const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int kz = blockIdx.x % ks;
@@ -331,9 +332,12 @@ __device__ __forceinline__ void split_g1_body(_Float16 (&sm)[2][ROWS * LD], int 
   const int n0 = (tile % ntn) * BN, m0 = (tile / ntn) * BM;
   const int tcols = ncols / 32;
   const int krows = Kdim / ks, kbase = kz * krows;
-  const int q8 = (tid & 7) * 8, rbase = tid >> 3, lr = lane & 31, lk = 8 * (lane >> 5);
+  const int lr = lane & 31, lk = 8 * (lane >> 5);
 
-  // counts: thread rows rbase + 64 i (i < 4) of aT.  X: thread takes rows
+  // counts: aO holds Abar^T in MFMA operand order, 1 KiB per 32 kappa x 16 K
+  // chunk (lane l: row l % 32, K columns 8 (l / 32) .. + 7), chunks of a kappa
+  // tile consecutive along K (lda / 16 of them), so a wave's operand of a
+  // stage is four contiguous 1 KiB reads.  X: thread takes rows
   // 2 rp, 2 rp + 1 and columns 2 cp, 2 cp + 1 of the stage; a wave covers 8
   // values of each (rp = 8 (wave / 2) + lane % 8, cp = 8 (wave % 2) + lane / 8),
   // so the transposed 4-byte limb stores (LDS word 36 (2 cp + q) + rp) fall in
@@ -342,9 +346,9 @@ __device__ __forceinline__ void split_g1_body(_Float16 (&sm)[2][ROWS * LD], int 
   // offsets that carry the stage offset (a loop-invariant sum would be hoisted
   // as VGPR pointers).
   const int rp = 8 * (wave >> 1) + (lane & 7), cp = 8 * (wave & 1) + (lane >> 3);
-  const char *cA = reinterpret_cast<const char *>(aT + (int64_t)m0 * lda + kbase);
+  const char *cA = reinterpret_cast<const char *>(aO + ((int64_t)(m0 / 32 + wave) * (lda / 16) + kbase / 16) * 64);
+  const uint32_t offA = (uint32_t)lane * 16u;
   const char *cX = reinterpret_cast<const char *>(X + (int64_t)kbase * ncols + n0);
-  const uint32_t offA = (uint32_t)(rbase * lda + q8) * 2u;
   const uint32_t offX = (uint32_t)(2 * rp * ncols + 2 * cp) * 8u;
   // triangle reads.  Byte offsets from X (< 2^30: quarter layouts have n <= 28): element (r, c) in
   // place at ((r0 + i) ncols + n0 + q) 8 + offX, transposed at ut(r0, i, q) +
@@ -360,8 +364,10 @@ __device__ __forceinline__ void split_g1_body(_Float16 (&sm)[2][ROWS * LD], int 
   int dq[2];
 #pragma unroll
   for (int q = 0; q < 2; ++q) dq[q] = 2 * rp - (left ? n0 + 2 * cp + q : ncols - 1 - n0 - 2 * cp - q);
-  // Prefetch: the counts (L2 hits: every block reads the same rows of aT) are
-  // loaded one stage ahead into one register slot; p (an HBM stream, too few
+  // Prefetch: the counts (L2 hits: every block reads the same rows of aO) are
+  // loaded straight into the MFMA operand registers, each 16-K chunk of stage
+  // s + 1 as soon as the multiplications of stage s have read that chunk (they
+  // never pass through LDS); p (an HBM stream, too few
   // bytes in flight per CU to cover its latency with one stage) comes through
   // a ring of three slots loaded three stages ahead, and is split into limbs
   // one stage before it is stored, while the previous stage is multiplied (the
@@ -374,16 +380,9 @@ __device__ __forceinline__ void split_g1_body(_Float16 (&sm)[2][ROWS * LD], int 
   half8 ra[4];
   double2 rx[D][2];
   int et[D];
-  auto gload_a = [&](int s) {
-    const uint32_t k2 = (uint32_t)(s * BK * 2);
-    // rows past KLp (padding of the last block) read row KLp - 1 instead: they
-    // only feed MFMA output rows that are never stored, and an unconditional load
-    // keeps the loop body branch-free for the wait counts
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int dr = min(64 * i, KLp - 1 - m0 - rbase);  // row offset from this thread's first row
-      ra[i] = *reinterpret_cast<const half8 *>(cA + (int64_t)dr * lda * 2 + (offA + k2));
-    }
+  // (aO has a zero row for every kappa up to the next multiple of 256)
+  auto aload = [&](int s, int kk) {
+    ra[kk] = *reinterpret_cast<const half8 *>(cA + (offA + (uint32_t)(s * (BK / 16) + kk) * 1024u));
   };
   // where: kGeneral picks each element's source; kInPlace (every element of
   // the stage canonical) reads as without TRI; kTransposed (none canonical)
@@ -478,11 +477,9 @@ __device__ __forceinline__ void split_g1_body(_Float16 (&sm)[2][ROWS * LD], int 
   auto sstore = [&](auto buf) {
     _Float16 *b = sm[decltype(buf)::value];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) *reinterpret_cast<half8 *>(&b[(rbase + 64 * i) * LD + q8]) = ra[i];
-#pragma unroll
     for (int l = 0; l < L; ++l)
 #pragma unroll
-      for (int q = 0; q < 2; ++q) *reinterpret_cast<half2_ *>(&b[(BM + l * BN + 2 * cp + q) * LD + 2 * rp]) = hl[l][q];
+      for (int q = 0; q < 2; ++q) *reinterpret_cast<half2_ *>(&b[(l * BN + 2 * cp + q) * LD + 2 * rp]) = hl[l][q];
   };
   auto opnd = [&](auto buf, int row, int kk) -> half8 {
     return *reinterpret_cast<const half8 *>(&sm[decltype(buf)::value][row * LD + kk * 16 + lk]);
@@ -496,13 +493,14 @@ __device__ __forceinline__ void split_g1_body(_Float16 (&sm)[2][ROWS * LD], int 
 #pragma unroll
   for (int l = 0; l < NA; ++l) acc[l] = (float16_){0};
   const int nst = krows / BK;
-  // Stage s is multiplied from LDS buffer s % 2 while stage s + 1 (its counts
-  // and limbs already in registers) is stored into the other buffer, so one
-  // barrier per stage suffices: the barrier at the end of stage s - 1 both
-  // frees buffer (s + 1) % 2 and publishes stage s.  Stage s: store stage
-  // s + 1; load the counts of stage s + 2 and p of stage s + 4 into slot
-  // `ld`; multiply stage s, splitting p of stage s + 2 (slot `sp`) between the
-  // groups of multiplications; promote every 128 rows (odd s) with the
+  // Stage s is multiplied from LDS buffer s % 2 while the limbs of stage s + 1
+  // (already in registers) are stored into the other buffer, so one barrier
+  // per stage suffices: the barrier at the end of stage s - 1 both frees
+  // buffer (s + 1) % 2 and publishes stage s.  Stage s: store the limbs of
+  // stage s + 1; load p of stage s + 4 into slot `ld`; multiply stage s,
+  // loading each count chunk of stage s + 1 once its multiplications are
+  // issued and splitting p of stage s + 2 (slot `sp`) between the groups of
+  // multiplications; promote every 128 rows (odd s) with the
   // exponent recorded when stage s was split (eh, by stage parity).  Past the
   // end the last stage is loaded again, so the body has no branch (`odd` is a
   // compile-time parity: a stage is one basic block, and the split can be
@@ -514,19 +512,19 @@ __device__ __forceinline__ void split_g1_body(_Float16 (&sm)[2][ROWS * LD], int 
     using Nxt = std::integral_constant<int, 1 - b>;
     const int e = eh[b];
     sstore(Nxt{});
-    gload_a(min(s + 2, nst - 1));
     gload_x(min(s + 4, nst - 1), ld, where);
     double v[2][2];
     split_begin(sp, v, std::integral_constant<bool, decltype(where)::value == kTransposed>{});
     eh[b] = et[decltype(sp)::value];
 #pragma unroll
     for (int kk = 0; kk < BK / 16; ++kk) {
-      const half8 a = opnd(Cur{}, wave * 32 + lr, kk);
+      const half8 a = ra[kk];
 #pragma unroll
       for (int l = 0; l < L; ++l) {
         const int la = l < NA ? l : NA - 1;
-        acc[la] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, opnd(Cur{}, BM + l * BN + lr, kk), acc[la], 0, 0, 0);
+        acc[la] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, opnd(Cur{}, l * BN + lr, kk), acc[la], 0, 0, 0);
       }
+      aload(min(s + 1, nst - 1), kk);
       // a share of the next stage's limbs per group of multiplications; the
       // scheduling barrier keeps it there (left alone, the scheduler sinks the
       // split past the barrier, to the LDS stores that consume it)
@@ -556,13 +554,12 @@ __device__ __forceinline__ void split_g1_body(_Float16 (&sm)[2][ROWS * LD], int 
   static_assert(D == 3, "the unrolled ring below has three slots");
   using E0 = std::false_type;
   using E1 = std::true_type;
-  // prologue: stage 0 into buffer 0; the counts and limbs of stage 1 in
-  // registers; p of stages 2 and 3 in slots 2 and 0
+  // prologue: the limbs of stage 0 into buffer 0; the counts of stage 0 and
+  // the limbs of stage 1 in registers; p of stages 2 and 3 in slots 2 and 0
   using GN0 = std::integral_constant<int, kGeneral>;
   gload_x(0, U0{}, GN0{});
   gload_x(min(1, nst - 1), U1{}, GN0{});
   gload_x(min(2, nst - 1), U2{}, GN0{});
-  gload_a(0);
   {
     double v[2][2];
     split_begin(U0{}, v, std::false_type{});
@@ -571,7 +568,8 @@ __device__ __forceinline__ void split_g1_body(_Float16 (&sm)[2][ROWS * LD], int 
     for (int l = 0; l < L; ++l) split_limb(l, v);
   }
   sstore(U0{});
-  gload_a(min(1, nst - 1));
+#pragma unroll
+  for (int kk = 0; kk < BK / 16; ++kk) aload(0, kk);
   {
     double v[2][2];
     split_begin(U1{}, v, std::false_type{});
@@ -651,10 +649,10 @@ __device__ __forceinline__ void split_g1_body(_Float16 (&sm)[2][ROWS * LD], int 
 }
 
 template <int L, bool TRI, bool MG, int IB>
-__global__ void __launch_bounds__(512) k_split_g1(int KLp, int Kdim, int lda, int ncols, const _Float16 *__restrict__ aT,
+__global__ void __launch_bounds__(512) k_split_g1(int KLp, int Kdim, int lda, int ncols, const half8 *__restrict__ aT,
                                                   const double *__restrict__ X, const int32_t *__restrict__ texp,
                                                   int bits, int ks, double *__restrict__ Tpart, int uniform_loops) {
-  constexpr int LD = 72, ROWS = 256 + L * 32;  // counts, then L limb slabs of 32 rows (x2)
+  constexpr int LD = 72, ROWS = L * 32;  // L limb slabs of 32 rows (x2)
   __shared__ __attribute__((aligned(16))) _Float16 sm[2][ROWS * LD];  // two stage buffers
   if constexpr (TRI) {
     const int n0 = ((int)(blockIdx.x / ks) % (ncols / 32)) * 32;  // (as in the body)
@@ -1184,11 +1182,13 @@ int launch_vw_limbs(const double *V, int KL, int KLp, int N2, const MirrorW &mw,
 int launch_split_g1(int KLp, int Kdim, int lda, int ncols, const _Float16 *aT, const double *X, const int32_t *texp,
                     int limbs, int bits, int ks, double *Tpart, int tri, int merge, int fields, int uniform_loops,
                     hipStream_t s) {
-  if (ks < 1 || Kdim % (ks * 128) || ncols % 64 || lda < Kdim || lda % 8) return BDC_ERR_ARG;
+  if (ks < 1 || Kdim % (ks * 128) || ncols % 64 || lda < Kdim || lda % 16) return BDC_ERR_ARG;
   if (tri && (ncols != 2 * Kdim || ncols < 256)) return BDC_ERR_ARG;
+  if (reinterpret_cast<uintptr_t>(aT) % 16) return BDC_ERR_ARG;
+  const half8 *aO = reinterpret_cast<const half8 *>(aT);
   const int blocks = ks * (ncols / 32) * ((KLp + 255) / 256);
 #define LG4(LL, TT, MM, IB)                                                                                          \
-  hipLaunchKernelGGL((k_split_g1<LL, TT, MM, IB>), dim3(blocks), dim3(512), 0, s, KLp, Kdim, lda, ncols, aT, X, texp, \
+  hipLaunchKernelGGL((k_split_g1<LL, TT, MM, IB>), dim3(blocks), dim3(512), 0, s, KLp, Kdim, lda, ncols, aO, X, texp, \
                      bits, ks, Tpart, uniform_loops)
 #define LG3(LL, TT, MM) LG4(LL, TT, MM, 0)
 #define LG(LL)                                              \
