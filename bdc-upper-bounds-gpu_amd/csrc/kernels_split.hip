@@ -516,13 +516,21 @@ __device__ __forceinline__ void split_g1_body(_Float16 (&sm)[2][ROWS * LD], int 
     double v[2][2];
     split_begin(sp, v, std::integral_constant<bool, decltype(where)::value == kTransposed>{});
     eh[b] = et[decltype(sp)::value];
+    // the limb operands of each group are read from LDS one group ahead
+    half8 bop[2][L];
+#pragma unroll
+    for (int l = 0; l < L; ++l) bop[0][l] = opnd(Cur{}, l * BN + lr, 0);
 #pragma unroll
     for (int kk = 0; kk < BK / 16; ++kk) {
       const half8 a = ra[kk];
+      if (kk + 1 < BK / 16) {
+#pragma unroll
+        for (int l = 0; l < L; ++l) bop[(kk + 1) & 1][l] = opnd(Cur{}, l * BN + lr, kk + 1);
+      }
 #pragma unroll
       for (int l = 0; l < L; ++l) {
         const int la = l < NA ? l : NA - 1;
-        acc[la] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, opnd(Cur{}, l * BN + lr, kk), acc[la], 0, 0, 0);
+        acc[la] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, bop[kk & 1][l], acc[la], 0, 0, 0);
       }
       aload(min(s + 1, nst - 1), kk);
       // a share of the next stage's limbs per group of multiplications; the
