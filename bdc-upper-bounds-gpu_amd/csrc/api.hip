@@ -1728,6 +1728,12 @@ int split_prepare(bdc_ctx *c) {
   // 8 (l / 32) + e), chunk (t, c) at (t N1 / 16 + c) * 512 values, kappa
   // padded with zero rows to a multiple of 256 (the output law's block height)
   std::vector<_Float16> ah(N1 * KLp), aTh((KLp + 255) / 256 * 256 * N1, (_Float16)0.0f), bh(N2 * KRp);
+  // ah: Abar for the cross term in MFMA operand order -- 32 x1 x 16 kappa
+  // chunks of 64 lanes x 8 values (lane l: x1 32 t + l % 32, kappa 16 c +
+  // 8 (l / 32) + e), chunk (t, c) at (t KLp / 16 + c) * 512 values
+  auto aop = [&](size_t x1, size_t kk) {
+    return (((x1 / 32) * (KLp / 16) + kk / 16) * 64 + (x1 % 32) + 32 * ((kk % 16) / 8)) * 8 + kk % 8;
+  };
   auto opidx = [&](size_t kk, size_t x1) {
     return (((kk / 32) * (N1 / 16) + x1 / 16) * 64 + (kk % 32) + 32 * ((x1 % 16) / 8)) * 8 + x1 % 8;
   };
@@ -1739,7 +1745,7 @@ int split_prepare(bdc_ctx *c) {
     for (size_t kk = 0; kk < KLp; ++kk) {
       const uint16_t v = a[x1 * KLp + kk];
       const _Float16 h = round16(v);
-      ah[x1 * KLp + kk] = h;
+      ah[aop(x1, kk)] = h;
       aTh[opidx(kk, x1)] = h;
       const float hf = (float)h;
       rs += std::fabs(hf);

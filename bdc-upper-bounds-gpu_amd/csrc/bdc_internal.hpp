@@ -107,7 +107,8 @@ struct bdc_ctx {
   bdc::DevBuf cnt1[bdc::kMaxHalf + 1], cnt2[bdc::kMaxHalf + 1];
   bdc::DevBuf abar16;   // Abar as u16, N1 x KLp (zero padded)
   bdc::DevBuf bbar16;   // Bbar as u16, N2 x KRp
-  bdc::DevBuf abarh;    // Abar as fp16 (counts rounded to fp16; residual lists below), N1 x KLp
+  bdc::DevBuf abarh;    // Abar as fp16 in MFMA operand order (the cross term; counts rounded to fp16,
+                        // residual lists below), N1 x KLp
   bdc::DevBuf abarTh;   // Abar^T as fp16 in MFMA operand order (the output law), KLp up to 256s x N1
   bdc::DevBuf bbarh;    // Bbar as fp16, N2 x KRp
 
@@ -165,7 +166,7 @@ struct bdc_ctx {
   bdc::Split split;
   bdc::DevBuf ptexp;   // per-tile exponents of P (int32), one per 128 x 64 tile
   bdc::DevBuf vlimb;   // Vbar limbs transposed, fp16 [L][N2][KLp]
-  bdc::DevBuf wlimb;   // Wbar limbs fp16 [L][N1][KRp]
+  bdc::DevBuf wlimb;   // Wbar limbs fp16, L x (rows x KRp) in MFMA operand order (wl_index)
   bdc::DevBuf residA;  // (x1, kappa, r) triples of Abar counts fp16 cannot hold
   bdc::DevBuf rowptrA, entA, rowptrB, entB;  // the same residuals as CSR over x1 (Abar) and x2 (Bbar)
   bdc::DevBuf blkA, blkB;  // flags per 128 rows of Abar / 64 rows of Bbar: any residual there

@@ -222,7 +222,7 @@ int launch_resid_g1(const int32_t *res, int nres, const double *X, int rows, int
                     hipStream_t s);
 struct SplitG2Args {
   int N1, N2, M, KL, KLp, KR, KRp;  // M: rows x1 computed (< N1 in the half-space step)
-  const _Float16 *A16, *vT, *wl, *B16;
+  const _Float16 *A16, *vT, *wl, *B16;  // A16 and wl in MFMA operand order (32-row x 16-K chunks)
   int limbs, bits, epi;
   const uint8_t *blkA, *blkB;
   const int32_t *rowptrA, *entA, *rowptrB, *entB;

@@ -187,19 +187,26 @@ __global__ void __launch_bounds__(256) k_split_v(const double *__restrict__ V, i
     }
   }
 }
-// W (M x KRp) -> wl[l][x1][kappa']
+// The W limbs as the cross term reads them, in MFMA operand order: limb l
+// (len = M KRp values each), row x1, column c in 1 KiB chunks of 32 rows x 16
+// columns, chunk (x1 / 32, c / 16) at ((x1 / 32) (KRp / 16) + c / 16) 512,
+// lane (x1 % 32) + 32 ((c % 16) / 8), value c % 8
+__device__ inline int64_t wl_index(int l, int64_t len, int x1, int c, int KRp) {
+  return l * len + ((int64_t)(x1 >> 5) * (KRp >> 4) + (c >> 4)) * 512 + ((x1 & 31) + 32 * ((c & 15) >> 3)) * 8 + (c & 7);
+}
+// W (M x KRp) -> the limbs wl (wl_index)
 template <int L>
 __global__ void k_split_w(const double *__restrict__ W, int64_t len, int KR, int KRp, int bits,
                           const double *__restrict__ scal, _Float16 *__restrict__ wl) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= len) return;
   const int e = (int)scal[kScalVScale];
-  const int c = (int)(i % KRp);
+  const int c = (int)(i % KRp), x1 = (int)(i / KRp);
   double v = c < KR ? ldexp(W[i], -e) : 0.0;
   const double radix = ldexp(1.0, bits);
   for (int l = 0; l < L; ++l) {
     const double li = rint(v);
-    wl[(int64_t)l * len + i] = int_to_half(li);
+    wl[wl_index(l, len, x1, c, KRp)] = int_to_half(li);
     v = (v - li) * radix;
   }
 }
@@ -273,7 +280,7 @@ __global__ void __launch_bounds__(256) k_vw_limbs(const double *__restrict__ V, 
     }
     for (int l = 0; l < L; ++l) {
       const double li = rint(v);
-      wl[(int64_t)l * len + o] = int_to_half(li);
+      wl[wl_index(l, len, (int)x1, kap, KRp)] = int_to_half(li);
       v = (v - li) * radix;
     }
   }
@@ -721,14 +728,17 @@ struct G2Out {
 // tile), 4 waves of 32 x 32: 16 L accumulators per lane leave room for two
 // blocks per CU, so one block's epilogue (FP64 VALU and HBM traffic) overlaps
 // the other's contraction.
-// Every operand is K-contiguous, so a K stage of 64 is one 128-byte line per
-// operand row: each line is fetched by 8 lanes with 16-byte loads, one L2
-// request per line.  A stage is staged through registers into one LDS buffer
-// whose rows are padded to 144 bytes (conflict-free for the 16-byte writes and
-// for the MFMA operand reads); the next stage's loads are in flight while the
-// current one is multiplied.
-//   phase 1 rows: A16 (128), then V limbs (L x 32)
-//   phase 2 rows: W limbs (L x 128), then B16 (32)
+// The operands each wave owns (its 32 rows of A16 and of the W limbs) are
+// kept in MFMA operand order (1 KiB per 32 rows x 16 K chunk) and loaded
+// straight into the operand registers, each chunk of stage s + 1 once the
+// multiplications of stage s have read it.  The operands all four waves share
+// (the V limbs, B16) are K-contiguous: a K stage of 64 is one 128-byte line per
+// row, fetched by 8 lanes with 16-byte loads and staged through registers into
+// one LDS buffer whose rows are padded to 144 bytes (conflict-free for the
+// 16-byte writes and for the MFMA operand reads); the next stage's loads are in
+// flight while the current one is multiplied.
+//   phase 1 (K = kappa): A16 x the V limbs (L x 32 LDS rows)
+//   phase 2 (K = kappa'): the W limbs x B16 (32 LDS rows)
 // The H and P loads of the epilogue are issued before the last stage's
 // multiplications, which hide part of their HBM latency.  Tiles are dealt XCD
 // by XCD in 8 x 8 patches so the operand rows of a patch stay in that XCD's L2.
@@ -736,11 +746,11 @@ struct G2Out {
 // partials and the exponent of the new P tile.
 template <int L>
 struct G2Rows {
-  static constexpr int NP2 = (L + 1) / 2;  // phase-2 sub-stages per K stage: W limbs in pairs
-  static constexpr int P1 = 128 + 32 * L;
-  static constexpr int P2 = 2 * 128 + 32;
-  static constexpr int MAXR = P1 > P2 ? P1 : P2;
-  static constexpr int LD = 72;  // halves per LDS row (64 + 8 pad)
+  static constexpr int P1 = 32 * L;  // phase-1 LDS rows: the V limbs
+  static constexpr int P2 = 32;      // phase-2 LDS rows: Bbar
+  static constexpr int LD = 72;      // halves per LDS row (64 + 8 pad)
+  // (the quarter epilogue stages 128 rows of 33 doubles in the same array)
+  static constexpr int HALVES = P1 * LD > 128 * 33 * 4 ? P1 * LD : 128 * 33 * 4;
 };
 
 // Quarter mode (QTR, the BA step with the columns of P stored bit-reversed, so
@@ -763,8 +773,7 @@ __global__ void __launch_bounds__(256, QTR ? 2 : 3) k_split_g2(int N2, int KLp, 
                                                     G2Resid rs, G2Out go, const int32_t *__restrict__ tiles) {
   using G = G2Rows<L>;
   constexpr int BM = 128, BN = 32, BK = 64, LD = G::LD;
-  constexpr int NR1 = G::P1 / 32, NR2 = G::P2 / 32, NRM = NR1 > NR2 ? NR1 : NR2;  // rows per thread
-  __shared__ __attribute__((aligned(16))) _Float16 sm[G::MAXR * LD];
+  __shared__ __attribute__((aligned(16))) _Float16 sm[G::HALVES];
   __shared__ double red[32];
   // the exp table lives in LDS: a global table could alias the P stores, which
   // would order every lookup behind the previous element's store
@@ -788,49 +797,46 @@ __global__ void __launch_bounds__(256, QTR ? 2 : 3) k_split_g2(int N2, int KLp, 
   // tiles touched by a residual of the rounded counts run in the second launch
   if (!RES && !QTR && (rs.blkA[m0 / 128] || rs.blkB[n0 / 64])) return;
   if (EPI == EPI_BA && tid < 32) etab[tid] = go.exptab[tid];
-  constexpr int NP2 = G::NP2;
-  const int nst1 = KLp / BK, nst2 = KRp / BK, S = nst1 + nst2 * NP2;  // sub-stages
+  const int nst1 = KLp / BK, nst2 = KRp / BK, S = nst1 + nst2;  // K stages: A16 V, then W B16^T
   const int q8 = (tid & 7) * 8, rbase = tid >> 3;  // 16-byte piece of a row, first row of this thread
   const int lr = lane & 31, lk = 8 * (lane >> 5);
 
-  half8 rg[NRM];
-  // thread rows are rbase + 32 i: a wave-uniform base per row block (scalar
-  // registers) plus one 32-bit per-lane byte offset per operand pair (phase 1:
-  // i < 4 A16, then V limb i - 4; phase-2 sub-stage p: i < 8 W limb 2p + i / 4
-  // row block i % 4, then B16)
+  // LDS side: thread rows rbase + 32 i (phase 1: V limb i; phase 2: B16, i = 0),
+  // a wave-uniform base per operand plus one 32-bit per-lane byte offset (the
+  // stage offset goes into it: a loop-invariant base + lane offset would be
+  // hoisted as one 64-bit VGPR pointer per row block)
+  half8 rg[L];
   const uint32_t off1 = (uint32_t)(rbase * KLp + q8) * 2u, off2 = (uint32_t)(rbase * KRp + q8) * 2u;
-  const char *cA = reinterpret_cast<const char *>(A16 + (int64_t)m0 * KLp);
   const char *cV = reinterpret_cast<const char *>(vT + (int64_t)n0 * KLp);
-  const char *cW = reinterpret_cast<const char *>(wl + (int64_t)m0 * KRp);
   const char *cB = reinterpret_cast<const char *>(B16 + (int64_t)n0 * KRp);
-  // (the stage offset goes into the per-lane offset: a loop-invariant base +
-  // lane offset would be hoisted as one 64-bit VGPR pointer per row block)
   auto gload1 = [&](int s) {
     const uint32_t v = off1 + (uint32_t)(s * BK * 2);
 #pragma unroll
-    for (int i = 0; i < NR1; ++i) {
-      const char *u = i < 4 ? cA + (int64_t)(32 * i) * KLp * 2 : cV + (int64_t)(i - 4) * N2 * KLp * 2;
-      rg[i] = *reinterpret_cast<const half8 *>(u + v);
-    }
+    for (int i = 0; i < L; ++i) rg[i] = *reinterpret_cast<const half8 *>(cV + (int64_t)i * N2 * KLp * 2 + v);
   };
-  auto gload2 = [&](int kst, auto pc) {
-    constexpr int p = decltype(pc)::value;
-    const uint32_t v = off2 + (uint32_t)(kst * BK * 2);
-#pragma unroll
-    for (int i = 0; i < NR2; ++i) {
-      const int l = 2 * p + (i >> 2);
-      if (i < 8 && l >= L) continue;  // odd limb count: the last pair has one limb
-      const char *u = i < 8 ? cW + ((int64_t)l * go.M + 32 * (i & 3)) * KRp * 2 : cB;
-      rg[i] = *reinterpret_cast<const half8 *>(u + v);
-    }
-  };
+  auto gload2 = [&](int kst) { rg[0] = *reinterpret_cast<const half8 *>(cB + (off2 + (uint32_t)(kst * BK * 2))); };
   auto sstore = [&](int s) {
-    const int nr = s < nst1 ? NR1 : NR2;
+    const int nr = s < nst1 ? L : 1;
 #pragma unroll
-    for (int i = 0; i < NRM; ++i)
+    for (int i = 0; i < L; ++i)
       if (i < nr) *reinterpret_cast<half8 *>(&sm[(rbase + 32 * i) * LD + q8]) = rg[i];
   };
   auto opnd = [&](int row, int ks) -> half8 { return *reinterpret_cast<const half8 *>(&sm[row * LD + ks * 16 + lk]); };
+  // register side: this wave's 32 rows (phase 1: ra[0][ks] = A16 chunk ks of
+  // the stage; phase 2: ra[l][ks] = W limb l chunk ks)
+  half8 ra[L][4];
+  const char *cAo = reinterpret_cast<const char *>(A16 + (int64_t)(m0 / 32 + wave) * (KLp / 16) * 512);
+  const char *cWo = reinterpret_cast<const char *>(wl + (int64_t)(m0 / 32 + wave) * (KRp / 16) * 512);
+  const int64_t wlimb = (int64_t)go.M * KRp * 2;  // bytes per W limb
+  const uint32_t offo = (uint32_t)lane * 16u;
+  auto aload1 = [&](int s, int ks) {
+    ra[0][ks] = *reinterpret_cast<const half8 *>(cAo + (offo + (uint32_t)(s * (BK / 16) + ks) * 1024u));
+  };
+  auto aload2 = [&](int kst, int ks) {
+#pragma unroll
+    for (int l = 0; l < L; ++l)
+      ra[l][ks] = *reinterpret_cast<const half8 *>(cWo + l * wlimb + (offo + (uint32_t)(kst * (BK / 16) + ks) * 1024u));
+  };
 
   float16_ acc[L];
 #pragma unroll
@@ -845,27 +851,26 @@ __global__ void __launch_bounds__(256, QTR ? 2 : 3) k_split_g2(int N2, int KLp, 
   double *Pu = EPI == EPI_BA ? ea.P + ubase : ea.D + ubase;
   double hv[16], pv[16];
 
-  gload1(0);
-  auto mult1 = [&]() {
+  // nxt: which stage's register operands to load behind the multiplications
+  // (0: none, 1: phase-1 stage s + 1, 2: phase-2 stage 0 / stage kst + 1)
+  auto mult1 = [&](int s, auto nxt) {
 #pragma unroll
     for (int ks = 0; ks < BK / 16; ++ks) {
-      const half8 a = opnd(wave * 32 + lr, ks);
+      const half8 a = ra[0][ks];
 #pragma unroll
       for (int l = 0; l < L; ++l)
-        acc[l] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, opnd(128 + l * 32 + lr, ks), acc[l], 0, 0, 0);
+        acc[l] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a, opnd(l * 32 + lr, ks), acc[l], 0, 0, 0);
+      if constexpr (decltype(nxt)::value == 1) aload1(s + 1, ks);
+      if constexpr (decltype(nxt)::value == 2) aload2(0, ks);
     }
   };
-  // phase-2 sub-stage p holds W limbs 2p, 2p + 1 (rows 0-255) and B16 (rows 256-287)
-  auto mult2 = [&](auto pc) {
-    constexpr int p = decltype(pc)::value;
+  auto mult2 = [&](int kst, auto nxt) {
 #pragma unroll
     for (int ks = 0; ks < BK / 16; ++ks) {
-      const half8 bb = opnd(256 + lr, ks);
+      const half8 bb = opnd(lr, ks);
 #pragma unroll
-      for (int li = 0; li < 2; ++li)
-        if (2 * p + li < L)
-          acc[2 * p + li] =
-              __builtin_amdgcn_mfma_f32_32x32x16_f16(opnd(li * 128 + wave * 32 + lr, ks), bb, acc[2 * p + li], 0, 0, 0);
+      for (int l = 0; l < L; ++l) acc[l] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ra[l][ks], bb, acc[l], 0, 0, 0);
+      if constexpr (decltype(nxt)::value == 2) aload2(kst + 1, ks);
     }
   };
   // the H and P loads of the epilogue overlap the last stage's multiplications
@@ -877,7 +882,7 @@ __global__ void __launch_bounds__(256, QTR ? 2 : 3) k_split_g2(int N2, int KLp, 
     }
   };
   // P is loaded once the accumulators are being consumed (holding both H and P
-  // through the last stage would exceed the register budget of 3 blocks per CU)
+  // through the last stage would raise the register count)
   auto p_loads = [&]() {
     if constexpr (EPI == EPI_BA) {
 #pragma unroll
@@ -893,52 +898,43 @@ __global__ void __launch_bounds__(256, QTR ? 2 : 3) k_split_g2(int N2, int KLp, 
     __syncthreads();
   };
   // one loop per phase (a phase branch inside one loop makes the compiler copy
-  // every accumulator where the two paths merge); the last sub-stage is peeled
-  // so the epilogue loads can join it
-  using P0 = std::integral_constant<int, 0>;
-  using P1c = std::integral_constant<int, 1>;
-  using PL = std::integral_constant<int, NP2 - 1>;
+  // every accumulator where the two paths merge); the last stage is peeled so
+  // the epilogue loads can join it
+  using N0 = std::integral_constant<int, 0>;
+  using N1 = std::integral_constant<int, 1>;
+  using N2c = std::integral_constant<int, 2>;
+  gload1(0);
+#pragma unroll
+  for (int ks = 0; ks < BK / 16; ++ks) aload1(0, ks);
   if (nst2 > 0) {
 #pragma unroll 1
     for (int s = 0; s + 1 < nst1; ++s) {
       stage_in(s);
       gload1(s + 1);
-      mult1();
+      mult1(s, N1{});
     }
     stage_in(nst1 - 1);
-    gload2(0, P0{});
-    mult1();
+    gload2(0);
+    mult1(nst1 - 1, N2c{});
 #pragma unroll 1
     for (int kst = 0; kst + 1 < nst2; ++kst) {
-      const int u = nst1 + kst * NP2;
-      stage_in(u);
-      if constexpr (NP2 == 2) {
-        gload2(kst, P1c{});
-        mult2(P0{});
-        stage_in(u + 1);
-      }
-      gload2(kst + 1, P0{});
-      mult2(PL{});
+      stage_in(nst1 + kst);
+      gload2(kst + 1);
+      mult2(kst, N2c{});
     }
-    const int u = nst1 + (nst2 - 1) * NP2;
-    stage_in(u);
-    if constexpr (NP2 == 2) {
-      gload2(nst2 - 1, P1c{});
-      mult2(P0{});
-      stage_in(u + 1);
-    }
+    stage_in(S - 1);
     epi_loads();
-    mult2(PL{});
+    mult2(nst2 - 1, N0{});
   } else {
 #pragma unroll 1
     for (int s = 0; s + 1 < S; ++s) {
       stage_in(s);
       gload1(s + 1);
-      mult1();
+      mult1(s, N1{});
     }
     stage_in(S - 1);
     epi_loads();
-    mult1();
+    mult1(S - 1, N0{});
   }
   __syncthreads();  // etab visible
 
