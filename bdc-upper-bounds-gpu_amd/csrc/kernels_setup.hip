@@ -375,6 +375,7 @@ struct EntropyArgs {
   int jlo, jhi;      // split range (backend/BAA.cu:355-357)
   int aLo, aHi;      // stored levels of A (non-identity) and of B
   int bLo, bHi;
+  int gLo;           // B levels [gLo, bHi] are built 8 entries per thread (none when gLo > bHi)
   int aWords, bWords;  // u16 entries of the A and B regions (multiples of 4)
   int groups;        // 4-output groups per quadrant
   uint32_t x1base;
@@ -449,6 +450,76 @@ __device__ inline void build_levels(const uint8_t *Th, const uint8_t *Tl, uint32
 }
 
 typedef unsigned short ushort2_ __attribute__((ext_vector_type(2)));
+// Per-representative copies of x2's half tables in LDS, read by build_b_grouped:
+// thb[2^i - 1 + w] = c(s_hi, w) in both u16 halves, tlf[2^d - 1 + w] = c(s_lo, w),
+// and tlr = c(s_lo, w) in the B row set's level layout (level d at lev_off(d, r),
+// restricted to w's bottom bit when r).
+struct BTables {
+  uint32_t thb[512];
+  uint16_t tlf[512];
+  uint16_t tlr[512];
+};
+__device__ inline void stage_b_tables(const uint8_t *htab, const EntropyArgs &ea, uint32_t x2, int r, uint32_t bit,
+                                      BTables &t) {
+  const int a = (ea.n2 + 1) / 2, bl = ea.n2 / 2;
+  const uint8_t *Th = half_row(htab, ea, a, x2 >> bl), *Tl = half_row(htab, ea, bl, x2 & ((1u << bl) - 1u));
+  const int nh = (2 << a) - 1, nl = (2 << bl) - 1, nr = r ? 1 << bl : 2 << bl;
+  for (int i = threadIdx.x; i < nh; i += blockDim.x) t.thb[i] = Th[i] * 0x10001u;
+  for (int i = threadIdx.x; i < nl; i += blockDim.x) t.tlf[i] = Tl[i];
+  for (int f = threadIdx.x; f < nr; f += blockDim.x) {
+    if (!r && !f) continue;  // the full layout starts at entry 1
+    const int d = r ? (f ? 32 - __clz(f) : 0) : 31 - __clz(f);
+    const uint32_t e = (uint32_t)(f - lev_off(d, r));
+    const uint32_t w = (r && d) ? ((e << 1) | bit) : e;
+    t.tlr[f] = Tl[(1 << d) - 1 + w];
+  }
+}
+// B row-set levels [lo, hi] (lo >= 4 when R, else >= 3: every level holds a
+// multiple of 8 entries at a 16-byte aligned offset), 8 consecutive entries per
+// thread.  With d = m - i low symbols taken from s_lo, the 8 entries share their
+// high part when d >= 4 (R) / 3, and their low parts are 8 consecutive tlr
+// entries: one 16-byte LDS load and four packed u16 multiply-adds per split.
+// Every count of a string of length <= 16 is at most C(16, 8) = 12870 < 2^16, so
+// the packed sums are exact.
+template <int R>
+__device__ inline void build_b_grouped(const BTables &t, int L, int lo, int hi, uint32_t bit, uint16_t *B) {
+  constexpr int D0 = R ? 4 : 3;
+  const int a = (L + 1) / 2, bl = L / 2;
+  const int f0 = lev_off(lo, R), f1 = lev_off(hi, R) + (R ? 1 << (hi - 1) : 1 << hi);
+  for (int f = f0 + 8 * (int)threadIdx.x; f < f1; f += 8 * (int)blockDim.x) {
+    const int m = R ? 32 - __clz(f) : 31 - __clz(f);
+    const uint32_t e0 = (uint32_t)(f - lev_off(m, R));
+    const uint32_t z0 = R ? ((e0 << 1) | bit) : e0;
+    ushort2_ acc[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) acc[q] = ushort2_{0, 0};
+    const int ilo = m - bl > 0 ? m - bl : 0, ihi = m < a ? m : a;
+    for (int i = ilo; i <= ihi; ++i) {
+      const int d = m - i;
+      if (d >= D0) {
+        const ushort2_ th = __builtin_bit_cast(ushort2_, t.thb[(1 << i) - 1 + (z0 >> d)]);
+        const uint4 v = *reinterpret_cast<const uint4 *>(t.tlr + lev_off(d, R) + ((z0 & ((1u << d) - 1u)) >> R));
+        acc[0] = th * __builtin_bit_cast(ushort2_, v.x) + acc[0];
+        acc[1] = th * __builtin_bit_cast(ushort2_, v.y) + acc[1];
+        acc[2] = th * __builtin_bit_cast(ushort2_, v.z) + acc[2];
+        acc[3] = th * __builtin_bit_cast(ushort2_, v.w) + acc[3];
+      } else {
+        for (int j = 0; j < 8; ++j) {
+          const uint32_t z = z0 + ((uint32_t)j << R);
+          const uint32_t p = (t.thb[(1 << i) - 1 + (z >> d)] & 0xffffu) * t.tlf[(1 << d) - 1 + (z & ((1u << d) - 1u))];
+          acc[j >> 1][j & 1] += (unsigned short)p;
+        }
+      }
+    }
+    uint4 o;
+    o.x = __builtin_bit_cast(uint32_t, acc[0]);
+    o.y = __builtin_bit_cast(uint32_t, acc[1]);
+    o.z = __builtin_bit_cast(uint32_t, acc[2]);
+    o.w = __builtin_bit_cast(uint32_t, acc[3]);
+    *reinterpret_cast<uint4 *>(B + f) = o;
+  }
+}
+
 __device__ inline uint32_t half_of(const uint32_t *w, int i) { return (w[i >> 1] >> (16 * (i & 1))) & 0xffffu; }
 
 // Counts of the 16 outputs y_e = ys | (e << R), e < 16 (ys has bits R..R+3
@@ -583,6 +654,16 @@ __global__ void __launch_bounds__(1024) k_entropy_reps(EntropyArgs ea, const uin
   // so a representative costs two barriers.
   uint32_t prev = 0xffffffffu;
   int par = 0;
+  __shared__ __attribute__((aligned(16))) BTables btab[2];  // x2's tables: this representative's and the next's
+  const bool grouped = ea.gLo <= ea.bHi;
+  auto next_rep = [&](uint32_t from) {  // uniform across the block
+    for (uint32_t y = from; y < N2; ++y)
+      if (rep_of((x1 << n2) | y, n) == ((x1 << n2) | y)) return y;
+    return N2;
+  };
+  uint32_t x2 = next_rep(0);
+  int tb = 0;
+  if (grouped && x2 < N2) stage_b_tables(htab, ea, x2, rb, (uint32_t)ea.b, btab[0]);
   auto flush = [&](int p) {
     if (threadIdx.x == 0 && prev != 0xffffffffu) {
       double h = 0.0;
@@ -591,14 +672,19 @@ __global__ void __launch_bounds__(1024) k_entropy_reps(EntropyArgs ea, const uin
       Hrep[r] = ea.accumulate ? Hrep[r] + h : h;
     }
   };
-  for (uint32_t x2 = 0; x2 < N2; ++x2) {
+  while (x2 < N2) {
     const uint32_t x = (x1 << n2) | x2;
-    if (rep_of(x, n) != x) continue;  // uniform across the block
-    __syncthreads();                  // A built / previous sweep done with B; hw[par ^ 1] complete
+    __syncthreads();  // A built / previous sweep done with B; hw[par ^ 1] and btab[tb] complete
     flush(par ^ 1);
     build_levels(half_row(htab, ea, a2, x2 >> b2), half_row(htab, ea, b2, x2 & ((1u << b2) - 1u)), x2, n2, ea.bLo,
-                 ea.bHi, rb, 0, (uint32_t)ea.b, B);
+                 grouped ? ea.gLo - 1 : ea.bHi, rb, 0, (uint32_t)ea.b, B);
+    if (grouped) {
+      if (rb) build_b_grouped<1>(btab[tb], n2, ea.gLo, ea.bHi, (uint32_t)ea.b, B);
+      else build_b_grouped<0>(btab[tb], n2, ea.gLo, ea.bHi, (uint32_t)ea.b, B);
+    }
     __syncthreads();
+    const uint32_t nx2 = next_rep(x2 + 1);
+    if (grouped && nx2 < N2) stage_b_tables(htab, ea, nx2, rb, (uint32_t)ea.b, btab[tb ^ 1]);  // read after the next barrier
     double h = 0.0;
     if (ea.groups >= 4) {  // 16 outputs per thread (k >= 2 rb + 4)
       const int nsg = ea.groups >> 2;
@@ -642,6 +728,8 @@ __global__ void __launch_bounds__(1024) k_entropy_reps(EntropyArgs ea, const uin
     if (lane == 0) hw[par][wave] = h;
     prev = x;
     par ^= 1;
+    x2 = nx2;
+    tb ^= 1;
   }
   __syncthreads();
   flush(par ^ 1);
@@ -672,6 +760,10 @@ int launch_entropy_reps(int n, int k, int n1, int n2, const uint64_t *mask, cons
   if (fq && fq[0] == '1' && k >= 4) r = 1;
   if (r && k < 4) return BDC_ERR_ARG;
   ea.rt = ea.rb = r;
+  // B levels with at least 8 entries are built 8 per thread from LDS copies of
+  // x2's half tables (BDC_ENTROPY_GROUPED=0: every entry on its own, for tests)
+  const char *fg = getenv("BDC_ENTROPY_GROUPED");
+  ea.gLo = (fg && fg[0] == '0') ? ea.bHi + 1 : std::max(ea.bLo, r ? 4 : 3);
   ea.aWords = words(ea.aHi, r);
   ea.bWords = words(ea.bHi, r);
   const size_t lds = 2 * (size_t)(ea.aWords + ea.bWords);

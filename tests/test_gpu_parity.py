@@ -91,6 +91,24 @@ def test_entropy_term_quadrant_layout(bdc, monkeypatch, n, k):
         np.testing.assert_allclose(e.entropy_term(), ref, rtol=0, atol=1e-12)
 
 
+@pytest.mark.parametrize("n,k", [(14, 7), (20, 10), (21, 10), (24, 12), (25, 9)])
+@pytest.mark.parametrize("quadrants", ["0", "1"])
+def test_entropy_grouped_row_sets_bitwise(bdc, monkeypatch, n, k, quadrants):
+    """The x2 row sets built 8 entries per thread with packed u16
+    multiply-adds from staged half tables hold the same integer counts as the
+    entry-by-entry build, so the entropy term is bitwise identical (both
+    layouts; half lengths 10-12 reach the 16-byte split loads)."""
+    say(f"entropy grouped row sets n={n} k={k} quadrants={quadrants}")
+    monkeypatch.setenv("BDC_ENTROPY_QUADRANTS", quadrants)
+    out = {}
+    for flag in ("0", "1"):
+        monkeypatch.setenv("BDC_ENTROPY_GROUPED", flag)
+        with bdc.Engine(n, k) as e:
+            out[flag] = e.entropy_term()
+    assert np.array_equal(out["0"], out["1"])
+    assert np.isfinite(out["1"]).all()
+
+
 # (18, 9) reaches the count-streaming stage kernel (2^6-column count factor)
 @pytest.mark.parametrize("n,k", [(8, 4), (10, 5), (11, 6), (12, 7), (12, 3), (18, 9)])
 def test_output_law_general_weights(bdc, oracle, n, k):
