@@ -627,7 +627,10 @@ __device__ inline double sweep16(const EntropyArgs &ea, const uint16_t *__restri
   return h;
 }
 
-__global__ void __launch_bounds__(1024) k_entropy_reps(EntropyArgs ea, const uint64_t *__restrict__ mask,
+// WAVES_PER_EU 8: two 1024-thread blocks per CU (64 VGPRs) where their LDS fits,
+// 4: one block per CU with up to 128 VGPRs.
+template <int WAVES_PER_EU>
+__global__ void __launch_bounds__(1024, WAVES_PER_EU) k_entropy_reps(EntropyArgs ea, const uint64_t *__restrict__ mask,
                                                       const uint32_t *__restrict__ wordbase,
                                                       const double *__restrict__ logtab,
                                                       const uint8_t *__restrict__ htab, double *__restrict__ Hrep) {
@@ -797,13 +800,19 @@ int launch_entropy_reps(int n, int k, int n1, int n2, const uint64_t *mask, cons
   // progress is reported
   const uint32_t X1 = 1u << (n1 - 1), chunk = (verbose && X1 >= 16) ? X1 / 16 : X1;
   const int quads = r ? 4 : 1;
+  // static LDS: two BTables, the per-wave sums and the log table
+  const size_t kStatic = 2 * sizeof(BTables) + 2 * 16 * sizeof(double) + 130 * sizeof(double);
+  const bool two_blocks = 2 * (lds + kStatic) <= 160 * 1024;
   for (int q = 0; q < quads; ++q) {
     ea.t = q >> 1;
     ea.b = q & 1;
     ea.accumulate = q > 0;
     for (uint32_t b = 0; b < X1; b += chunk) {
       ea.x1base = b;
-      hipLaunchKernelGGL(k_entropy_reps, dim3(chunk), dim3(threads), lds, s, ea, mask, wordbase, logtab, htab, Hrep);
+      if (two_blocks)
+        hipLaunchKernelGGL(k_entropy_reps<8>, dim3(chunk), dim3(threads), lds, s, ea, mask, wordbase, logtab, htab, Hrep);
+      else
+        hipLaunchKernelGGL(k_entropy_reps<4>, dim3(chunk), dim3(threads), lds, s, ea, mask, wordbase, logtab, htab, Hrep);
       if (hipGetLastError() != hipSuccess) {
         (void)hipFree(htab);
         return BDC_ERR_HIP;
