@@ -802,7 +802,8 @@ int launch_entropy_reps(int n, int k, int n1, int n2, const uint64_t *mask, cons
   const int quads = r ? 4 : 1;
   // static LDS: two BTables, the per-wave sums and the log table
   const size_t kStatic = 2 * sizeof(BTables) + 2 * 16 * sizeof(double) + 130 * sizeof(double);
-  const bool two_blocks = 2 * (lds + kStatic) <= 160 * 1024;
+  constexpr size_t kLdsPerCu = 160 * 1024;  // gfx950
+  const bool two_blocks = 2 * (lds + kStatic) <= kLdsPerCu;
   for (int q = 0; q < quads; ++q) {
     ea.t = q >> 1;
     ea.b = q & 1;
