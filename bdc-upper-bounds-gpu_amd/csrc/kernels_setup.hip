@@ -375,7 +375,7 @@ struct EntropyArgs {
   int jlo, jhi;      // split range (backend/BAA.cu:355-357)
   int aLo, aHi;      // stored levels of A (non-identity) and of B
   int bLo, bHi;
-  int gLo;           // B levels [gLo, bHi] are built 16 entries per thread (none when gLo > bHi)
+  int gLo;           // B levels [gLo, bHi] are built 8 entries per thread (none when gLo > bHi)
   int aWords, bWords;  // u16 entries of the A and B regions (multiples of 4)
   int groups;        // 4-output groups per quadrant
   uint32_t x1base;
@@ -474,70 +474,49 @@ __device__ inline void stage_b_tables(const uint8_t *htab, const EntropyArgs &ea
     t.tlr[f] = Tl[(1 << d) - 1 + w];
   }
 }
-// B row-set levels [lo, hi] (lo >= 5 when R, else >= 4: every level holds a
-// multiple of 16 entries at a 32-byte aligned offset), 16 consecutive entries
-// per thread as two halves of 8.  With d = m - i low symbols taken from s_lo,
-// the 8 entries of a half share their high part when d >= 4 (R) / 3, and their
-// low parts are 8 consecutive tlr entries: per split and half one 16-byte LDS
-// load and four packed u16 multiply-adds.  The splits with fewer low symbols
-// (small levels only) take their factors entry by entry, in a loop of their
-// own.  Every count of a string of length <= 16 is at most C(16, 8) = 12870 <
-// 2^16, so the packed sums are exact.
+// B row-set levels [lo, hi] (lo >= 4 when R, else >= 3: every level holds a
+// multiple of 8 entries at a 16-byte aligned offset), 8 consecutive entries per
+// thread.  With d = m - i low symbols taken from s_lo, the 8 entries share their
+// high part when d >= 4 (R) / 3, and their low parts are 8 consecutive tlr
+// entries: one 16-byte LDS load and four packed u16 multiply-adds per split.
+// Every count of a string of length <= 16 is at most C(16, 8) = 12870 < 2^16, so
+// the packed sums are exact.
 template <int R>
 __device__ inline void build_b_grouped(const BTables &t, int L, int lo, int hi, uint32_t bit, uint16_t *B) {
   constexpr int D0 = R ? 4 : 3;
-  constexpr uint32_t HZ = 8u << R;  // z distance between the two halves
   const int a = (L + 1) / 2, bl = L / 2;
   const int f0 = lev_off(lo, R), f1 = lev_off(hi, R) + (R ? 1 << (hi - 1) : 1 << hi);
-  for (int f = f0 + 16 * (int)threadIdx.x; f < f1; f += 16 * (int)blockDim.x) {
+  for (int f = f0 + 8 * (int)threadIdx.x; f < f1; f += 8 * (int)blockDim.x) {
     const int m = R ? 32 - __clz(f) : 31 - __clz(f);
     const uint32_t e0 = (uint32_t)(f - lev_off(m, R));
     const uint32_t z0 = R ? ((e0 << 1) | bit) : e0;
-    ushort2_ acc[8];
+    ushort2_ acc[4];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) acc[q] = ushort2_{0, 0};
+    for (int q = 0; q < 4; ++q) acc[q] = ushort2_{0, 0};
     const int ilo = m - bl > 0 ? m - bl : 0, ihi = m < a ? m : a;
-    auto mad8 = [&](int h, ushort2_ s, const uint16_t *tl) {
-      const uint4 v = *reinterpret_cast<const uint4 *>(tl);
-      acc[4 * h + 0] = s * __builtin_bit_cast(ushort2_, v.x) + acc[4 * h + 0];
-      acc[4 * h + 1] = s * __builtin_bit_cast(ushort2_, v.y) + acc[4 * h + 1];
-      acc[4 * h + 2] = s * __builtin_bit_cast(ushort2_, v.z) + acc[4 * h + 2];
-      acc[4 * h + 3] = s * __builtin_bit_cast(ushort2_, v.w) + acc[4 * h + 3];
-    };
-    // d >= D0 + 1: one high part for all 16 entries, 16 consecutive tlr entries
-    const int ish = ihi < m - D0 - 1 ? ihi : m - D0 - 1;
-    for (int i = ilo; i <= ish; ++i) {
+    for (int i = ilo; i <= ihi; ++i) {
       const int d = m - i;
-      const ushort2_ s = __builtin_bit_cast(ushort2_, t.thb[(1 << i) - 1 + (z0 >> d)]);
-      const uint16_t *tl = t.tlr + (R ? 1 << (d - 1) : 1 << d) + ((z0 & ((1u << d) - 1u)) >> R);
-      mad8(0, s, tl);
-      mad8(1, s, tl + 8);
-    }
-    // d = D0: each half has its own high part; both read the same 8 tlr entries
-    const int i2 = m - D0;
-    if (i2 >= ilo && i2 <= ihi) {
-      const uint16_t *tl = t.tlr + (R ? 1 << (D0 - 1) : 1 << D0) + ((z0 & ((1u << D0) - 1u)) >> R);
-#pragma unroll
-      for (int h = 0; h < 2; ++h)
-        mad8(h, __builtin_bit_cast(ushort2_, t.thb[(1 << i2) - 1 + ((z0 + h * HZ) >> D0)]), tl);
-    }
-    for (int i = i2 + 1 > ilo ? i2 + 1 : ilo; i <= ihi; ++i) {
-      const int d = m - i;
-      for (int j = 0; j < 16; ++j) {
-        const uint32_t z = z0 + ((uint32_t)j << R);
-        const uint32_t p = (t.thb[(1 << i) - 1 + (z >> d)] & 0xffffu) * t.tlf[(1 << d) - 1 + (z & ((1u << d) - 1u))];
-        acc[j >> 1][j & 1] += (unsigned short)p;
+      if (d >= D0) {
+        const ushort2_ th = __builtin_bit_cast(ushort2_, t.thb[(1 << i) - 1 + (z0 >> d)]);
+        const uint4 v = *reinterpret_cast<const uint4 *>(t.tlr + lev_off(d, R) + ((z0 & ((1u << d) - 1u)) >> R));
+        acc[0] = th * __builtin_bit_cast(ushort2_, v.x) + acc[0];
+        acc[1] = th * __builtin_bit_cast(ushort2_, v.y) + acc[1];
+        acc[2] = th * __builtin_bit_cast(ushort2_, v.z) + acc[2];
+        acc[3] = th * __builtin_bit_cast(ushort2_, v.w) + acc[3];
+      } else {
+        for (int j = 0; j < 8; ++j) {
+          const uint32_t z = z0 + ((uint32_t)j << R);
+          const uint32_t p = (t.thb[(1 << i) - 1 + (z >> d)] & 0xffffu) * t.tlf[(1 << d) - 1 + (z & ((1u << d) - 1u))];
+          acc[j >> 1][j & 1] += (unsigned short)p;
+        }
       }
     }
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      uint4 o;
-      o.x = __builtin_bit_cast(uint32_t, acc[4 * h + 0]);
-      o.y = __builtin_bit_cast(uint32_t, acc[4 * h + 1]);
-      o.z = __builtin_bit_cast(uint32_t, acc[4 * h + 2]);
-      o.w = __builtin_bit_cast(uint32_t, acc[4 * h + 3]);
-      *reinterpret_cast<uint4 *>(B + f + 8 * h) = o;
-    }
+    uint4 o;
+    o.x = __builtin_bit_cast(uint32_t, acc[0]);
+    o.y = __builtin_bit_cast(uint32_t, acc[1]);
+    o.z = __builtin_bit_cast(uint32_t, acc[2]);
+    o.w = __builtin_bit_cast(uint32_t, acc[3]);
+    *reinterpret_cast<uint4 *>(B + f) = o;
   }
 }
 
@@ -784,10 +763,10 @@ int launch_entropy_reps(int n, int k, int n1, int n2, const uint64_t *mask, cons
   if (fq && fq[0] == '1' && k >= 4) r = 1;
   if (r && k < 4) return BDC_ERR_ARG;
   ea.rt = ea.rb = r;
-  // B levels with at least 16 entries are built 16 per thread from LDS copies of
+  // B levels with at least 8 entries are built 8 per thread from LDS copies of
   // x2's half tables (BDC_ENTROPY_GROUPED=0: every entry on its own, for tests)
   const char *fg = getenv("BDC_ENTROPY_GROUPED");
-  ea.gLo = (fg && fg[0] == '0') ? ea.bHi + 1 : std::max(ea.bLo, r ? 5 : 4);
+  ea.gLo = (fg && fg[0] == '0') ? ea.bHi + 1 : std::max(ea.bLo, r ? 4 : 3);
   ea.aWords = words(ea.aHi, r);
   ea.bWords = words(ea.bHi, r);
   const size_t lds = 2 * (size_t)(ea.aWords + ea.bWords);

@@ -1,7 +1,7 @@
 """CPU check of the entropy kernel's grouped x2 row-set build (k_entropy_reps,
 build_b_grouped over the tables stage_b_tables copies into LDS): for the full
 layout (r = 0) and the quadrant layout (r = 1, entries restricted to z's
-bottom bit), the index arithmetic of every group of 16 entries is evaluated as
+bottom bit), the index arithmetic of every group of 8 entries is evaluated as
 the kernel forms it, and must
 
 - write every entry of the levels it builds exactly once, 16-byte aligned;
@@ -38,8 +38,7 @@ def half_table(h, l):
 
 def check(n2, x2, r, bit, lo, hi):
     a, bl = (n2 + 1) // 2, n2 // 2
-    D0, M0 = (4, 5) if r else (3, 4)  # 8 entries share their high part / levels of >= 16 entries
-    HZ = 8 << r  # z distance between a group's two halves of 8 entries
+    D0, M0 = (4, 4) if r else (3, 3)
     assert lo >= M0
     thb = half_table(x2 >> bl, a)  # the low u16 half of the staged u32
     tl = half_table(x2 & ((1 << bl) - 1), bl)
@@ -55,41 +54,29 @@ def check(n2, x2, r, bit, lo, hi):
     f0, f1 = lev_off(lo, r), lev_off(hi, r) + ((1 << (hi - 1)) if r else 1 << hi)
     written = np.zeros(f1, dtype=np.int64)
     B = np.zeros(f1, dtype=np.int64)
-    for f in range(f0, f1, 16):
-        assert f % 16 == 0  # two 16-byte stores
+    for f in range(f0, f1, 8):
+        assert f % 8 == 0  # 16-byte store
         m = f.bit_length() if r else f.bit_length() - 1
         e0 = f - lev_off(m, r)
         z0 = ((e0 << 1) | bit) if r else e0
-        acc = np.zeros(16, dtype=np.int64)
-        ilo, ihi = max(0, m - bl), min(m, a)
-        for i in range(ilo, min(ihi, m - D0 - 1) + 1):  # d > D0: one high part, 16 tlr entries
+        acc = np.zeros(8, dtype=np.int64)
+        for i in range(max(0, m - bl), min(m, a) + 1):
             d = m - i
-            ih = (1 << i) - 1 + (z0 >> d)
-            assert ih < (2 << a) - 1
-            assert all((z0 + (j << r)) >> d == z0 >> d for j in range(16))
-            il = lev_off(d, r) + ((z0 & ((1 << d) - 1)) >> r)
-            assert il % 8 == 0 and il + 16 <= nr  # two 16-byte LDS loads
-            acc += thb[ih] * tlr[il:il + 16]
-            assert acc.max() < 1 << 16  # packed u16 sums stay exact
-        i2 = m - D0
-        if ilo <= i2 <= ihi:  # d = D0: a high part per half of 8, the same 8 tlr entries
-            il = lev_off(D0, r) + ((z0 & ((1 << D0) - 1)) >> r)
-            assert il % 8 == 0 and il + 8 <= nr
-            for h in range(2):
-                ih = (1 << i2) - 1 + ((z0 + h * HZ) >> D0)
+            if d >= D0:
+                ih = (1 << i) - 1 + (z0 >> d)
                 assert ih < (2 << a) - 1
-                acc[8 * h:8 * h + 8] += thb[ih] * tlr[il:il + 8]
-            assert acc.max() < 1 << 16
-        for i in range(max(i2 + 1, ilo), ihi + 1):  # fewer low symbols: entry by entry
-            d = m - i
-            for j in range(16):
-                z = z0 + (j << r)
-                ih, il = (1 << i) - 1 + (z >> d), (1 << d) - 1 + (z & ((1 << d) - 1))
-                assert ih < (2 << a) - 1 and il < (2 << bl) - 1
-                acc[j] += thb[ih] * tl[il]
-            assert acc.max() < 1 << 16
-        B[f:f + 16] = acc
-        written[f:f + 16] += 1
+                il = lev_off(d, r) + ((z0 & ((1 << d) - 1)) >> r)
+                assert il % 8 == 0 and il + 8 <= nr  # one 16-byte LDS load
+                acc += thb[ih] * tlr[il:il + 8]
+            else:
+                for j in range(8):
+                    z = z0 + (j << r)
+                    ih, il = (1 << i) - 1 + (z >> d), (1 << d) - 1 + (z & ((1 << d) - 1))
+                    assert ih < (2 << a) - 1 and il < (2 << bl) - 1
+                    acc[j] += thb[ih] * tl[il]
+            assert acc.max() < 1 << 16  # packed u16 sums stay exact
+        B[f:f + 8] = acc
+        written[f:f + 8] += 1
     assert np.all(written[f0:f1] == 1)
     for m in range(lo, hi + 1):
         ref = counts_all(x2, n2, m)
@@ -103,9 +90,9 @@ def check(n2, x2, r, bit, lo, hi):
 def test_grouped_row_sets_match_subsequence_counts(n2, r):
     rng = np.random.default_rng(n2 * 2 + r)
     hi = n2 - 1
-    lo = 5 if r else 4
+    lo = 4 if r else 3
     if hi < lo:
-        pytest.skip("no level of 16 or more entries")
+        pytest.skip("no level of 8 or more entries")
     xs = [0, (1 << n2) - 1, int("10" * (n2 // 2) + "1" * (n2 % 2), 2)] + [int(v) for v in rng.integers(0, 1 << n2, 2)]
     for x2 in xs:
         for bit in ((0, 1) if r else (0,)):
